@@ -1,0 +1,159 @@
+/*
+ * spgo.h -- C ABI of the MI355X (gfx950) stereo bundle-adjustment LM step.
+ *
+ * One opaque solver handle replaces the reference's C++ facade `struct Solver`
+ * (include/Solver.hpp:18-85) and its Cython wrapper `CudaSolver`
+ * (src/CythonAPI/Solver.pyx:16-107).  Every entry point returns an int status:
+ * 0 on success, a negative SPGO_E* code on failure, with a message available
+ * from spgo_last_error().  No torch, HIP or C++ types cross this boundary.
+ *
+ * Conventions (as in the reference):
+ *   N incremental poses X_k = T_curr_to_next (N+1 frames, frame 0 fixed at I),
+ *   N*M landmark slots holding inverse depths, 4x4 poses row-major float32,
+ *   intrinsics packed [fxL, fyL, cxL, cyL, fxR, fyR, cxR, cyR],
+ *   T_r_to_l = pose of the right camera in the left camera frame.
+ */
+#ifndef SPGO_H
+#define SPGO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPGO_OK 0
+#define SPGO_E_ARG -1        /* invalid argument or id out of range           */
+#define SPGO_E_CAPACITY -2   /* more measurements than N(N+1)M                 */
+#define SPGO_E_LAYOUT -3     /* problem structure this build does not support  */
+#define SPGO_E_DEVICE -4     /* no gfx950 device / HIP runtime error           */
+#define SPGO_E_STATE -5      /* call not valid in the solver's current state   */
+
+/* Robust kernels.  CAUCHY with param c^2 = 9 and the x4 near-frame boost is
+ * the reference weighting (Solver_cuda.cu:215-228, include/LMutils.h:48). */
+#define SPGO_ROBUST_CAUCHY 0
+#define SPGO_ROBUST_HUBER 1
+#define SPGO_ROBUST_NONE 2
+
+/* REFERENCE: lambda = 0, every step accepted (Solver_cuda.cu:594-782).
+ * LM: lambda on the reduced pose diagonal, cost-based accept/reject. */
+#define SPGO_MODE_REFERENCE 0
+#define SPGO_MODE_LM 1
+
+/* FP64: every phase in double.  MIXED: residual/Jacobian, assembly and the
+ * landmark Schur contraction in float, the reduced pose system factored with
+ * double diagonal blocks and refined in double. */
+#define SPGO_PREC_FP64 0
+#define SPGO_PREC_MIXED 1
+
+typedef struct spgo_options {
+    int32_t mode;            /* SPGO_MODE_*                                    */
+    int32_t robust;          /* SPGO_ROBUST_*                                  */
+    int32_t near_boost;      /* 1: weight x4 when |target - anchor| < 2        */
+    int32_t precision;       /* SPGO_PREC_*                                    */
+    double robust_param;     /* c^2 (Cauchy) or delta in px (Huber)            */
+    double eps;              /* landmark diagonal regulariser (LMutils.h:50)   */
+    double lambda0;          /* initial LM damping                             */
+    double lam_up, lam_down, lam_min, lam_max;
+    double alpha_min, alpha_max;  /* inverse-depth clamp (LMutils.h:54-55)     */
+    int32_t refine_steps;    /* iterative-refinement steps (MIXED)             */
+    int32_t use_graph;       /* replay each LM iteration as a captured graph   */
+    int32_t chunk_landmarks; /* tile shape: landmarks per LDS chunk (<= 64)    */
+    int32_t group_chunks;    /* tile shape: chunks per workgroup               */
+    int32_t reserved[8];
+} spgo_options;
+
+typedef struct spgo_iter_stats {
+    double cost_candidate;   /* robust cost of the candidate state             */
+    double cost;             /* cost of the state kept after accept/reject     */
+    double lambda;           /* damping used by this iteration's solve         */
+    int32_t accepted;
+    int32_t chol_ok;         /* 0 if the reduced system was not positive definite */
+} spgo_iter_stats;
+
+typedef struct spgo_layout_info {
+    int32_t n_poses, n_slots, n_measurements, n_landmarks;
+    int32_t n_groups, n_chunks, max_span, n_anchors_used;
+} spgo_layout_info;
+
+typedef struct spgo_solver spgo_solver;
+
+/* Defaults: LM mode, Cauchy c^2 = 9 with boost, FP64, lambda0 = 1e-4. */
+int spgo_default_options(spgo_options* out);
+
+/* Solver(int N, int M) (include/Solver.hpp:20-57; Solver.pyx:19-20).  Device
+ * memory is taken on the first spgo_step, so every host-side call works
+ * without a GPU. */
+int spgo_create(int32_t n_poses, int32_t m_per_frame, const spgo_options* opts, spgo_solver** out);
+int spgo_destroy(spgo_solver* h);
+int spgo_set_options(spgo_solver* h, const spgo_options* opts);
+int spgo_get_options(spgo_solver* h, spgo_options* out);
+
+/* Solver::loadCalibration (Solver.cpp:29-38; Solver.pyx:28-32). */
+int spgo_load_calibration(spgo_solver* h, const float intrinsics[8], const float T_r_to_l[16]);
+/* Solver::getCalibration (Solver.cpp:101-109; Solver.pyx:75-88). */
+int spgo_get_calibration(spgo_solver* h, float intrinsics[8], float T_r_to_l[16]);
+
+/* Solver::writeObservations (Solver.cpp:40-56; Solver.pyx:34-38): writes the
+ * (target, feat) cell and appends one measurement. */
+int spgo_write_observation(spgo_solver* h, int32_t anchor, int32_t target, int32_t feat,
+                           const float left[3], const float right[3]);
+/* Bulk form of the same: count measurements, uvLR = count x [uL, vL, uR, vR]
+ * (homogeneous third coordinate 1).  No reference counterpart: replaces
+ * count calls of writeObservations. */
+int spgo_write_observations_bulk(spgo_solver* h, int32_t count, const int32_t* anchor,
+                                 const int32_t* target, const int32_t* feat, const float* uvLR);
+/* Solver::getObservation (Solver.cpp:86-98; Solver.pyx:59-72). */
+int spgo_get_observation(spgo_solver* h, int32_t frame, int32_t feat, float left[3], float right[3]);
+
+/* Solver::step -> updateState (Solver.cpp:5-27; Solver_cuda.cu:557-795). */
+int spgo_step(spgo_solver* h, int32_t iterations);
+
+/* Solver::getIncrementalPose (Solver.cpp:59-67; Solver.pyx:41-52). */
+int spgo_get_incremental_pose(spgo_solver* h, int32_t k, float T[16]);
+/* Solver::writeIncrementalPose, rows 0-2 only (Solver.cpp:69-84). */
+int spgo_write_incremental_pose(spgo_solver* h, int32_t k, const float T[16]);
+/* Solver::loadInverseDepths / getInverseDepths (Solver.cpp:112-128), N*M values. */
+int spgo_load_inverse_depths(spgo_solver* h, const float* alpha);
+int spgo_get_inverse_depths(spgo_solver* h, float* alpha);
+/* CudaSolver.setStepSize (Solver.pyx:106-107): scales the pose update only. */
+int spgo_set_step_size(spgo_solver* h, float step_size);
+
+/* Double-precision state access: poses N x 16 row-major, alpha N*M.  No
+ * reference counterpart (the reference state is float32). */
+int spgo_get_state_f64(spgo_solver* h, double* poses, double* alpha);
+int spgo_set_state_f64(spgo_solver* h, const double* poses, const double* alpha);
+
+/* Per-iteration record of the last spgo_step: up to max_records entries,
+ * n_out set to the number of iterations run, initial_cost to the cost of the
+ * state the step started from. */
+int spgo_get_stats(spgo_solver* h, spgo_iter_stats* out, int32_t max_records, int32_t* n_out,
+                   double* initial_cost);
+/* Current LM damping (persists across spgo_step calls; reset to lambda0 when
+ * the state is written). */
+int spgo_get_lambda(spgo_solver* h, double* lambda);
+int spgo_set_lambda(spgo_solver* h, double lambda);
+
+/* Linearise the current state and return the reduced pose system
+ * H = A - B diag(1/(C+eps)) B^T + lambda I (P x P row-major, P = 6N, both
+ * triangles filled), its right-hand side g = g_T - B (g_alpha/(C+eps)) and the
+ * robust cost.  The products of Solver_cuda.cu:636-730, exposed for
+ * verification; no reference entry point returns them (the declared
+ * get_H_and_g_* of include/Solver.hpp:68-69 are never defined). */
+int spgo_linearize(spgo_solver* h, double lambda, double* H, double* g, double* cost);
+
+/* Host-side layout summary of the measurements written so far (builds the
+ * layout without a GPU). */
+int spgo_problem_info(spgo_solver* h, spgo_layout_info* out);
+
+/* Elapsed device time (ms) of the iterations of the last spgo_step, measured
+ * with events on the solver's stream. */
+int spgo_last_step_ms(spgo_solver* h, double* ms);
+
+/* Message of the last failed call on h (h == NULL: last failed spgo_create). */
+const char* spgo_last_error(spgo_solver* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPGO_H */

@@ -1,0 +1,189 @@
+// State update, global pose chain and the LM accept/reject decision.
+//
+// k_pose_update: X_k <- X_k Exp(s delta_k) (Solver_cuda.cu:511-536) and the
+// global chain G_0 = I, G_{k+1} = G_k X_k^-1, G^R = G T_r_to_l
+// (Solver_cuda.cu:440-493), computed as a parallel prefix product instead of
+// the reference's single-thread loop: every thread multiplies its chunk of
+// poses, a Hillis-Steele scan over the chunk products runs in LDS, and each
+// thread re-walks its chunk from its exclusive prefix.
+// k_accept: robust cost of the candidate (sum of the group partials in a fixed
+// order), accept if it decreased, lambda update, per-iteration record.
+#include "spgo_device.h"
+#include "spgo_kernels.h"
+
+namespace spgo {
+
+template <typename T>
+__global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, StateBuf<T> B0, StateBuf<T> B1,
+                                                    const Control* ctl, const double* delta, double step,
+                                                    int init) {
+    __shared__ double scan[2][WG][12];
+    const int tid = threadIdx.x;
+    const int N = lay.N;
+    const int cur = ctl->cur;
+    const StateBuf<T>& bin = cur ? B1 : B0;
+    const StateBuf<T>& bout = init ? bin : (cur ? B0 : B1);
+    const bool ok = init ? false : (ctl->chol_ok != 0);
+
+    const int chunk = (N + WG - 1) / WG;
+    const int k0 = min(N, tid * chunk), k1 = min(N, k0 + chunk);
+    double P[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    for (int k = k0; k < k1; ++k) {
+        double X[12], Xi[12], tmp[12];
+        pose_load(bin.X + 12 * k, X);
+        if (!init) {
+            double xi[6], E[12];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) xi[q] = ok ? step * delta[6 * k + q] : 0.0;
+            exp_se3(xi, E);
+            pose_mul(X, E, tmp);
+#pragma unroll
+            for (int q = 0; q < 12; ++q) {
+                X[q] = tmp[q];
+                bout.X[12 * k + q] = tmp[q];
+            }
+        }
+        pose_inv(X, Xi);
+        pose_mul(P, Xi, tmp);
+#pragma unroll
+        for (int q = 0; q < 12; ++q) P[q] = tmp[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 12; ++q) scan[0][tid][q] = P[q];
+    __syncthreads();
+    int src = 0;
+    for (int d = 1; d < WG; d <<= 1) {
+        double out[12];
+        if (tid >= d) {
+            pose_mul(scan[src][tid - d], scan[src][tid], out);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 12; ++q) out[q] = scan[src][tid][q];
+        }
+#pragma unroll
+        for (int q = 0; q < 12; ++q) scan[src ^ 1][tid][q] = out[q];
+        src ^= 1;
+        __syncthreads();
+    }
+    // exclusive prefix = inclusive scan of the previous thread
+    double G[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    if (tid > 0) {
+#pragma unroll
+        for (int q = 0; q < 12; ++q) G[q] = scan[src][tid - 1][q];
+    }
+    double Trl[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) Trl[q] = cam.Trl[q];
+    auto emit = [&](int f, const double* Gf) {
+        double Gi[12], GR[12], GRi[12];
+        pose_inv(Gf, Gi);
+        pose_mul(Gf, Trl, GR);
+        pose_inv(GR, GRi);
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+            bout.G[12 * f + q] = Gf[q];
+            bout.GinvL[12 * f + q] = Gi[q];
+            bout.GinvR[12 * f + q] = GRi[q];
+        }
+    };
+    if (tid == 0) emit(0, G);
+    for (int k = k0; k < k1; ++k) {
+        double X[12], Xi[12], tmp[12];
+        pose_load(bout.X + 12 * k, X);
+        pose_inv(X, Xi);
+        pose_mul(G, Xi, tmp);
+#pragma unroll
+        for (int q = 0; q < 12; ++q) G[q] = tmp[q];
+        emit(k + 1, G);
+    }
+}
+
+template <typename T>
+void launch_pose_update(const Layout& lay, const Camera& cam, StateBuf<T>* bufs, const Control* ctl,
+                        const double* delta, double step_size, int init, hipStream_t s) {
+    hipLaunchKernelGGL(k_pose_update<T>, dim3(1), dim3(WG), 0, s, lay, cam, bufs[0], bufs[1], ctl, delta,
+                       step_size, init);
+}
+template void launch_pose_update<double>(const Layout&, const Camera&, StateBuf<double>*, const Control*,
+                                         const double*, double, int, hipStream_t);
+template void launch_pose_update<float>(const Layout&, const Camera&, StateBuf<float>*, const Control*,
+                                        const double*, double, int, hipStream_t);
+
+__global__ __launch_bounds__(WG) void k_accept(const double* c0, const double* c1, int n_groups, Control* ctl,
+                                              IterRecord* rec, int reference_mode, double lam_up,
+                                              double lam_down, double lam_min, double lam_max, int init) {
+    __shared__ double red[WG];
+    const int tid = threadIdx.x;
+    const int cur = ctl->cur;
+    const int out = init ? cur : cur ^ 1;
+    const double* cp = out ? c1 : c0;
+    double s = 0.0;
+    for (int g = tid; g < n_groups; g += WG) s += cp[g];
+    red[tid] = s;
+    __syncthreads();
+    for (int w = WG / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    const double cand = red[0];
+    if (init) {
+        ctl->cost = cand;
+        ctl->cost0 = cand;
+        ctl->iter = 0;
+        return;
+    }
+    const int ok = ctl->chol_ok;
+    const double lam_used = ctl->lambda;
+    int accept;
+    if (reference_mode) {
+        accept = 1;
+    } else {
+        accept = (cand < ctl->cost) ? 1 : 0;
+        double l = accept ? lam_used * lam_down : lam_used * lam_up;
+        ctl->lambda = fmin(fmax(l, lam_min), lam_max);
+    }
+    if (accept) {
+        ctl->cur = out;
+        ctl->cost = cand;
+    }
+    IterRecord r;
+    r.cost_candidate = cand;
+    r.cost = ctl->cost;
+    r.lambda = lam_used;
+    r.accepted = accept;
+    r.chol_ok = ok;
+    rec[ctl->iter] = r;
+    ctl->iter = ctl->iter + 1;
+}
+
+void launch_accept(const double* cost_buf0, const double* cost_buf1, int n_groups, Control* ctl, IterRecord* rec,
+                   int reference_mode, double lam_up, double lam_down, double lam_min, double lam_max, int init,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(k_accept, dim3(1), dim3(WG), 0, s, cost_buf0, cost_buf1, n_groups, ctl, rec,
+                       reference_mode, lam_up, lam_down, lam_min, lam_max, init);
+}
+
+__global__ void k_fill(double* p, double v, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_set_diag(double* H, int Pp, int from, double v) {
+    const int i = from + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < Pp) H[(long long)i * Pp + i] = v;
+}
+
+void launch_fill(double* p, double v, long long n, hipStream_t s) {
+    if (n <= 0) return;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
+}
+
+void launch_set_diag(double* H, int Pp, int from, double v, hipStream_t s) {
+    if (from >= Pp) return;
+    hipLaunchKernelGGL(k_set_diag, dim3((Pp - from + 255) / 256), dim3(256), 0, s, H, Pp, from, v);
+}
+
+}  // namespace spgo

@@ -1,0 +1,149 @@
+// Device-side SE(3) helpers and wavefront reductions for gfx950.
+// Poses are 3x4 row-major [R | t] (the top three rows of the reference's
+// row-major 4x4 matrices).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "spgo_internal.h"
+
+namespace spgo {
+
+// Rotation-angle threshold of the reference's small-angle branches
+// (include/lie.h:4).
+constexpr double SMALL_ANGLE = 1e-4;
+
+template <typename T>
+__device__ __forceinline__ void pose_load(const double* __restrict__ p, T* out) {
+#pragma unroll
+    for (int i = 0; i < 12; ++i) out[i] = (T)p[i];
+}
+
+// C = A * B for affine 3x4 poses.
+template <typename T>
+__device__ __forceinline__ void pose_mul(const T* A, const T* B, T* C) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            T s = A[4 * r + 0] * B[c] + A[4 * r + 1] * B[4 + c] + A[4 * r + 2] * B[8 + c];
+            if (c == 3) s += A[4 * r + 3];
+            C[4 * r + c] = s;
+        }
+    }
+}
+
+// Rigid inverse [R^T | -R^T t] (src/LieUtils/utils.cu:38-69).
+template <typename T>
+__device__ __forceinline__ void pose_inv(const T* A, T* B) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) B[4 * r + c] = A[4 * c + r];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        B[4 * r + 3] = -(B[4 * r + 0] * A[3] + B[4 * r + 1] * A[7] + B[4 * r + 2] * A[11]);
+}
+
+// Exp of the twist (psi, rho): R = Rodrigues(psi), t = Jl(psi) rho, with the
+// reference's second-order branch below |psi| = 1e-4
+// (SE3.cu:3-38, SO3.cu:5-32, SO3.cu:82-107).
+__device__ __forceinline__ void exp_se3(const double* xi, double* T) {
+    const double p0 = xi[0], p1 = xi[1], p2 = xi[2];
+    const double th = sqrt(p0 * p0 + p1 * p1 + p2 * p2);
+    double S[9] = {0.0, -p2, p1, p2, 0.0, -p0, -p1, p0, 0.0};
+    double S2[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            S2[3 * r + c] = S[3 * r] * S[c] + S[3 * r + 1] * S[3 + c] + S[3 * r + 2] * S[6 + c];
+    double a, b, c1, c2;
+    if (th > SMALL_ANGLE) {
+        double s, co;
+        sincos(th, &s, &co);
+        a = s / th;
+        b = (1.0 - co) / (th * th);
+        c1 = b;
+        c2 = (th - s) / (th * th * th);
+    } else {
+        a = 1.0;
+        b = 0.5;
+        c1 = 0.5;
+        c2 = 1.0 / 6.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double t = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double id = (r == c) ? 1.0 : 0.0;
+            T[4 * r + c] = id + a * S[3 * r + c] + b * S2[3 * r + c];
+            t += (id + c1 * S[3 * r + c] + c2 * S2[3 * r + c]) * xi[3 + c];
+        }
+        T[4 * r + 3] = t;
+    }
+}
+
+// out = Ad(G) x with Ad = [[R, 0], [[t]x R, R]] acting on (psi, rho).
+__device__ __forceinline__ void adjoint_apply(const double* G, const double* x, double* out) {
+    double Rp[3], Rr[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        Rp[r] = G[4 * r] * x[0] + G[4 * r + 1] * x[1] + G[4 * r + 2] * x[2];
+        Rr[r] = G[4 * r] * x[3] + G[4 * r + 1] * x[4] + G[4 * r + 2] * x[5];
+    }
+    const double t0 = G[3], t1 = G[7], t2 = G[11];
+    out[0] = Rp[0];
+    out[1] = Rp[1];
+    out[2] = Rp[2];
+    out[3] = t1 * Rp[2] - t2 * Rp[1] + Rr[0];
+    out[4] = t2 * Rp[0] - t0 * Rp[2] + Rr[1];
+    out[5] = t0 * Rp[1] - t1 * Rp[0] + Rr[2];
+}
+
+// Entry (r, c) of Ad(G) (6x6).
+__device__ __forceinline__ double adjoint_entry(const double* G, int r, int c) {
+    if (r < 3) return c < 3 ? G[4 * r + c] : 0.0;
+    if (c >= 3) return G[4 * (r - 3) + (c - 3)];
+    // ([t]x R)[r-3][c]
+    const int i = r - 3;
+    const double t0 = G[3], t1 = G[7], t2 = G[11];
+    const double R0 = G[c], R1 = G[4 + c], R2 = G[8 + c];
+    if (i == 0) return t1 * R2 - t2 * R1;
+    if (i == 1) return t2 * R0 - t0 * R2;
+    return t0 * R1 - t1 * R0;
+}
+
+// ---------------------------------------------------------------------------
+// wavefront (64-lane) reductions
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ int wave_min_int(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Index of (p, q), p <= q < 6, in the row-major upper triangle (21 entries).
+__device__ __forceinline__ int sym6(int p, int q) {
+    if (p > q) {
+        const int t = p;
+        p = q;
+        q = t;
+    }
+    return p * 6 - (p * (p - 1)) / 2 + (q - p);
+}
+
+}  // namespace spgo

@@ -1,0 +1,86 @@
+// Internal types shared by the host runtime and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spgo {
+
+// Upper bound on the pose span (max target - anchor) of one landmark group.
+// The per-group dense block is 6*KMAX square.
+constexpr int KMAX = 16;
+constexpr int CHUNK_LM = 64;        // landmarks per LDS chunk (upper bound)
+constexpr int CHUNK_OBS = 512;      // measurements per LDS chunk (upper bound)
+constexpr int NB = 32;              // Cholesky block size
+constexpr int WG = 256;             // threads per workgroup of the wide kernels
+
+// Device control block: which state buffer is current, damping, cost.
+struct Control {
+    int cur;                // index (0/1) of the accepted state buffers
+    int iter;               // records written in this step
+    int chol_ok;            // set by the factorisation of this iteration
+    int pad;
+    double lambda;
+    double cost;            // cost of the current state
+    double cost0;           // cost at the start of the step
+};
+
+struct IterRecord {
+    double cost_candidate, cost, lambda;
+    int accepted, chol_ok;
+};
+
+struct Camera {
+    double fx[2], fy[2], cx[2], cy[2];   // left, right
+    double Trl[12];                       // right camera in left frame, 3x4
+};
+
+struct Robust {
+    int kind;            // 0 cauchy, 1 huber, 2 none
+    int near_boost;
+    double param;        // c^2 or delta
+};
+
+// Problem layout on the device (built by the host, immutable during a step).
+struct Layout {
+    int N, F, L;                  // poses, frames, landmark slots
+    int n_act;                    // active landmarks (with measurements)
+    int n_groups, n_chunks, n_obs, kmax;
+    // measurements, chunk/slot-major order inside each group
+    const int* ob_t;              // target frame
+    const uint32_t* ob_meta;      // bits 0-15: slot-major -> landmark-major position in chunk,
+                                  // bits 16-31: landmark index inside chunk
+    const float4* ob_uv;          // uL vL uR vR
+    // active landmarks, sorted by (anchor, slot id)
+    const int* lm_id;             // landmark slot
+    const float2* lm_ray;         // anchor-left pixel
+    const int2* lm_pos;           // (first landmark-major position in chunk, count)
+    // chunks
+    const int4* chunk;            // (first active landmark, n landmarks, first obs, n obs)
+    // groups
+    const int* grp_anchor;
+    const int* grp_K;             // pose span of the group
+    const int2* grp_chunks;       // [begin, end) chunk range
+    const int* grp_lm_begin;      // first active landmark
+    const long long* grp_v_off;   // offset of the group's v block (6K per landmark)
+    const long long* grp_D_off;   // offset of the group's (6K)^2 block
+    const int* grp_d_off;         // offset of the group's 6K vector
+    const int* anchor_grp;        // CSR over anchor frames, F+1 entries
+};
+
+// One of the two state/linearisation buffer sets (current and candidate).
+template <typename T>
+struct StateBuf {
+    double* X;        // N x 12 incremental poses (3x4 row-major), always double
+    double* alpha;    // L inverse depths
+    double* G;        // F x 12 global left poses
+    double* GinvL;    // F x 12 inverses
+    double* GinvR;    // F x 12 inverses of the right camera poses
+    T* v;             // per landmark 6K_g values (B columns in the global frame)
+    T* Ct;            // per active landmark C + eps
+    T* g;             // per active landmark g_alpha
+    T* D;             // per group (6K)^2 reduced block
+    T* d;             // per group 6K reduced gradient
+    double* cost;     // per group robust cost
+};
+
+}  // namespace spgo

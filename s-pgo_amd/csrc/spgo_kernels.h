@@ -1,0 +1,41 @@
+// Host-side launchers of the gfx950 kernels (defined in k_*.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "spgo_internal.h"
+
+namespace spgo {
+
+struct AsmArgs {
+    Layout lay;
+    Camera cam;
+    Robust rb;
+    double eps, alpha_min, alpha_max;
+    const Control* ctl;
+    int init;                 // 1: linearise the current state in place
+    const double* delta;      // reduced-system solution (candidate mode)
+};
+
+template <typename T>
+void launch_assemble(const AsmArgs& a, const StateBuf<T>* bufs, hipStream_t s);
+
+template <typename T>
+void launch_form_system(const Layout& lay, const StateBuf<T>* bufs, Control* ctl,
+                        double* H, double* g, int Pp, hipStream_t s);
+
+void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t s);
+void launch_chol_solve(const double* H, const double* Linv, const double* g, double* delta, int Pp,
+                       hipStream_t s);
+
+template <typename T>
+void launch_pose_update(const Layout& lay, const Camera& cam, StateBuf<T>* bufs, const Control* ctl,
+                        const double* delta, double step_size, int init, hipStream_t s);
+
+void launch_accept(const double* cost_buf0, const double* cost_buf1, int n_groups, Control* ctl,
+                   IterRecord* rec, int reference_mode, double lam_up, double lam_down, double lam_min,
+                   double lam_max, int init, hipStream_t s);
+
+void launch_fill(double* p, double v, long long n, hipStream_t s);
+void launch_set_diag(double* H, int Pp, int from, double v, hipStream_t s);
+
+}  // namespace spgo

@@ -1,0 +1,952 @@
+// Host runtime behind the C ABI of include/spgo.h.
+//
+// Replaces the reference's C++ facade (include/Solver.hpp:18-85,
+// src/Optimization/Solver.cpp:1-128) and the orchestration of updateState
+// (src/Optimization/Solver_cuda.cu:557-795):
+//   * measurements are kept host-side and laid out for the device once, on the
+//     first spgo_step after they change (grouped by anchor frame, chunked for
+//     LDS, structure-of-arrays);
+//   * device buffers persist across steps (the reference allocates and frees
+//     every buffer on every step, LMutils.cu:5-185);
+//   * one LM iteration is a fixed launch sequence that reads its buffer
+//     selection and damping from a device control block, so it is captured
+//     once in a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/spgo.h"
+#include "spgo_internal.h"
+#include "spgo_kernels.h"
+
+using namespace spgo;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevMem {
+    std::vector<void*> ptrs;
+    template <typename T>
+    T* alloc(size_t n) {
+        void* p = nullptr;
+        if (n == 0) n = 1;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return (T*)p;
+    }
+    void release() {
+        for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+    }
+};
+
+struct HostLayout {
+    std::vector<int> ob_t;
+    std::vector<uint32_t> ob_meta;
+    std::vector<float4> ob_uv;
+    std::vector<int> lm_id;
+    std::vector<float2> lm_ray;
+    std::vector<int2> lm_pos;
+    std::vector<int4> chunk;
+    std::vector<int> grp_anchor, grp_K, grp_lm_begin, grp_d_off;
+    std::vector<int2> grp_chunks;
+    std::vector<long long> grp_v_off, grp_D_off;
+    std::vector<int> anchor_grp;
+    int kmax = 0;
+    long long v_total = 0, D_total = 0, d_total = 0;
+    int anchors_used = 0;
+};
+
+struct EngineBase {
+    virtual ~EngineBase() {}
+};
+
+}  // namespace
+
+struct spgo_solver {
+    int N = 0, M = 0, L = 0, F = 0;
+    long long max_meas = 0;
+    spgo_options opt{};
+    float step_size = 1.0f;
+    float intr[8] = {0};
+    float Trl[16] = {0};
+
+    std::unordered_map<long long, int> cell_of;   // frame * L + feat -> cell
+    std::vector<float> cells;                     // 6 floats per cell
+    std::vector<int> ma, mt, ml;
+
+    std::vector<double> X;        // N x 12
+    std::vector<double> alpha;    // L
+    bool host_newer = true;       // host mirror must be uploaded
+    bool dev_newer = false;       // device state newer than host mirror
+    bool layout_dirty = true;
+    bool lin_valid = false;
+    bool lambda_reset = true;
+    bool graph_dirty = true;
+
+    std::string err;
+
+    // device
+    bool dev_ready = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevMem mem;          // problem + buffers (rebuilt with the layout)
+    HostLayout hl;
+    Layout lay{};
+    int prec = SPGO_PREC_FP64;
+    StateBuf<double> bd[2]{};
+    StateBuf<float> bf[2]{};
+    double *H = nullptr, *gvec = nullptr, *delta = nullptr, *Linv = nullptr;
+    int Pp = 0;
+    Control* ctl = nullptr;
+    IterRecord* rec = nullptr;
+    int rec_cap = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+
+    std::vector<IterRecord> last_rec;
+    double last_cost0 = 0.0;
+    double last_ms = 0.0;
+
+    int fail(int code, const std::string& m) {
+        err = m;
+        return code;
+    }
+};
+
+namespace {
+
+int check_hip(spgo_solver* h, hipError_t e, const char* what) {
+    if (e == hipSuccess) return SPGO_OK;
+    return h->fail(SPGO_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void mat4_to_3x4(const float* T16, double* X12) {
+    for (int i = 0; i < 12; ++i) X12[i] = (double)T16[i];
+}
+
+void destroy_graph(spgo_solver* h) {
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->gexec = nullptr;
+    h->graph = nullptr;
+    h->graph_dirty = true;
+}
+
+// Build the device layout from the measurement list.
+int build_layout(spgo_solver* h, HostLayout& out) {
+    const int O = (int)h->ma.size();
+    const int chunk_lm = std::max(1, std::min(h->opt.chunk_landmarks > 0 ? h->opt.chunk_landmarks : CHUNK_LM, CHUNK_LM));
+    const int group_chunks = std::max(1, h->opt.group_chunks > 0 ? h->opt.group_chunks : 1);
+    HostLayout hl;
+    // measurements of each landmark, targets ascending (stable)
+    std::vector<int> order(O);
+    for (int i = 0; i < O; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (h->ml[x] != h->ml[y]) return h->ml[x] < h->ml[y];
+        return h->mt[x] < h->mt[y];
+    });
+    struct Lm {
+        int id, anchor, begin, count, smax;
+    };
+    std::vector<Lm> lms;
+    for (int i = 0; i < O;) {
+        const int l = h->ml[order[i]];
+        int j = i;
+        const int a = h->ma[order[i]];
+        int smax = 0;
+        while (j < O && h->ml[order[j]] == l) {
+            if (h->ma[order[j]] != a)
+                return h->fail(SPGO_E_LAYOUT, "landmark " + std::to_string(l) +
+                                                  " has measurements with different anchor frames");
+            smax = std::max(smax, h->mt[order[j]] - a);
+            ++j;
+        }
+        if (smax > KMAX)
+            return h->fail(SPGO_E_LAYOUT, "landmark " + std::to_string(l) + " spans " + std::to_string(smax) +
+                                              " poses; this build supports at most " + std::to_string(KMAX));
+        if (j - i > CHUNK_OBS)
+            return h->fail(SPGO_E_LAYOUT, "landmark " + std::to_string(l) + " has more than " +
+                                              std::to_string(CHUNK_OBS) + " measurements");
+        lms.push_back({l, a, i, j - i, smax});
+        i = j;
+    }
+    std::stable_sort(lms.begin(), lms.end(), [](const Lm& x, const Lm& y) {
+        if (x.anchor != y.anchor) return x.anchor < y.anchor;
+        return x.id < y.id;
+    });
+    const int n_act = (int)lms.size();
+    hl.lm_id.resize(n_act);
+    hl.lm_ray.resize(n_act);
+    hl.lm_pos.resize(n_act);
+    hl.ob_t.reserve(O);
+    hl.ob_meta.reserve(O);
+    hl.ob_uv.reserve(O);
+    hl.anchor_grp.assign(h->F + 1, 0);
+
+    std::vector<int> grp_of_anchor_count(h->F, 0);
+    int li = 0;
+    while (li < n_act) {
+        // one group: landmarks of one anchor, up to group_chunks chunks
+        const int a = lms[li].anchor;
+        const int g = (int)hl.grp_anchor.size();
+        hl.grp_anchor.push_back(a);
+        hl.grp_lm_begin.push_back(li);
+        const int chunk_begin = (int)hl.chunk.size();
+        int K = 0;
+        int nch = 0;
+        while (li < n_act && lms[li].anchor == a && nch < group_chunks) {
+            // one chunk
+            const int lm0 = li;
+            int nobs = 0;
+            while (li < n_act && lms[li].anchor == a && li - lm0 < chunk_lm && nobs + lms[li].count <= CHUNK_OBS) {
+                nobs += lms[li].count;
+                ++li;
+            }
+            const int nl = li - lm0;
+            const int ob0 = (int)hl.ob_t.size();
+            // landmark-major positions, then slot-major processing order
+            struct Ob {
+                int slot, j, rank, m, pos;
+            };
+            std::vector<Ob> obs;
+            obs.reserve(nobs);
+            int pos = 0;
+            for (int j = 0; j < nl; ++j) {
+                const Lm& L = lms[lm0 + j];
+                hl.lm_id[lm0 + j] = L.id;
+                hl.lm_pos[lm0 + j] = make_int2(pos, L.count);
+                K = std::max(K, L.smax);
+                for (int r = 0; r < L.count; ++r) {
+                    const int m = order[L.begin + r];
+                    const int slot = std::max(h->mt[m] - a, 0);
+                    obs.push_back({slot, j, r, m, pos});
+                    ++pos;
+                }
+                // ray from the anchor cell's left pixel (Solver_cuda.cu:147); zero if never written
+                auto it = h->cell_of.find((long long)a * h->L + L.id);
+                if (it != h->cell_of.end())
+                    hl.lm_ray[lm0 + j] = make_float2(h->cells[6 * it->second], h->cells[6 * it->second + 1]);
+                else
+                    hl.lm_ray[lm0 + j] = make_float2(0.f, 0.f);
+            }
+            std::stable_sort(obs.begin(), obs.end(), [](const Ob& x, const Ob& y) {
+                if (x.slot != y.slot) return x.slot < y.slot;
+                return x.j < y.j;
+            });
+            for (const Ob& o : obs) {
+                const int m = o.m;
+                hl.ob_t.push_back(h->mt[m]);
+                hl.ob_meta.push_back((uint32_t)o.pos | ((uint32_t)o.j << 16));
+                const int cell = h->cell_of.at((long long)h->mt[m] * h->L + h->ml[m]);
+                const float* c = &h->cells[6 * cell];
+                hl.ob_uv.push_back(make_float4(c[0], c[1], c[3], c[4]));
+            }
+            hl.chunk.push_back(make_int4(lm0, nl, ob0, nobs));
+            ++nch;
+        }
+        hl.grp_chunks.push_back(make_int2(chunk_begin, (int)hl.chunk.size()));
+        hl.grp_K.push_back(K);
+        hl.kmax = std::max(hl.kmax, K);
+        hl.grp_v_off.push_back(hl.v_total);
+        hl.grp_D_off.push_back(hl.D_total);
+        hl.grp_d_off.push_back((int)hl.d_total);
+        const long long nlg = li - hl.grp_lm_begin[g];
+        hl.v_total += nlg * 6 * K;
+        hl.D_total += (long long)(6 * K) * (6 * K);
+        hl.d_total += 6 * K;
+        grp_of_anchor_count[a] += 1;
+    }
+    for (int a = 0; a < h->F; ++a) {
+        hl.anchor_grp[a + 1] = hl.anchor_grp[a] + grp_of_anchor_count[a];
+        if (grp_of_anchor_count[a]) hl.anchors_used++;
+    }
+    hl.kmax = std::max(hl.kmax, 1);
+    out = std::move(hl);
+    return SPGO_OK;
+}
+
+template <typename T>
+int alloc_state(spgo_solver* h, StateBuf<T>* b, const HostLayout& hl, int n_act) {
+    for (int i = 0; i < 2; ++i) {
+        b[i].X = h->mem.alloc<double>((size_t)h->N * 12);
+        b[i].alpha = h->mem.alloc<double>((size_t)h->L);
+        b[i].G = h->mem.alloc<double>((size_t)h->F * 12);
+        b[i].GinvL = h->mem.alloc<double>((size_t)h->F * 12);
+        b[i].GinvR = h->mem.alloc<double>((size_t)h->F * 12);
+        b[i].v = h->mem.alloc<T>((size_t)hl.v_total);
+        b[i].Ct = h->mem.alloc<T>((size_t)n_act);
+        b[i].g = h->mem.alloc<T>((size_t)n_act);
+        b[i].D = h->mem.alloc<T>((size_t)hl.D_total);
+        b[i].d = h->mem.alloc<T>((size_t)hl.d_total);
+        b[i].cost = h->mem.alloc<double>(hl.grp_anchor.size());
+        if (!b[i].X || !b[i].alpha || !b[i].G || !b[i].GinvL || !b[i].GinvR || !b[i].v || !b[i].Ct || !b[i].g ||
+            !b[i].D || !b[i].d || !b[i].cost)
+            return h->fail(SPGO_E_DEVICE, "device allocation failed");
+    }
+    return SPGO_OK;
+}
+
+template <typename V>
+int upload(spgo_solver* h, const std::vector<V>& v, const V** dst) {
+    V* p = h->mem.alloc<V>(v.size());
+    if (!p) return h->fail(SPGO_E_DEVICE, "device allocation failed");
+    if (!v.empty()) {
+        hipError_t e = hipMemcpy(p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return check_hip(h, e, "hipMemcpy");
+    }
+    *dst = p;
+    return SPGO_OK;
+}
+
+int ensure_device(spgo_solver* h) {
+    if (h->dev_ready) return SPGO_OK;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) return h->fail(SPGO_E_DEVICE, "no HIP device available");
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, 0);
+    if (e != hipSuccess) return check_hip(h, e, "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return h->fail(SPGO_E_DEVICE, std::string("device 0 is ") + prop.gcnArchName + ", this library is built for gfx950");
+    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return check_hip(h, e, "hipStreamCreate");
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+        return h->fail(SPGO_E_DEVICE, "hipEventCreate failed");
+    h->dev_ready = true;
+    return SPGO_OK;
+}
+
+int setup_layout(spgo_solver* h) {
+    destroy_graph(h);
+    h->mem.release();
+    h->rec = nullptr;
+    h->rec_cap = 0;
+    HostLayout hl;
+    int rc = build_layout(h, hl);
+    if (rc) return rc;
+    h->hl = std::move(hl);
+    HostLayout& L = h->hl;
+    Layout& lay = h->lay;
+    lay = Layout{};
+    lay.N = h->N;
+    lay.F = h->F;
+    lay.L = h->L;
+    lay.n_act = (int)L.lm_id.size();
+    lay.n_groups = (int)L.grp_anchor.size();
+    lay.n_chunks = (int)L.chunk.size();
+    lay.n_obs = (int)L.ob_t.size();
+    lay.kmax = L.kmax;
+    if ((rc = upload(h, L.ob_t, &lay.ob_t)) || (rc = upload(h, L.ob_meta, &lay.ob_meta)) ||
+        (rc = upload(h, L.ob_uv, &lay.ob_uv)) || (rc = upload(h, L.lm_id, &lay.lm_id)) ||
+        (rc = upload(h, L.lm_ray, &lay.lm_ray)) || (rc = upload(h, L.lm_pos, &lay.lm_pos)) ||
+        (rc = upload(h, L.chunk, &lay.chunk)) || (rc = upload(h, L.grp_anchor, &lay.grp_anchor)) ||
+        (rc = upload(h, L.grp_K, &lay.grp_K)) || (rc = upload(h, L.grp_chunks, &lay.grp_chunks)) ||
+        (rc = upload(h, L.grp_lm_begin, &lay.grp_lm_begin)) || (rc = upload(h, L.grp_v_off, &lay.grp_v_off)) ||
+        (rc = upload(h, L.grp_D_off, &lay.grp_D_off)) || (rc = upload(h, L.grp_d_off, &lay.grp_d_off)) ||
+        (rc = upload(h, L.anchor_grp, &lay.anchor_grp)))
+        return rc;
+    h->prec = h->opt.precision;
+    if (h->prec == SPGO_PREC_MIXED)
+        rc = alloc_state<float>(h, h->bf, L, lay.n_act);
+    else
+        rc = alloc_state<double>(h, h->bd, L, lay.n_act);
+    if (rc) return rc;
+    const int P = 6 * h->N;
+    h->Pp = ((P + NB - 1) / NB) * NB;
+    h->H = h->mem.alloc<double>((size_t)h->Pp * h->Pp);
+    h->gvec = h->mem.alloc<double>(h->Pp);
+    h->delta = h->mem.alloc<double>(h->Pp);
+    h->Linv = h->mem.alloc<double>((size_t)(h->Pp / NB) * NB * NB);
+    h->ctl = h->mem.alloc<Control>(1);
+    if (!h->H || !h->gvec || !h->delta || !h->Linv || !h->ctl)
+        return h->fail(SPGO_E_DEVICE, "device allocation failed");
+    launch_fill(h->H, 0.0, (long long)h->Pp * h->Pp, h->stream);
+    launch_set_diag(h->H, h->Pp, P, 1.0, h->stream);
+    launch_fill(h->gvec, 0.0, h->Pp, h->stream);
+    launch_fill(h->delta, 0.0, h->Pp, h->stream);
+    Control c{};
+    c.cur = 0;
+    c.chol_ok = 1;
+    c.lambda = 0.0;
+    rc = check_hip(h, hipMemcpy(h->ctl, &c, sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");
+    if (rc) return rc;
+    h->layout_dirty = false;
+    h->lin_valid = false;
+    h->host_newer = true;   // state buffers are new
+    h->lambda_reset = true;
+    return SPGO_OK;
+}
+
+Camera make_camera(const spgo_solver* h) {
+    Camera c;
+    for (int k = 0; k < 2; ++k) {
+        c.fx[k] = h->intr[4 * k + 0];
+        c.fy[k] = h->intr[4 * k + 1];
+        c.cx[k] = h->intr[4 * k + 2];
+        c.cy[k] = h->intr[4 * k + 3];
+    }
+    for (int i = 0; i < 12; ++i) c.Trl[i] = h->Trl[i];
+    return c;
+}
+
+AsmArgs make_asm(const spgo_solver* h, int init) {
+    AsmArgs a;
+    a.lay = h->lay;
+    a.cam = make_camera(h);
+    a.rb.kind = h->opt.robust;
+    a.rb.near_boost = h->opt.near_boost;
+    a.rb.param = h->opt.robust_param;
+    a.eps = h->opt.eps;
+    a.alpha_min = h->opt.alpha_min;
+    a.alpha_max = h->opt.alpha_max;
+    a.ctl = h->ctl;
+    a.init = init;
+    a.delta = h->delta;
+    return a;
+}
+
+template <typename T>
+void launch_iteration(spgo_solver* h, StateBuf<T>* b) {
+    hipStream_t s = h->stream;
+    launch_form_system<T>(h->lay, b, h->ctl, h->H, h->gvec, h->Pp, s);
+    launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
+    launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, s);
+    launch_pose_update<T>(h->lay, make_camera(h), b, h->ctl, h->delta, (double)h->step_size, 0, s);
+    launch_assemble<T>(make_asm(h, 0), b, s);
+    const int n_groups = h->lay.n_groups;
+    launch_accept(b[0].cost, b[1].cost, n_groups, h->ctl, h->rec, h->opt.mode == SPGO_MODE_REFERENCE,
+                  h->opt.lam_up, h->opt.lam_down, h->opt.lam_min, h->opt.lam_max, 0, s);
+}
+
+template <typename T>
+void launch_linearise(spgo_solver* h, StateBuf<T>* b) {
+    hipStream_t s = h->stream;
+    launch_pose_update<T>(h->lay, make_camera(h), b, h->ctl, h->delta, 0.0, 1, s);
+    launch_assemble<T>(make_asm(h, 1), b, s);
+    launch_accept(b[0].cost, b[1].cost, h->lay.n_groups, h->ctl, h->rec, 0, 0, 0, 0, 0, 1, s);
+}
+
+// Copy the device state into the host mirror if the device is newer.
+int pull_state(spgo_solver* h) {
+    if (!h->dev_newer) return SPGO_OK;
+    Control c;
+    int rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (rc) return rc;
+    const double* X = h->prec == SPGO_PREC_MIXED ? h->bf[c.cur].X : h->bd[c.cur].X;
+    const double* al = h->prec == SPGO_PREC_MIXED ? h->bf[c.cur].alpha : h->bd[c.cur].alpha;
+    rc = check_hip(h, hipMemcpy(h->X.data(), X, h->X.size() * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (rc) return rc;
+    rc = check_hip(h, hipMemcpy(h->alpha.data(), al, h->alpha.size() * sizeof(double), hipMemcpyDeviceToHost),
+                   "hipMemcpy");
+    if (rc) return rc;
+    h->dev_newer = false;
+    return SPGO_OK;
+}
+
+// The host mirror is about to be modified.
+int begin_state_write(spgo_solver* h) {
+    int rc = pull_state(h);
+    if (rc) return rc;
+    h->host_newer = true;
+    h->lin_valid = false;
+    h->lambda_reset = true;
+    return SPGO_OK;
+}
+
+int push_state(spgo_solver* h) {
+    Control c;
+    int rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (rc) return rc;
+    for (int i = 0; i < 2; ++i) {
+        double* X = h->prec == SPGO_PREC_MIXED ? h->bf[i].X : h->bd[i].X;
+        double* al = h->prec == SPGO_PREC_MIXED ? h->bf[i].alpha : h->bd[i].alpha;
+        rc = check_hip(h, hipMemcpy(X, h->X.data(), h->X.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        if (rc) return rc;
+        rc = check_hip(h, hipMemcpy(al, h->alpha.data(), h->alpha.size() * sizeof(double), hipMemcpyHostToDevice),
+                       "hipMemcpy");
+        if (rc) return rc;
+    }
+    h->host_newer = false;
+    h->lin_valid = false;
+    return SPGO_OK;
+}
+
+int run_step(spgo_solver* h, int iterations) {
+    int rc = ensure_device(h);
+    if (rc) return rc;
+    if (h->layout_dirty || h->prec != h->opt.precision) {
+        rc = pull_state(h);
+        if (rc) return rc;
+        rc = setup_layout(h);
+        if (rc) return rc;
+    }
+    if (h->host_newer) {
+        rc = push_state(h);
+        if (rc) return rc;
+    }
+    if (iterations > h->rec_cap) {
+        destroy_graph(h);
+        const int cap = std::max(iterations, 64);
+        h->rec = h->mem.alloc<IterRecord>(cap);
+        if (!h->rec) return h->fail(SPGO_E_DEVICE, "device allocation failed");
+        h->rec_cap = cap;
+    }
+    Control c;
+    rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (rc) return rc;
+    if (h->lambda_reset) {
+        c.lambda = h->opt.mode == SPGO_MODE_REFERENCE ? 0.0 : h->opt.lambda0;
+        h->lambda_reset = false;
+    }
+    c.iter = 0;
+    c.chol_ok = 1;
+    rc = check_hip(h, hipMemcpy(h->ctl, &c, sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");
+    if (rc) return rc;
+    const bool mixed = h->prec == SPGO_PREC_MIXED;
+    if (!h->lin_valid) {
+        if (mixed)
+            launch_linearise<float>(h, h->bf);
+        else
+            launch_linearise<double>(h, h->bd);
+        h->lin_valid = true;
+    } else {
+        // keep the linearisation; restart the record and the initial cost
+        Control c2;
+        rc = check_hip(h, hipMemcpy(&c2, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
+        if (rc) return rc;
+        c2.cost0 = c2.cost;
+        c2.iter = 0;
+        rc = check_hip(h, hipMemcpy(h->ctl, &c2, sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");
+        if (rc) return rc;
+    }
+    rc = check_hip(h, hipGetLastError(), "kernel launch");
+    if (rc) return rc;
+    if (h->opt.use_graph && iterations > 0 && h->graph_dirty) {
+        destroy_graph(h);
+        rc = check_hip(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        if (rc) return rc;
+        if (mixed)
+            launch_iteration<float>(h, h->bf);
+        else
+            launch_iteration<double>(h, h->bd);
+        rc = check_hip(h, hipStreamEndCapture(h->stream, &h->graph), "hipStreamEndCapture");
+        if (rc) return rc;
+        rc = check_hip(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        if (rc) return rc;
+        h->graph_dirty = false;
+    }
+    rc = check_hip(h, hipEventRecord(h->ev0, h->stream), "hipEventRecord");
+    if (rc) return rc;
+    for (int it = 0; it < iterations; ++it) {
+        if (h->opt.use_graph) {
+            rc = check_hip(h, hipGraphLaunch(h->gexec, h->stream), "hipGraphLaunch");
+            if (rc) return rc;
+        } else if (mixed) {
+            launch_iteration<float>(h, h->bf);
+        } else {
+            launch_iteration<double>(h, h->bd);
+        }
+    }
+    rc = check_hip(h, hipEventRecord(h->ev1, h->stream), "hipEventRecord");
+    if (rc) return rc;
+    rc = check_hip(h, hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+    if (rc) return rc;
+    rc = check_hip(h, hipGetLastError(), "kernel execution");
+    if (rc) return rc;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
+    h->last_ms = ms;
+    rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (rc) return rc;
+    h->last_rec.resize(c.iter);
+    if (c.iter > 0) {
+        rc = check_hip(h, hipMemcpy(h->last_rec.data(), h->rec, c.iter * sizeof(IterRecord), hipMemcpyDeviceToHost),
+                       "hipMemcpy");
+        if (rc) return rc;
+    }
+    h->last_cost0 = c.cost0;
+    h->dev_newer = true;
+    return SPGO_OK;
+}
+
+bool valid_opts(const spgo_options* o, std::string& why) {
+    if (o->mode != SPGO_MODE_REFERENCE && o->mode != SPGO_MODE_LM) return why = "mode", false;
+    if (o->robust < 0 || o->robust > 2) return why = "robust", false;
+    if (o->precision != SPGO_PREC_FP64 && o->precision != SPGO_PREC_MIXED) return why = "precision", false;
+    if (!(o->eps >= 0.0)) return why = "eps", false;
+    if (o->robust != SPGO_ROBUST_NONE && !(o->robust_param > 0.0)) return why = "robust_param", false;
+    if (!(o->alpha_min > 0.0) || !(o->alpha_max >= o->alpha_min)) return why = "alpha bounds", false;
+    if (o->chunk_landmarks < 0 || o->chunk_landmarks > CHUNK_LM) return why = "chunk_landmarks", false;
+    if (o->group_chunks < 0) return why = "group_chunks", false;
+    return true;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int spgo_default_options(spgo_options* o) {
+    if (!o) return SPGO_E_ARG;
+    std::memset(o, 0, sizeof(*o));
+    o->mode = SPGO_MODE_LM;
+    o->robust = SPGO_ROBUST_CAUCHY;
+    o->near_boost = 1;
+    o->precision = SPGO_PREC_FP64;
+    o->robust_param = 9.0;
+    o->eps = 1e-4;
+    o->lambda0 = 1e-4;
+    o->lam_up = 10.0;
+    o->lam_down = 0.1;
+    o->lam_min = 1e-10;
+    o->lam_max = 1e10;
+    o->alpha_min = 1.0 / 30.0;
+    o->alpha_max = 1.0 / 0.2;
+    o->refine_steps = 3;
+    o->use_graph = 1;
+    o->chunk_landmarks = CHUNK_LM;
+    o->group_chunks = 1;
+    return SPGO_OK;
+}
+
+int spgo_create(int32_t n_poses, int32_t m_per_frame, const spgo_options* opts, spgo_solver** out) {
+    if (!out) return SPGO_E_ARG;
+    *out = nullptr;
+    if (n_poses < 1 || m_per_frame < 1) {
+        g_create_error = "N and M must be positive";
+        return SPGO_E_ARG;
+    }
+    if ((long long)n_poses * m_per_frame > (1ll << 30)) {
+        g_create_error = "N*M too large";
+        return SPGO_E_ARG;
+    }
+    std::unique_ptr<spgo_solver> h(new spgo_solver());
+    h->N = n_poses;
+    h->M = m_per_frame;
+    h->L = n_poses * m_per_frame;
+    h->F = n_poses + 1;
+    h->max_meas = (long long)n_poses * (n_poses + 1) * m_per_frame;
+    if (opts) {
+        std::string why;
+        if (!valid_opts(opts, why)) {
+            g_create_error = "invalid option: " + why;
+            return SPGO_E_ARG;
+        }
+        h->opt = *opts;
+    } else {
+        spgo_default_options(&h->opt);
+    }
+    // identity incremental poses, inverse depths 1.0 (include/Solver.hpp:36-44)
+    h->X.assign((size_t)h->N * 12, 0.0);
+    for (int k = 0; k < h->N; ++k) h->X[12 * k + 0] = h->X[12 * k + 5] = h->X[12 * k + 10] = 1.0;
+    h->alpha.assign(h->L, 1.0);
+    *out = h.release();
+    return SPGO_OK;
+}
+
+int spgo_destroy(spgo_solver* h) {
+    if (!h) return SPGO_E_ARG;
+    destroy_graph(h);
+    h->mem.release();
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return SPGO_OK;
+}
+
+int spgo_set_options(spgo_solver* h, const spgo_options* o) {
+    if (!h || !o) return SPGO_E_ARG;
+    std::string why;
+    if (!valid_opts(o, why)) return h->fail(SPGO_E_ARG, "invalid option: " + why);
+    const bool relayout = o->precision != h->opt.precision || o->chunk_landmarks != h->opt.chunk_landmarks ||
+                          o->group_chunks != h->opt.group_chunks;
+    const bool relin = relayout || o->robust != h->opt.robust || o->robust_param != h->opt.robust_param ||
+                       o->near_boost != h->opt.near_boost || o->eps != h->opt.eps;
+    if (o->mode != h->opt.mode || o->lambda0 != h->opt.lambda0) h->lambda_reset = true;
+    if (relayout) {
+        int rc = pull_state(h);
+        if (rc) return rc;
+        h->layout_dirty = true;
+    }
+    if (relin) h->lin_valid = false;
+    h->opt = *o;
+    destroy_graph(h);
+    return SPGO_OK;
+}
+
+int spgo_get_options(spgo_solver* h, spgo_options* out) {
+    if (!h || !out) return SPGO_E_ARG;
+    *out = h->opt;
+    return SPGO_OK;
+}
+
+int spgo_load_calibration(spgo_solver* h, const float intrinsics[8], const float T_r_to_l[16]) {
+    if (!h || !intrinsics || !T_r_to_l) return SPGO_E_ARG;
+    std::memcpy(h->intr, intrinsics, sizeof(h->intr));
+    std::memcpy(h->Trl, T_r_to_l, sizeof(h->Trl));
+    h->lin_valid = false;
+    destroy_graph(h);
+    return SPGO_OK;
+}
+
+int spgo_get_calibration(spgo_solver* h, float intrinsics[8], float T_r_to_l[16]) {
+    if (!h || !intrinsics || !T_r_to_l) return SPGO_E_ARG;
+    std::memcpy(intrinsics, h->intr, sizeof(h->intr));
+    std::memcpy(T_r_to_l, h->Trl, sizeof(h->Trl));
+    return SPGO_OK;
+}
+
+static int write_one(spgo_solver* h, int a, int t, int l, const float* left, const float* right) {
+    if (a < 0 || a >= h->F || t < 0 || t >= h->F || l < 0 || l >= h->L)
+        return h->fail(SPGO_E_ARG, "frame or landmark id out of range");
+    if ((long long)h->ma.size() >= h->max_meas) return h->fail(SPGO_E_CAPACITY, "measurement capacity N(N+1)M exceeded");
+    const long long key = (long long)t * h->L + l;
+    auto it = h->cell_of.find(key);
+    int cell;
+    if (it == h->cell_of.end()) {
+        cell = (int)(h->cells.size() / 6);
+        h->cells.resize(h->cells.size() + 6);
+        h->cell_of.emplace(key, cell);
+    } else {
+        cell = it->second;
+    }
+    float* c = &h->cells[6 * cell];
+    for (int i = 0; i < 3; ++i) {
+        c[i] = left[i];
+        c[3 + i] = right[i];
+    }
+    h->ma.push_back(a);
+    h->mt.push_back(t);
+    h->ml.push_back(l);
+    return SPGO_OK;
+}
+
+int spgo_write_observation(spgo_solver* h, int32_t anchor, int32_t target, int32_t feat, const float left[3],
+                           const float right[3]) {
+    if (!h || !left || !right) return SPGO_E_ARG;
+    int rc = write_one(h, anchor, target, feat, left, right);
+    if (rc) return rc;
+    h->layout_dirty = true;
+    h->lin_valid = false;
+    return SPGO_OK;
+}
+
+int spgo_write_observations_bulk(spgo_solver* h, int32_t count, const int32_t* anchor, const int32_t* target,
+                                 const int32_t* feat, const float* uvLR) {
+    if (!h || count < 0 || (count > 0 && (!anchor || !target || !feat || !uvLR))) return SPGO_E_ARG;
+    h->ma.reserve(h->ma.size() + count);
+    h->mt.reserve(h->mt.size() + count);
+    h->ml.reserve(h->ml.size() + count);
+    h->cell_of.reserve(h->cell_of.size() + count);
+    for (int i = 0; i < count; ++i) {
+        const float left[3] = {uvLR[4 * i], uvLR[4 * i + 1], 1.f};
+        const float right[3] = {uvLR[4 * i + 2], uvLR[4 * i + 3], 1.f};
+        int rc = write_one(h, anchor[i], target[i], feat[i], left, right);
+        if (rc) return rc;
+    }
+    h->layout_dirty = true;
+    h->lin_valid = false;
+    return SPGO_OK;
+}
+
+int spgo_get_observation(spgo_solver* h, int32_t frame, int32_t feat, float left[3], float right[3]) {
+    if (!h || !left || !right) return SPGO_E_ARG;
+    if (frame < 0 || frame >= h->F || feat < 0 || feat >= h->L) return h->fail(SPGO_E_ARG, "id out of range");
+    auto it = h->cell_of.find((long long)frame * h->L + feat);
+    for (int i = 0; i < 3; ++i) {
+        left[i] = it == h->cell_of.end() ? 0.f : h->cells[6 * it->second + i];
+        right[i] = it == h->cell_of.end() ? 0.f : h->cells[6 * it->second + 3 + i];
+    }
+    return SPGO_OK;
+}
+
+int spgo_step(spgo_solver* h, int32_t iterations) {
+    if (!h || iterations < 0) return SPGO_E_ARG;
+    return run_step(h, iterations);
+}
+
+int spgo_get_incremental_pose(spgo_solver* h, int32_t k, float T[16]) {
+    if (!h || !T) return SPGO_E_ARG;
+    if (k < 0 || k >= h->N) return h->fail(SPGO_E_ARG, "pose index out of range");
+    int rc = pull_state(h);
+    if (rc) return rc;
+    for (int i = 0; i < 12; ++i) T[i] = (float)h->X[12 * k + i];
+    T[12] = T[13] = T[14] = 0.f;
+    T[15] = 1.f;
+    return SPGO_OK;
+}
+
+int spgo_write_incremental_pose(spgo_solver* h, int32_t k, const float T[16]) {
+    if (!h || !T) return SPGO_E_ARG;
+    if (k < 0 || k >= h->N) return h->fail(SPGO_E_ARG, "pose index out of range");
+    int rc = begin_state_write(h);
+    if (rc) return rc;
+    mat4_to_3x4(T, &h->X[12 * k]);   // rows 0-2 only (Solver.cpp:69-84)
+    return SPGO_OK;
+}
+
+int spgo_load_inverse_depths(spgo_solver* h, const float* alpha) {
+    if (!h || !alpha) return SPGO_E_ARG;
+    int rc = begin_state_write(h);
+    if (rc) return rc;
+    for (int i = 0; i < h->L; ++i) h->alpha[i] = (double)alpha[i];
+    return SPGO_OK;
+}
+
+int spgo_get_inverse_depths(spgo_solver* h, float* alpha) {
+    if (!h || !alpha) return SPGO_E_ARG;
+    int rc = pull_state(h);
+    if (rc) return rc;
+    for (int i = 0; i < h->L; ++i) alpha[i] = (float)h->alpha[i];
+    return SPGO_OK;
+}
+
+int spgo_set_step_size(spgo_solver* h, float s) {
+    if (!h) return SPGO_E_ARG;
+    if (s != h->step_size) destroy_graph(h);
+    h->step_size = s;
+    return SPGO_OK;
+}
+
+int spgo_get_state_f64(spgo_solver* h, double* poses, double* alpha) {
+    if (!h) return SPGO_E_ARG;
+    int rc = pull_state(h);
+    if (rc) return rc;
+    if (poses) {
+        for (int k = 0; k < h->N; ++k) {
+            for (int i = 0; i < 12; ++i) poses[16 * k + i] = h->X[12 * k + i];
+            poses[16 * k + 12] = poses[16 * k + 13] = poses[16 * k + 14] = 0.0;
+            poses[16 * k + 15] = 1.0;
+        }
+    }
+    if (alpha) std::memcpy(alpha, h->alpha.data(), h->alpha.size() * sizeof(double));
+    return SPGO_OK;
+}
+
+int spgo_set_state_f64(spgo_solver* h, const double* poses, const double* alpha) {
+    if (!h) return SPGO_E_ARG;
+    int rc = begin_state_write(h);
+    if (rc) return rc;
+    if (poses)
+        for (int k = 0; k < h->N; ++k)
+            for (int i = 0; i < 12; ++i) h->X[12 * k + i] = poses[16 * k + i];
+    if (alpha) std::memcpy(h->alpha.data(), alpha, h->alpha.size() * sizeof(double));
+    return SPGO_OK;
+}
+
+int spgo_get_stats(spgo_solver* h, spgo_iter_stats* out, int32_t max_records, int32_t* n_out, double* initial_cost) {
+    if (!h) return SPGO_E_ARG;
+    const int n = (int)h->last_rec.size();
+    if (n_out) *n_out = n;
+    if (initial_cost) *initial_cost = h->last_cost0;
+    if (out) {
+        for (int i = 0; i < n && i < max_records; ++i) {
+            out[i].cost_candidate = h->last_rec[i].cost_candidate;
+            out[i].cost = h->last_rec[i].cost;
+            out[i].lambda = h->last_rec[i].lambda;
+            out[i].accepted = h->last_rec[i].accepted;
+            out[i].chol_ok = h->last_rec[i].chol_ok;
+        }
+    }
+    return SPGO_OK;
+}
+
+int spgo_get_lambda(spgo_solver* h, double* lambda) {
+    if (!h || !lambda) return SPGO_E_ARG;
+    if (!h->ctl || h->lambda_reset) {
+        *lambda = h->opt.mode == SPGO_MODE_REFERENCE ? 0.0 : h->opt.lambda0;
+        return SPGO_OK;
+    }
+    Control c;
+    int rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (rc) return rc;
+    *lambda = c.lambda;
+    return SPGO_OK;
+}
+
+int spgo_set_lambda(spgo_solver* h, double lambda) {
+    if (!h || !(lambda >= 0.0)) return SPGO_E_ARG;
+    if (!h->ctl) return h->fail(SPGO_E_STATE, "set_lambda needs a solver that has stepped once");
+    Control c;
+    int rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (rc) return rc;
+    c.lambda = lambda;
+    h->lambda_reset = false;
+    return check_hip(h, hipMemcpy(h->ctl, &c, sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+int spgo_linearize(spgo_solver* h, double lambda, double* H, double* g, double* cost) {
+    if (!h || !(lambda >= 0.0)) return SPGO_E_ARG;
+    int rc = run_step(h, 0);   // layout, state upload and linearisation of the current state
+    if (rc) return rc;
+    Control c;
+    rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (rc) return rc;
+    const double lam_keep = c.lambda;
+    c.lambda = lambda;
+    rc = check_hip(h, hipMemcpy(h->ctl, &c, sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");
+    if (rc) return rc;
+    if (h->prec == SPGO_PREC_MIXED)
+        launch_form_system<float>(h->lay, h->bf, h->ctl, h->H, h->gvec, h->Pp, h->stream);
+    else
+        launch_form_system<double>(h->lay, h->bd, h->ctl, h->H, h->gvec, h->Pp, h->stream);
+    rc = check_hip(h, hipStreamSynchronize(h->stream), "form system");
+    if (rc) return rc;
+    const int P = 6 * h->N;
+    std::vector<double> Hp((size_t)h->Pp * h->Pp);
+    rc = check_hip(h, hipMemcpy(Hp.data(), h->H, Hp.size() * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (rc) return rc;
+    if (H)
+        for (int i = 0; i < P; ++i)
+            for (int j = 0; j <= i; ++j) H[(size_t)i * P + j] = H[(size_t)j * P + i] = Hp[(size_t)i * h->Pp + j];
+    if (g) {
+        rc = check_hip(h, hipMemcpy(g, h->gvec, P * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+        if (rc) return rc;
+    }
+    rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (rc) return rc;
+    if (cost) *cost = c.cost;
+    c.lambda = lam_keep;
+    return check_hip(h, hipMemcpy(h->ctl, &c, sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+int spgo_problem_info(spgo_solver* h, spgo_layout_info* out) {
+    if (!h || !out) return SPGO_E_ARG;
+    HostLayout hl;
+    int rc = build_layout(h, hl);
+    if (rc) return rc;
+    out->n_poses = h->N;
+    out->n_slots = h->L;
+    out->n_measurements = (int)h->ma.size();
+    out->n_landmarks = (int)hl.lm_id.size();
+    out->n_groups = (int)hl.grp_anchor.size();
+    out->n_chunks = (int)hl.chunk.size();
+    out->max_span = hl.grp_K.empty() ? 0 : *std::max_element(hl.grp_K.begin(), hl.grp_K.end());
+    out->n_anchors_used = hl.anchors_used;
+    return SPGO_OK;
+}
+
+int spgo_last_step_ms(spgo_solver* h, double* ms) {
+    if (!h || !ms) return SPGO_E_ARG;
+    *ms = h->last_ms;
+    return SPGO_OK;
+}
+
+const char* spgo_last_error(spgo_solver* h) {
+    if (!h) return g_create_error.c_str();
+    return h->err.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+"""ctypes binding of ``libspgo.so`` (the C ABI declared in include/spgo.h).
+
+The library is loaded from ``spgo/lib/libspgo.so``; there is no other
+implementation behind these calls -- if the library is missing or fails to
+load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_void_p
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libspgo.so")
+
+SPGO_MODE_REFERENCE = 0
+SPGO_MODE_LM = 1
+SPGO_ROBUST_CAUCHY = 0
+SPGO_ROBUST_HUBER = 1
+SPGO_ROBUST_NONE = 2
+SPGO_PREC_FP64 = 0
+SPGO_PREC_MIXED = 1
+
+ERRORS = {-1: "invalid argument", -2: "capacity exceeded", -3: "unsupported layout",
+          -4: "device error", -5: "invalid state"}
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("mode", c_int32), ("robust", c_int32), ("near_boost", c_int32), ("precision", c_int32),
+                ("robust_param", c_double), ("eps", c_double), ("lambda0", c_double),
+                ("lam_up", c_double), ("lam_down", c_double), ("lam_min", c_double), ("lam_max", c_double),
+                ("alpha_min", c_double), ("alpha_max", c_double),
+                ("refine_steps", c_int32), ("use_graph", c_int32),
+                ("chunk_landmarks", c_int32), ("group_chunks", c_int32),
+                ("reserved", c_int32 * 8)]
+
+
+class IterStats(ctypes.Structure):
+    _fields_ = [("cost_candidate", c_double), ("cost", c_double), ("lambda_", c_double),
+                ("accepted", c_int32), ("chol_ok", c_int32)]
+
+
+class LayoutInfo(ctypes.Structure):
+    _fields_ = [("n_poses", c_int32), ("n_slots", c_int32), ("n_measurements", c_int32),
+                ("n_landmarks", c_int32), ("n_groups", c_int32), ("n_chunks", c_int32),
+                ("max_span", c_int32), ("n_anchors_used", c_int32)]
+
+
+F32P = POINTER(c_float)
+F64P = POINTER(c_double)
+I32P = POINTER(c_int32)
+
+# name -> (restype, argtypes); every symbol declared in include/spgo.h
+SIGNATURES = {
+    "spgo_default_options": (c_int32, [POINTER(Options)]),
+    "spgo_create": (c_int32, [c_int32, c_int32, POINTER(Options), POINTER(c_void_p)]),
+    "spgo_destroy": (c_int32, [c_void_p]),
+    "spgo_set_options": (c_int32, [c_void_p, POINTER(Options)]),
+    "spgo_get_options": (c_int32, [c_void_p, POINTER(Options)]),
+    "spgo_load_calibration": (c_int32, [c_void_p, F32P, F32P]),
+    "spgo_get_calibration": (c_int32, [c_void_p, F32P, F32P]),
+    "spgo_write_observation": (c_int32, [c_void_p, c_int32, c_int32, c_int32, F32P, F32P]),
+    "spgo_write_observations_bulk": (c_int32, [c_void_p, c_int32, I32P, I32P, I32P, F32P]),
+    "spgo_get_observation": (c_int32, [c_void_p, c_int32, c_int32, F32P, F32P]),
+    "spgo_step": (c_int32, [c_void_p, c_int32]),
+    "spgo_get_incremental_pose": (c_int32, [c_void_p, c_int32, F32P]),
+    "spgo_write_incremental_pose": (c_int32, [c_void_p, c_int32, F32P]),
+    "spgo_load_inverse_depths": (c_int32, [c_void_p, F32P]),
+    "spgo_get_inverse_depths": (c_int32, [c_void_p, F32P]),
+    "spgo_set_step_size": (c_int32, [c_void_p, c_float]),
+    "spgo_get_state_f64": (c_int32, [c_void_p, F64P, F64P]),
+    "spgo_set_state_f64": (c_int32, [c_void_p, F64P, F64P]),
+    "spgo_get_stats": (c_int32, [c_void_p, POINTER(IterStats), c_int32, I32P, F64P]),
+    "spgo_get_lambda": (c_int32, [c_void_p, F64P]),
+    "spgo_set_lambda": (c_int32, [c_void_p, c_double]),
+    "spgo_linearize": (c_int32, [c_void_p, c_double, F64P, F64P, F64P]),
+    "spgo_problem_info": (c_int32, [c_void_p, POINTER(LayoutInfo)]),
+    "spgo_last_step_ms": (c_int32, [c_void_p, F64P]),
+    "spgo_last_error": (c_char_p, [c_void_p]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libspgo.so once and attach the prototypes."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libspgo.so not built at {path}: run __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, handle=None) -> None:
+    if rc != 0:
+        lib = load()
+        msg = lib.spgo_last_error(handle)
+        msg = msg.decode() if msg else ""
+        raise RuntimeError(f"spgo error {rc} ({ERRORS.get(rc, '?')}): {msg}")
+
+
+def default_options() -> Options:
+    o = Options()
+    check(load().spgo_default_options(ctypes.byref(o)))
+    return o

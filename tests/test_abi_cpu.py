@@ -1,0 +1,143 @@
+"""C-ABI library checks that need no GPU: the library loads, exports every
+entry point include/spgo.h declares, and the host-side API (ingestion,
+getters, layout construction, argument errors) behaves like the reference's
+Solver facade (src/Optimization/Solver.cpp, src/CythonAPI/Solver.pyx)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from spgo import _lib
+from spgo.problems import make_problem
+from spgo.solver import Solver, make_options
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    src = open(os.path.join(ROOT, "include", "spgo.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(spgo_\w+)\s*\(", src, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = _header_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    # the ctypes prototypes cover exactly the declared functions
+    assert sorted(_lib.SIGNATURES) == names
+
+
+def test_default_options_and_struct_layout():
+    o = _lib.default_options()
+    assert o.mode == _lib.SPGO_MODE_LM and o.robust == _lib.SPGO_ROBUST_CAUCHY
+    assert o.robust_param == 9.0 and o.eps == 1e-4 and o.lambda0 == 1e-4
+    assert o.alpha_min == pytest.approx(1 / 30) and o.alpha_max == pytest.approx(5.0)
+    assert ctypes.sizeof(_lib.Options) == 4 * 4 + 9 * 8 + 4 * 4 + 8 * 4
+
+
+def test_reference_facade_semantics_on_host():
+    s = Solver(3, 2)
+    # identity poses, inverse depths 1.0 (include/Solver.hpp:36-44)
+    assert np.array_equal(s.get_incremental_pose(1), np.eye(4, dtype=np.float32))
+    assert np.array_equal(s.get_inverse_depths(), np.ones(6, np.float32))
+    intr = np.array([500, 501, 320, 240, 502, 503, 321, 241], np.float32)
+    T = np.eye(4, dtype=np.float32)
+    T[0, 3] = 0.12
+    s.load_calibration(intr, T.reshape(16))
+    i2, t2 = s.get_calibration()
+    assert np.array_equal(i2, intr) and np.array_equal(t2, T.reshape(16))
+    # observation cells: later writes win, unwritten cells read zero
+    s.write_observation(0, 1, 4, [1, 2, 1], [3, 4, 1])
+    s.write_observation(0, 1, 4, [5, 6, 1], [7, 8, 1])
+    l3, r3 = s.get_observation(1, 4)
+    assert np.array_equal(l3, [5, 6, 1]) and np.array_equal(r3, [7, 8, 1])
+    l3, r3 = s.get_observation(2, 5)
+    assert not l3.any() and not r3.any()
+    # writeIncrementalPose writes rows 0-2 only (Solver.cpp:69-84)
+    P = np.arange(16, dtype=np.float32)
+    s.write_incremental_pose(2, P)
+    got = s.get_incremental_pose(2)
+    assert np.array_equal(got[:3], P.reshape(4, 4)[:3]) and np.array_equal(got[3], [0, 0, 0, 1])
+    a = np.linspace(0.1, 0.6, 6).astype(np.float32)
+    s.load_inverse_depths(a)
+    assert np.array_equal(s.get_inverse_depths(), a)
+    X, al = s.get_state()
+    assert X.dtype == np.float64 and np.array_equal(al, a.astype(np.float64))
+
+
+def test_argument_errors():
+    s = Solver(3, 2)
+    with pytest.raises(RuntimeError, match="out of range"):
+        s.write_observation(0, 4, 0, [0, 0, 1], [0, 0, 1])      # frames are 0..N
+    with pytest.raises(RuntimeError, match="out of range"):
+        s.write_observation(0, 1, 6, [0, 0, 1], [0, 0, 1])      # slots are 0..N*M-1
+    with pytest.raises(RuntimeError, match="out of range"):
+        s.get_incremental_pose(3)
+    with pytest.raises(RuntimeError):
+        Solver(0, 5)
+    # capacity N(N+1)M (include/Solver.hpp:20)
+    s1 = Solver(1, 1)
+    s1.write_observation(0, 0, 0, [0, 0, 1], [0, 0, 1])
+    s1.write_observation(0, 1, 0, [0, 0, 1], [0, 0, 1])
+    with pytest.raises(RuntimeError, match="capacity"):
+        s1.write_observation(0, 1, 0, [0, 0, 1], [0, 0, 1])
+
+
+def test_layout_of_generated_problem():
+    p = make_problem(10, 500, 4, seed=1001)
+    s = Solver(p.n_poses, p.m_per_frame, make_options(chunk_landmarks=64, group_chunks=1))
+    s.write_observations(p.meas_a, p.meas_t, p.meas_l, p.uv)
+    info = s.layout_info()
+    assert info["n_measurements"] == 2000 and info["n_landmarks"] == 500
+    assert info["max_span"] == 3                      # host + 3 following frames
+    per_anchor = np.bincount(p.meas_a[p.meas_t == p.meas_a])
+    assert info["n_groups"] == int(np.sum((per_anchor + 63) // 64))
+    assert info["n_anchors_used"] == int(np.count_nonzero(per_anchor))
+
+
+def test_layout_rejects_mixed_anchors_and_wide_spans():
+    s = Solver(20, 2)
+    s.write_observation(0, 0, 1, [1, 1, 1], [1, 1, 1])
+    s.write_observation(1, 2, 1, [1, 1, 1], [1, 1, 1])
+    with pytest.raises(RuntimeError, match="different anchor"):
+        s.layout_info()
+    s2 = Solver(20, 2)
+    s2.write_observation(0, 0, 1, [1, 1, 1], [1, 1, 1])
+    s2.write_observation(0, 19, 1, [1, 1, 1], [1, 1, 1])
+    with pytest.raises(RuntimeError, match="spans"):
+        s2.layout_info()
+
+
+def test_step_without_gpu_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    p = make_problem(4, 20, 3, seed=3)
+    s = Solver(p.n_poses, p.m_per_frame)
+    s.load_calibration(p.intrinsics, p.T_r_to_l)
+    s.write_observations(p.meas_a, p.meas_t, p.meas_l, p.uv)
+    with pytest.raises(RuntimeError, match="device"):
+        s.step(1)
+
+
+def test_dropin_module_surface():
+    from build_ip import Solver as S
+    cs = S.CudaSolver(3, 4)
+    for name in ["step", "loadCalibration", "writeObservations", "getIncrementalPose", "writeIncrementalPose",
+                 "getObservation", "getCalibration", "loadInverseDepths", "getInverseDepths", "setStepSize"]:
+        assert callable(getattr(cs, name))
+    intr = np.arange(8, dtype=np.float32)
+    cs.loadCalibration(intr, np.eye(4, dtype=np.float32).reshape(16))
+    ci, cT = cs.getCalibration()
+    assert ci.shape == (4, 2) and np.array_equal(ci.reshape(8), intr) and cT.shape == (4, 4)
+    cs.writeObservations(0, 1, 2, np.array([1, 2, 1], np.float32), np.array([3, 4, 1], np.float32))
+    l, r = cs.getObservation(1, 2)
+    assert l.shape == (3, 1) and r.shape == (3, 1) and l[1, 0] == 2
+    assert cs.getIncrementalPose(0).shape == (4, 4) and cs.getIncrementalPose(0).dtype == np.float32
+    assert cs.getInverseDepths().shape == (12,)
+    with pytest.raises(ValueError, match="dtype"):
+        cs.loadInverseDepths(np.ones(12))                       # float64 buffer, as Cython's float[:] check
