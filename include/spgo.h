@@ -150,6 +150,15 @@ int spgo_problem_info(spgo_solver* h, spgo_layout_info* out);
  * with events on the solver's stream. */
 int spgo_last_step_ms(spgo_solver* h, double* ms);
 
+/* Run `iterations` LM iterations with one launch per phase (no graph) and
+ * HIP events around each phase on the solver's stream; ms[p] receives the
+ * mean device time of phase p per iteration:
+ * 0 reduced-system formation, 1 Cholesky factorisation, 2 triangular solves,
+ * 3 pose update + global chain, 4 linearisation (assembly + landmark Schur),
+ * 5 accept/reject.  No reference counterpart (instrumentation). */
+#define SPGO_N_PHASES 6
+int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHASES]);
+
 /* Message of the last failed call on h (h == NULL: last failed spgo_create). */
 const char* spgo_last_error(spgo_solver* h);
 

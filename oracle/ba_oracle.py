@@ -111,6 +111,19 @@ def invert_pose(T: np.ndarray) -> np.ndarray:
     return out
 
 
+def project_rotations(T: np.ndarray) -> np.ndarray:
+    """Poses with their rotation block replaced by the nearest rotation (polar
+    factor U V^T of the SVD).  The library does this to every pose it uploads
+    and to T_r_to_l; float32 inputs are rotations only to ~1e-7, and the
+    reference (which never projects) computes in float32 anyway."""
+    T = np.array(T, dtype=np.float64, copy=True)
+    U, _, Vt = np.linalg.svd(T[..., :3, :3])
+    R = U @ Vt
+    ok = np.linalg.det(T[..., :3, :3]) > 1e-6
+    T[..., :3, :3] = np.where(ok[..., None, None], R, T[..., :3, :3])
+    return T
+
+
 def global_poses(X: np.ndarray, T_rl: np.ndarray):
     """G_0 = I, G_{k+1} = G_k X_k^-1, G^R_k = G_k T_rl, serial
     (Solver_cuda.cu:440-493)."""
@@ -211,7 +224,7 @@ def problem_arrays(N: int, L: int, intrinsics, T_r_to_l, meas_a, meas_t, meas_l,
     for c in range(2):
         fx, fy, cx, cy = intr[4 * c: 4 * c + 4]
         Kc[c] = [[fx, 0, cx], [0, fy, cy], [0, 0, 1]]
-    T = np.asarray(T_r_to_l, dtype=np.float32).astype(np.float64).reshape(4, 4)
+    T = project_rotations(np.asarray(T_r_to_l, dtype=np.float32).astype(np.float64).reshape(4, 4))
     return ProblemArrays(N=int(N), L=int(L), a=a, t=t, l=l, z=z, ray=ray, Kc=Kc, T_rl=T)
 
 
@@ -412,7 +425,7 @@ def lm_solve(pa: ProblemArrays, X0: np.ndarray, alpha0: np.ndarray, iterations: 
     step repeated ``iterations`` times (Solver_cuda.cu:594-782).
     """
     opts = opts or Options()
-    X = np.asarray(X0, dtype=np.float64).copy()
+    X = project_rotations(np.asarray(X0, dtype=np.float64))
     alpha = np.asarray(alpha0, dtype=np.float64).copy()
     ref = opts.mode == "reference"
     if lam is None:

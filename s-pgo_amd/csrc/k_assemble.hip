@@ -358,7 +358,7 @@ __global__ __launch_bounds__(WG) void k_assemble(AsmArgs A, StateBuf<T> B0, Stat
             const int c = tile - r * (r + 1) / 2;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = r * 16 + 4 * (lane >> 4) + v;
+                const int row = r * 16 + mfma16_row((T)0, lane, v);
                 const int col = c * 16 + (lane & 15);
                 if (row < K6 && col < K6) {
                     const int kb = max(row / 6, col / 6);

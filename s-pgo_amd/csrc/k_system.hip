@@ -226,7 +226,7 @@ __global__ __launch_bounds__(256) void k_chol_update(double* H, int Pp, int k, i
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const int rr = sr + 4 * (lane >> 4) + v, cc = sc + (lane & 15);
+        const int rr = sr + mfma16_row(0.0, lane, v), cc = sc + (lane & 15);
         H[(long long)(bi * NB + rr) * Pp + bj * NB + cc] -= acc[v];
     }
 }

@@ -136,6 +136,12 @@ __device__ __forceinline__ int wave_max_int(int v) {
     return v;
 }
 
+// Row of the 16x16 MFMA accumulator held in register v of lane `lane`
+// (column = lane & 15).  Measured on gfx950: the f32 16x16x4 result keeps
+// four consecutive rows per lane group, the f64 16x16x4 result interleaves them.
+__device__ __forceinline__ int mfma16_row(float, int lane, int v) { return 4 * (lane >> 4) + v; }
+__device__ __forceinline__ int mfma16_row(double, int lane, int v) { return (lane >> 4) + 4 * v; }
+
 // Index of (p, q), p <= q < 6, in the row-major upper triangle (21 entries).
 __device__ __forceinline__ int sym6(int p, int q) {
     if (p > q) {

@@ -133,6 +133,32 @@ void mat4_to_3x4(const float* T16, double* X12) {
     for (int i = 0; i < 12; ++i) X12[i] = (double)T16[i];
 }
 
+// Replace the rotation part of a 3x4 pose by its polar factor (the nearest
+// rotation), in double: R <- (R + R^-T) / 2 until it stops changing.  The
+// anchor-frame factorisation of the pose Jacobian relies on Ad(A B) =
+// Ad(A) Ad(B), which holds for rotations only; float32 inputs are rotations to
+// about 1e-7.  Matrices that are singular or reflections are left unchanged.
+void project_rotation(double* P) {
+    double R[9] = {P[0], P[1], P[2], P[4], P[5], P[6], P[8], P[9], P[10]};
+    for (int it = 0; it < 30; ++it) {
+        double C[9] = {R[4] * R[8] - R[5] * R[7], R[5] * R[6] - R[3] * R[8], R[3] * R[7] - R[4] * R[6],
+                       R[2] * R[7] - R[1] * R[8], R[0] * R[8] - R[2] * R[6], R[1] * R[6] - R[0] * R[7],
+                       R[1] * R[5] - R[2] * R[4], R[2] * R[3] - R[0] * R[5], R[0] * R[4] - R[1] * R[3]};
+        const double det = R[0] * C[0] + R[1] * C[1] + R[2] * C[2];
+        if (!(det > 1e-6) || !std::isfinite(det)) return;
+        double change = 0.0;
+        for (int i = 0; i < 9; ++i) {
+            const double n = 0.5 * (R[i] + C[i] / det);   // C / det = R^-T
+            change = std::max(change, std::fabs(n - R[i]));
+            R[i] = n;
+        }
+        if (change < 1e-17) break;
+    }
+    P[0] = R[0], P[1] = R[1], P[2] = R[2];
+    P[4] = R[3], P[5] = R[4], P[6] = R[5];
+    P[8] = R[6], P[9] = R[7], P[10] = R[8];
+}
+
 void destroy_graph(spgo_solver* h) {
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
@@ -395,6 +421,7 @@ Camera make_camera(const spgo_solver* h) {
         c.cy[k] = h->intr[4 * k + 3];
     }
     for (int i = 0; i < 12; ++i) c.Trl[i] = h->Trl[i];
+    project_rotation(c.Trl);
     return c;
 }
 
@@ -466,6 +493,7 @@ int push_state(spgo_solver* h) {
     Control c;
     int rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
     if (rc) return rc;
+    for (int k = 0; k < h->N; ++k) project_rotation(&h->X[12 * k]);
     for (int i = 0; i < 2; ++i) {
         double* X = h->prec == SPGO_PREC_MIXED ? h->bf[i].X : h->bd[i].X;
         double* al = h->prec == SPGO_PREC_MIXED ? h->bf[i].alpha : h->bd[i].alpha;
@@ -920,6 +948,60 @@ int spgo_linearize(spgo_solver* h, double lambda, double* H, double* g, double* 
     if (cost) *cost = c.cost;
     c.lambda = lam_keep;
     return check_hip(h, hipMemcpy(h->ctl, &c, sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHASES]) {
+    if (!h || !ms || iterations < 1) return SPGO_E_ARG;
+    int rc = run_step(h, 0);
+    if (rc) return rc;
+    if (iterations > h->rec_cap) {
+        destroy_graph(h);
+        h->rec = h->mem.alloc<IterRecord>(iterations);
+        if (!h->rec) return h->fail(SPGO_E_DEVICE, "device allocation failed");
+        h->rec_cap = iterations;
+    }
+    hipEvent_t ev[SPGO_N_PHASES + 1];
+    for (int i = 0; i <= SPGO_N_PHASES; ++i)
+        if (hipEventCreate(&ev[i]) != hipSuccess) return h->fail(SPGO_E_DEVICE, "hipEventCreate failed");
+    double acc[SPGO_N_PHASES] = {0};
+    hipStream_t s = h->stream;
+    const bool mixed = h->prec == SPGO_PREC_MIXED;
+    for (int it = 0; it < iterations && rc == SPGO_OK; ++it) {
+        (void)hipEventRecord(ev[0], s);
+        if (mixed) launch_form_system<float>(h->lay, h->bf, h->ctl, h->H, h->gvec, h->Pp, s);
+        else launch_form_system<double>(h->lay, h->bd, h->ctl, h->H, h->gvec, h->Pp, s);
+        (void)hipEventRecord(ev[1], s);
+        launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
+        (void)hipEventRecord(ev[2], s);
+        launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, s);
+        (void)hipEventRecord(ev[3], s);
+        if (mixed) launch_pose_update<float>(h->lay, make_camera(h), h->bf, h->ctl, h->delta, h->step_size, 0, s);
+        else launch_pose_update<double>(h->lay, make_camera(h), h->bd, h->ctl, h->delta, h->step_size, 0, s);
+        (void)hipEventRecord(ev[4], s);
+        if (mixed) launch_assemble<float>(make_asm(h, 0), h->bf, s);
+        else launch_assemble<double>(make_asm(h, 0), h->bd, s);
+        (void)hipEventRecord(ev[5], s);
+        if (mixed)
+            launch_accept(h->bf[0].cost, h->bf[1].cost, h->lay.n_groups, h->ctl, h->rec,
+                          h->opt.mode == SPGO_MODE_REFERENCE, h->opt.lam_up, h->opt.lam_down, h->opt.lam_min,
+                          h->opt.lam_max, 0, s);
+        else
+            launch_accept(h->bd[0].cost, h->bd[1].cost, h->lay.n_groups, h->ctl, h->rec,
+                          h->opt.mode == SPGO_MODE_REFERENCE, h->opt.lam_up, h->opt.lam_down, h->opt.lam_min,
+                          h->opt.lam_max, 0, s);
+        (void)hipEventRecord(ev[6], s);
+        rc = check_hip(h, hipEventSynchronize(ev[6]), "phase timing");
+        for (int p = 0; p < SPGO_N_PHASES && rc == SPGO_OK; ++p) {
+            float t = 0.f;
+            (void)hipEventElapsedTime(&t, ev[p], ev[p + 1]);
+            acc[p] += t;
+        }
+    }
+    for (int i = 0; i <= SPGO_N_PHASES; ++i) (void)hipEventDestroy(ev[i]);
+    if (rc) return rc;
+    for (int p = 0; p < SPGO_N_PHASES; ++p) ms[p] = acc[p] / iterations;
+    h->dev_newer = true;
+    return SPGO_OK;
 }
 
 int spgo_problem_info(spgo_solver* h, spgo_layout_info* out) {

@@ -76,6 +76,7 @@ SIGNATURES = {
     "spgo_linearize": (c_int32, [c_void_p, c_double, F64P, F64P, F64P]),
     "spgo_problem_info": (c_int32, [c_void_p, POINTER(LayoutInfo)]),
     "spgo_last_step_ms": (c_int32, [c_void_p, F64P]),
+    "spgo_profile_phases": (c_int32, [c_void_p, c_int32, F64P]),
     "spgo_last_error": (c_char_p, [c_void_p]),
 }
 

@@ -199,6 +199,15 @@ class Solver:
         self._check(self._lib.spgo_problem_info(self._h, ctypes.byref(info)))
         return {k: getattr(info, k) for k, _ in LayoutInfo._fields_}
 
+    PHASES = ("form_system", "cholesky", "solve", "pose_update", "assemble", "accept")
+
+    def profile_phases(self, iterations: int) -> dict:
+        """Mean device ms per LM iteration of each phase, from HIP events on the
+        solver's stream over ``iterations`` directly launched iterations."""
+        ms = np.zeros(len(self.PHASES), np.float64)
+        self._check(self._lib.spgo_profile_phases(self._h, int(iterations), ms.ctypes.data_as(F64P)))
+        return dict(zip(self.PHASES, ms.tolist()))
+
     def last_step_ms(self) -> float:
         v = ctypes.c_double()
         self._check(self._lib.spgo_last_step_ms(self._h, ctypes.byref(v)))

@@ -63,7 +63,9 @@ def test_reduced_system_matches_oracle():
     s = solver_from_problem(p, go)
     lam = 0.25
     H, g, cost = s.linearize(lam)
-    lin = O.linearize(pa, p.X0.astype(np.float64), p.alpha0.astype(np.float64), oo)
+    # the library projects uploaded rotations onto SO(3); so does the oracle's lm_solve
+    X0 = O.project_rotations(p.X0.astype(np.float64))
+    lin = O.linearize(pa, X0, p.alpha0.astype(np.float64), oo)
     Ho, go_ = O.reduced_system(lin, lam, oo.eps)
     print(f"[reduced] |H|={np.max(np.abs(Ho)):.4g} err={np.max(np.abs(H - Ho)):.3g} "
           f"|g|={np.max(np.abs(go_)):.4g} err={np.max(np.abs(g - go_)):.3g}", flush=True)
