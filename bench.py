@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""LM iterations/s of the MI355X stereo bundle-adjustment step.
+
+Workload (default): one seeded problem with 64 keyframes, 20,000 landmarks and
+8 stereo observations per landmark (160,000 stereo measurements), solved in
+FP64 by damped LM (lambda0 = 1e-4, Cauchy weights).  A step is one LM
+iteration: reduced-system formation, Cholesky, solves, pose update and the
+linearisation of the candidate with accept/reject -- all on the GPU, inputs
+resident in HBM.  Prints one JSON line.
+
+The roofline object describes the phase that takes the most device time,
+timed with HIP events on the solver's stream over the timed iterations
+(a second pass of the same iterations launched one phase at a time).  The CPU
+baseline is the FP64 oracle timed on this host on a bounded sample (two LM
+iterations of the same problem).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "s-pgo_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def assembly_bytes(p, real_bytes: int = 8) -> int:
+    """Algorithmic HBM bytes of one linearisation launch (candidate mode):
+    per measurement the target id, packed landmark/position index and four
+    float32 pixels (24 B); per landmark its slot id, ray pixel, position range,
+    inverse depth in and out, C and g_alpha in and out, and its B column v_l
+    (6 k_l values, read for the back-substitution and written new); per anchor
+    frame the reduced (6 k_a)^2 block and 6 k_a gradient written."""
+    a, t, l = p.meas_a.astype(np.int64), p.meas_t.astype(np.int64), p.meas_l.astype(np.int64)
+    O = a.size
+    span = np.zeros(p.n_slots, np.int64)
+    np.maximum.at(span, l, np.maximum(t - a, 0))
+    used = np.unique(l)
+    k_l = span[used]
+    anchors = np.zeros(p.n_slots, np.int64)
+    anchors[l] = a
+    k_a = np.zeros(p.n_poses + 1, np.int64)
+    np.maximum.at(k_a, anchors[used], k_l)
+    b = O * 24
+    b += used.size * (4 + 8 + 8 + 2 * real_bytes + 4 * real_bytes)
+    b += int(np.sum(6 * k_l)) * real_bytes * 2
+    b += int(np.sum((6 * k_a) ** 2 + 6 * k_a)) * real_bytes
+    return int(b)
+
+
+def cholesky_flops(n_poses: int) -> float:
+    P = 6 * n_poses
+    return P ** 3 / 3.0
+
+
+def cpu_baseline(p, n_iter: int, gpu_recs):
+    """FP64 oracle on the host: seconds per LM iteration on n_iter iterations,
+    and the relative cost difference of the GPU's first iterations."""
+    from oracle import ba_oracle as O
+
+    try:
+        from threadpoolctl import threadpool_info
+        cores = max([i.get("num_threads", 1) for i in threadpool_info()] + [1])
+    except Exception:
+        cores = 1
+    pa = O.from_problem(p)
+    X0 = p.X0.astype(np.float64)
+    a0 = p.alpha0.astype(np.float64)
+    t0 = time.perf_counter()
+    O.lm_solve(pa, X0, a0, 0, O.Options())
+    t1 = time.perf_counter()
+    _, _, _, c0, recs = O.lm_solve(pa, X0, a0, n_iter, O.Options())
+    t2 = time.perf_counter()
+    per_iter = max((t2 - t1) - (t1 - t0), 1e-9) / n_iter
+    errs = [abs(q["cost"] - r.cost) / abs(r.cost) for q, r in zip(gpu_recs, recs)]
+    return per_iter, cores, max(errs) if errs else None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--chunk-landmarks", type=int, default=0)
+    ap.add_argument("--group-chunks", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the single-problem LM step is measured on one MI355X)", file=sys.stderr)
+        return 2
+
+    from spgo.problems import make_config
+    from spgo.solver import make_options, solver_from_problem
+
+    p = make_config("c2_fp64")
+    kw = {}
+    if args.chunk_landmarks:
+        kw["chunk_landmarks"] = args.chunk_landmarks
+    if args.group_chunks:
+        kw["group_chunks"] = args.group_chunks
+    s = solver_from_problem(p, make_options(mode="lm", **kw))
+
+    # warm-up iterations, then restart from the initial state
+    if args.warmup > 0:
+        s.step(args.warmup)
+    X0 = p.X0.astype(np.float64)
+    s.set_state(X0, p.alpha0.astype(np.float64))
+    s.step(0)                                   # linearise the initial state (synchronises)
+    t0 = time.perf_counter()
+    s.step(args.steps)                          # synchronises before returning
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    dev_ms = s.last_step_ms()
+    c0, recs = s.stats()
+
+    # second pass: the same iterations, one launch per phase, HIP events per phase
+    s.set_state(X0, p.alpha0.astype(np.float64))
+    phases = s.profile_phases(args.steps)
+    dom = max(phases, key=phases.get)
+    if dom == "cholesky":
+        flops = cholesky_flops(p.n_poses)
+        ach = flops / (phases[dom] * 1e-3) / 1e12
+        roof = {"kernel": "cholesky (k_chol_panel + k_chol_update)", "bound": "mfma", "achieved": ach,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None}
+    else:
+        nbytes = assembly_bytes(p)
+        ach = nbytes / (phases["assemble"] * 1e-3) / 1e9
+        roof = {"kernel": "k_assemble<double>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": nbytes}
+        if dom != "assemble":
+            roof["note"] = f"dominant phase is {dom}; roofline given for the assembly kernel"
+
+    cpu = None
+    err = None
+    if not args.no_cpu:
+        per_iter, cores, err = cpu_baseline(p, 2, recs)
+        cpu = {"value": 1.0 / per_iter, "unit": "LM iterations/s", "cores": cores, "kind": "port",
+               "sample": "FP64 oracle (numpy/scipy restatement of the reference step), 2 LM iterations "
+                         "of the same 64-keyframe problem"}
+
+    line = {
+        "metric": "LM iters/s & batched solves/s vs plain FP64 GPU solver at matched final cost",
+        "value": args.steps / wall,
+        "unit": "LM iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "device_ms_per_step": dev_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator: circle trajectory r=2 m, landmarks in a 4 m box, 1 px noise)",
+        "config": {"workload": "64 keyframes, 20000 landmarks, 8 stereo observations each (160000 stereo "
+                               "measurements), FP64 plain LM, lambda0 1e-4, Cauchy c^2=9",
+                   "keyframes": p.n_poses, "landmarks": p.n_landmarks, "measurements": p.n_obs,
+                   "reduced_system": 6 * p.n_poses},
+        "cost_initial": c0,
+        "cost_final": recs[-1]["cost"] if recs else c0,
+        "oracle_rel_err": err,
+        "phases_ms": phases,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

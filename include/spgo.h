@@ -46,6 +46,12 @@ extern "C" {
 #define SPGO_PREC_FP64 0
 #define SPGO_PREC_MIXED 1
 
+/* Reduced-system factorisation: AUTO picks ENVELOPE when the block profile of
+ * the reduced system is narrow, DENSE (multi-kernel blocked) otherwise. */
+#define SPGO_FACTOR_AUTO 0
+#define SPGO_FACTOR_DENSE 1
+#define SPGO_FACTOR_ENVELOPE 2
+
 typedef struct spgo_options {
     int32_t mode;            /* SPGO_MODE_*                                    */
     int32_t robust;          /* SPGO_ROBUST_*                                  */
@@ -60,7 +66,8 @@ typedef struct spgo_options {
     int32_t use_graph;       /* replay each LM iteration as a captured graph   */
     int32_t chunk_landmarks; /* tile shape: landmarks per LDS chunk (<= 64)    */
     int32_t group_chunks;    /* tile shape: chunks per workgroup               */
-    int32_t reserved[8];
+    int32_t factorization;   /* SPGO_FACTOR_*                                   */
+    int32_t reserved[7];
 } spgo_options;
 
 typedef struct spgo_iter_stats {

@@ -13,8 +13,9 @@
 //   * per landmark C, g_alpha and v_l(i) = sum_{t>i} J_a^T j_alpha (its B column),
 //   * Y = sum_l v_l v_l^T / (C_l + eps) and y = sum_l v_l g_l / (C_l + eps)
 //     on MFMA (the landmark Schur contraction, Solver_cuda.cu:690-712),
-// and writes D = Q - Y, d = q - y.  The pose system is formed from these
-// blocks by k_form_system.  Reductions are deterministic: wavefront-private
+// and writes D = Q - Y, d = q - y mapped to the incremental-pose parameters
+// with the block-diagonal adjoints Ad(G_a^-1 G_i).  The pose system is the sum
+// of these blocks over the groups that cover each pose pair (k_form_system).  Reductions are deterministic: wavefront-private
 // LDS partials combined in a fixed order, no global atomics.
 #include "spgo_device.h"
 #include "spgo_kernels.h"
@@ -27,6 +28,7 @@ constexpr int VS = 66;                 // LDS row stride of the V tile (doubles)
 constexpr int VR = 6 * KMAX;           // rows of the V tile (96)
 constexpr int SCR = 27;                // per-lane stride of the pair scratch
 constexpr int NWAVE = WG / 64;
+constexpr int VSH = (VR * VR > NWAVE * 64 * SCR) ? VR * VR : NWAVE * 64 * SCR;   // >= VR * VS
 constexpr int MAXTILE = ((VR / 16) * (VR / 16 + 1) / 2 + NWAVE - 1) / NWAVE;   // tiles per wave
 
 template <typename T> struct Vec4;
@@ -92,11 +94,12 @@ __global__ __launch_bounds__(WG) void k_assemble(AsmArgs A, StateBuf<T> B0, Stat
     __shared__ T dl[KMAX][6];                   // Ad(G_a^-1 G_{a+k}) delta_{a+k} (back-substitution)
     __shared__ T stage[CHUNK_OBS][8];           // per measurement: J_a^T j_alpha (6), j_alpha^2, j_alpha r
     __shared__ int stslot[CHUNK_OBS];
-    __shared__ T Vsh[(VR * VS > NWAVE * 64 * SCR) ? VR * VS : NWAVE * 64 * SCR];
+    __shared__ T Vsh[VSH];
     __shared__ int sslot[NWAVE][64];
     __shared__ T gt[CHUNK_LM];
     __shared__ T lmP[CHUNK_LM][6];              // P_a and -P_a/alpha in the anchor frame
     __shared__ double red[NWAVE];
+    __shared__ double Adk[KMAX][36];            // Ad(G_a^-1 G_{a+k}) of the state being linearised
 
     // ---- group set-up ------------------------------------------------------
     for (int i = tid; i < NWAVE * (KMAX + 1) * 27; i += WG) (&S_w[0][0][0])[i] = (T)0;
@@ -109,6 +112,14 @@ __global__ __launch_bounds__(WG) void k_assemble(AsmArgs A, StateBuf<T> B0, Stat
         pose_mul(Gti, Ga, Tr);
 #pragma unroll
         for (int i = 0; i < 12; ++i) Trel[c][s][i] = (T)Tr[i];
+    }
+    if (tid >= 64 && tid < 64 + K) {
+        const int k = tid - 64;
+        double Gai[12], Gi[12], E[12];
+        pose_load(bout.GinvL + 12 * a, Gai);
+        pose_load(bout.G + 12 * (a + k), Gi);
+        pose_mul(Gai, Gi, E);
+        for (int e = 0; e < 36; ++e) Adk[k][e] = adjoint_entry(E, e / 6, e % 6);
     }
     if (update && tid < K) {
         double Gai[12], Gi[12], E[12], x[6], out[6];
@@ -348,7 +359,9 @@ __global__ __launch_bounds__(WG) void k_assemble(AsmArgs A, StateBuf<T> B0, Stat
         }
     }
     __syncthreads();
-    T* Dg = bout.D + lay.grp_D_off[g];
+    // D = Q - Y into LDS (row stride 6K), d = q - y beside it
+    T* Dl = Vsh;
+    T* dlv = &stage[0][0] + KMAX * 27;
 #pragma unroll
     for (int ti = 0; ti < MAXTILE; ++ti) {
         const int tile = wv + ti * NWAVE;
@@ -364,13 +377,48 @@ __global__ __launch_bounds__(WG) void k_assemble(AsmArgs A, StateBuf<T> B0, Stat
                     const int kb = max(row / 6, col / 6);
                     const T q = Rsuf[kb * 27 + sym6(row % 6, col % 6)];
                     const T val = q - acc[ti][v];
-                    Dg[(long long)row * K6 + col] = val;
-                    Dg[(long long)col * K6 + row] = val;
+                    Dl[row * K6 + col] = val;
+                    Dl[col * K6 + row] = val;
                 }
             }
         }
     }
-    if (tid < K6) bout.d[lay.grp_d_off[g] + tid] = Rsuf[(tid / 6) * 27 + 21 + (tid % 6)] - yr;
+    if (tid < K6) dlv[tid] = Rsuf[(tid / 6) * 27 + 21 + (tid % 6)] - yr;
+    __syncthreads();
+    // to incremental-pose coordinates: block (i, j) <- Ad_i^T D(i, j) Ad_j, d_i <- Ad_i^T d_i,
+    // Ad_k = Ad(G_a^-1 G_{a+k}); lower blocks only (the reduced system reads i >= j)
+    T* Dg = bout.D + lay.grp_D_off[g];
+    for (int pr = tid; pr < K * (K + 1) / 2; pr += WG) {
+        int bi = 0;
+        while ((bi + 1) * (bi + 2) / 2 <= pr) ++bi;
+        const int bj = pr - bi * (bi + 1) / 2;
+        double M1[36];
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                double s = 0.0;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) s += (double)Dl[(6 * bi + r) * K6 + 6 * bj + q] * Adk[bj][6 * q + c];
+                M1[6 * r + c] = s;
+            }
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                double s = 0.0;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) s += Adk[bi][6 * q + r] * M1[6 * q + c];
+                Dg[(long long)(6 * bi + r) * K6 + 6 * bj + c] = (T)s;
+            }
+    }
+    if (tid < K6) {
+        const int k = tid / 6, r = tid % 6;
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) s += Adk[k][6 * q + r] * (double)dlv[6 * k + q];
+        bout.d[lay.grp_d_off[g] + tid] = (T)s;
+    }
     cost = wave_sum(cost);
     if (lane == 0) red[wv] = cost;
     __syncthreads();

@@ -106,6 +106,8 @@ struct spgo_solver {
     StateBuf<float> bf[2]{};
     double *H = nullptr, *gvec = nullptr, *delta = nullptr, *Linv = nullptr;
     int Pp = 0;
+    bool band = false;          // envelope Cholesky in one workgroup (narrow profile)
+    const int* first = nullptr; // first nonzero block column of every block row
     Control* ctl = nullptr;
     IterRecord* rec = nullptr;
     int rec_cap = 0;
@@ -387,7 +389,42 @@ int setup_layout(spgo_solver* h) {
         rc = alloc_state<double>(h, h->bd, L, lay.n_act);
     if (rc) return rc;
     const int P = 6 * h->N;
-    h->Pp = ((P + NB - 1) / NB) * NB;
+    const int BBk = chol_band_block();
+    h->Pp = ((P + BBk - 1) / BBk) * BBk;
+    {
+        // envelope of the reduced system: pose j couples back to the smallest
+        // anchor among the groups covering it
+        std::vector<int> fpose(h->N);
+        for (int j = 0; j < h->N; ++j) fpose[j] = j;
+        for (size_t g = 0; g < L.grp_anchor.size(); ++g)
+            for (int k = 0; k < L.grp_K[g]; ++k) {
+                const int j = L.grp_anchor[g] + k;
+                if (j < h->N) fpose[j] = std::min(fpose[j], L.grp_anchor[g]);
+            }
+        // k_form_system stages the groups of the anchors that can reach a pose row
+        for (int j = 0; j < h->N; ++j) {
+            const int amin = std::max(0, j - L.kmax + 1);
+            if (L.anchor_grp[j + 1] - L.anchor_grp[amin] > 512)
+                return h->fail(SPGO_E_LAYOUT, "too many landmark groups reach pose " + std::to_string(j) +
+                                                  "; raise chunk_landmarks or group_chunks");
+        }
+        const int nbk = h->Pp / BBk;
+        std::vector<int> fb(nbk);
+        int bw = 0;
+        for (int i = 0; i < nbk; ++i) {
+            int f = i;
+            for (int r = i * BBk; r < (i + 1) * BBk && r < P; ++r) f = std::min(f, (6 * fpose[r / 6]) / BBk);
+            fb[i] = f;
+            bw = std::max(bw, i - f);
+        }
+        const bool fits = bw <= chol_band_max_blocks() && chol_band_lds(h->Pp) <= 160 * 1024 - 40 * 1024;
+        if (h->opt.factorization == SPGO_FACTOR_ENVELOPE && !fits)
+            return h->fail(SPGO_E_LAYOUT, "reduced-system profile too wide for the envelope factorisation");
+        h->band = fits && h->opt.factorization != SPGO_FACTOR_DENSE;
+        if ((rc = upload(h, fb, &h->first))) return rc;
+        if (h->band && prepare_chol_band(h->Pp) != 0)
+            return h->fail(SPGO_E_DEVICE, "cannot reserve LDS for the envelope Cholesky");
+    }
     h->H = h->mem.alloc<double>((size_t)h->Pp * h->Pp);
     h->gvec = h->mem.alloc<double>(h->Pp);
     h->delta = h->mem.alloc<double>(h->Pp);
@@ -445,8 +482,12 @@ template <typename T>
 void launch_iteration(spgo_solver* h, StateBuf<T>* b) {
     hipStream_t s = h->stream;
     launch_form_system<T>(h->lay, b, h->ctl, h->H, h->gvec, h->Pp, s);
-    launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
-    launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, s);
+    if (h->band) {
+        launch_chol_band(h->H, h->Pp, h->first, h->gvec, h->delta, h->Linv, h->ctl, s);
+    } else {
+        launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
+        launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, s);
+    }
     launch_pose_update<T>(h->lay, make_camera(h), b, h->ctl, h->delta, (double)h->step_size, 0, s);
     launch_assemble<T>(make_asm(h, 0), b, s);
     const int n_groups = h->lay.n_groups;
@@ -615,6 +656,7 @@ bool valid_opts(const spgo_options* o, std::string& why) {
     if (!(o->alpha_min > 0.0) || !(o->alpha_max >= o->alpha_min)) return why = "alpha bounds", false;
     if (o->chunk_landmarks < 0 || o->chunk_landmarks > CHUNK_LM) return why = "chunk_landmarks", false;
     if (o->group_chunks < 0) return why = "group_chunks", false;
+    if (o->factorization < 0 || o->factorization > 2) return why = "factorization", false;
     return true;
 }
 
@@ -699,7 +741,7 @@ int spgo_set_options(spgo_solver* h, const spgo_options* o) {
     std::string why;
     if (!valid_opts(o, why)) return h->fail(SPGO_E_ARG, "invalid option: " + why);
     const bool relayout = o->precision != h->opt.precision || o->chunk_landmarks != h->opt.chunk_landmarks ||
-                          o->group_chunks != h->opt.group_chunks;
+                          o->group_chunks != h->opt.group_chunks || o->factorization != h->opt.factorization;
     const bool relin = relayout || o->robust != h->opt.robust || o->robust_param != h->opt.robust_param ||
                        o->near_boost != h->opt.near_boost || o->eps != h->opt.eps;
     if (o->mode != h->opt.mode || o->lambda0 != h->opt.lambda0) h->lambda_reset = true;
@@ -971,9 +1013,14 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
         if (mixed) launch_form_system<float>(h->lay, h->bf, h->ctl, h->H, h->gvec, h->Pp, s);
         else launch_form_system<double>(h->lay, h->bd, h->ctl, h->H, h->gvec, h->Pp, s);
         (void)hipEventRecord(ev[1], s);
-        launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
-        (void)hipEventRecord(ev[2], s);
-        launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, s);
+        if (h->band) {
+            launch_chol_band(h->H, h->Pp, h->first, h->gvec, h->delta, h->Linv, h->ctl, s);
+            (void)hipEventRecord(ev[2], s);
+        } else {
+            launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
+            (void)hipEventRecord(ev[2], s);
+            launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, s);
+        }
         (void)hipEventRecord(ev[3], s);
         if (mixed) launch_pose_update<float>(h->lay, make_camera(h), h->bf, h->ctl, h->delta, h->step_size, 0, s);
         else launch_pose_update<double>(h->lay, make_camera(h), h->bd, h->ctl, h->delta, h->step_size, 0, s);

@@ -20,6 +20,9 @@ SPGO_ROBUST_HUBER = 1
 SPGO_ROBUST_NONE = 2
 SPGO_PREC_FP64 = 0
 SPGO_PREC_MIXED = 1
+SPGO_FACTOR_AUTO = 0
+SPGO_FACTOR_DENSE = 1
+SPGO_FACTOR_ENVELOPE = 2
 
 ERRORS = {-1: "invalid argument", -2: "capacity exceeded", -3: "unsupported layout",
           -4: "device error", -5: "invalid state"}
@@ -32,7 +35,7 @@ class Options(ctypes.Structure):
                 ("alpha_min", c_double), ("alpha_max", c_double),
                 ("refine_steps", c_int32), ("use_graph", c_int32),
                 ("chunk_landmarks", c_int32), ("group_chunks", c_int32),
-                ("reserved", c_int32 * 8)]
+                ("factorization", c_int32), ("reserved", c_int32 * 7)]
 
 
 class IterStats(ctypes.Structure):

@@ -37,6 +37,7 @@ def test_default_options_and_struct_layout():
     assert o.robust_param == 9.0 and o.eps == 1e-4 and o.lambda0 == 1e-4
     assert o.alpha_min == pytest.approx(1 / 30) and o.alpha_max == pytest.approx(5.0)
     assert ctypes.sizeof(_lib.Options) == 4 * 4 + 9 * 8 + 4 * 4 + 8 * 4
+    assert o.factorization == _lib.SPGO_FACTOR_AUTO
 
 
 def test_reference_facade_semantics_on_host():

@@ -47,11 +47,12 @@ def _check_run(p_or_pa, s, iters, oo, X0, a0, tag):
     return X, al, recs
 
 
-@pytest.mark.parametrize("mode", ["lm", "reference"])
-def test_c1_parity(mode):
+@pytest.mark.parametrize("mode,factor", [("lm", 0), ("reference", 0), ("lm", 1), ("reference", 1)])
+def test_c1_parity(mode, factor):
+    # factor 0: automatic (envelope Cholesky here), 1: dense blocked Cholesky
     p = make_config("c1_small")
     pa = O.from_problem(p)
-    go, oo = _opts_pair(mode=mode)
+    go, oo = _opts_pair(mode=mode, factorization=factor)
     s = solver_from_problem(p, go)
     _check_run(pa, s, 10, oo, p.X0.astype(np.float64), p.alpha0.astype(np.float64), f"c1-{mode}")
 
@@ -74,10 +75,11 @@ def test_reduced_system_matches_oracle():
     assert np.max(np.abs(g - go_)) <= 1e-10 * np.max(np.abs(go_))
 
 
-def test_medium_parity_huber_and_options():
+@pytest.mark.parametrize("factor", [0, 1])
+def test_medium_parity_huber_and_options(factor):
     p = make_problem(24, 3000, 6, seed=31, outlier_frac=0.05)
     pa = O.from_problem(p)
-    go, oo = _opts_pair(robust="huber", near_boost=False)
+    go, oo = _opts_pair(robust="huber", near_boost=False, factorization=factor)
     s = solver_from_problem(p, go)
     _check_run(pa, s, 8, oo, p.X0.astype(np.float64), p.alpha0.astype(np.float64), "medium-huber")
 
