@@ -241,20 +241,33 @@ void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t 
 // ---------------------------------------------------------------------------
 // Envelope (profile) Cholesky and both triangular solves in one workgroup.
 //
-// H is block-banded: block row i (BB x BB blocks) has nonzero blocks only in
-// columns first[i] .. i (first[] nondecreasing), and the Cholesky factor keeps
-// that envelope.  Right-looking over block columns k: the tall panel (the
-// diagonal block and the m <= BMAXB blocks below it with first <= k) sits in
-// registers, one wavefront per block and one row per lane; every column j is
-// one LDS broadcast of the diagonal block's column and one barrier.  The
-// right-hand side rides along as an extra column, so the forward solve is done
-// with the factorisation.  The trailing update of the blocks below is on MFMA
-// and the backward solve runs block by block from the stored factor.
+// H is block-banded with EB x EB blocks: block row i has nonzero blocks only
+// in columns first[i] .. i (first[] nondecreasing, i - first[i] <= D), and
+// the Cholesky factor keeps that envelope.  The blocks that are still to be
+// updated live in a sliding LDS window of W = D + 3 block rows; block (i, j)
+// sits in the slot of the unordered pair (i mod W, j mod W), which is unique
+// among the resident rows.  Right-looking over block columns k, two barriers
+// per column:
+//   phase A  wavefront 0 (and 1, 2 for profiles deeper than 3 blocks) factors
+//            the panel of column k from registers, one row per lane: lanes
+//            0-15 the diagonal block, lanes 16-63 three blocks below it (a
+//            second or third wavefront repeats the diagonal block for its own
+//            three blocks, so no wavefront waits on another inside the column
+//            loop).  The right-hand side rides along, so the forward solve is
+//            done with the factorisation.  Meanwhile the other wavefronts apply
+//            the part of the previous column's trailing update that the next
+//            panel does not need, and wavefront 3 moves the block row it
+//            prefetched into the window and prefetches the next one.
+//   phase B  the trailing update of block column k + 1 (the next panel) on
+//            MFMA, one 16x16 tile per wavefront.
+// The factor goes back to global memory as it is produced; the backward solve
+// then runs block by block from the bottom, its loads one block ahead.
 // ---------------------------------------------------------------------------
-constexpr int BB = 32;      // block size of the envelope factorisation
-constexpr int BMAXB = 3;    // blocks below the diagonal (tall panel of at most 128 rows)
-constexpr int PR = BB * (BMAXB + 1);
-constexpr int FST = BB + 1; // LDS row stride of the staged panel
+constexpr int EB = 16;            // block size: one MFMA tile
+constexpr int EDMAX = 7;          // deepest profile: blocks below the diagonal
+constexpr int EST = EB + 1;       // LDS row stride of a window block
+constexpr int EBLK = EB * EST;    // doubles per window block
+constexpr int EPRE = 4 * (EDMAX + 1);   // prefetched doubles per lane of one block row
 
 // 1/sqrt(x) in double: the hardware estimate refined by one Halley step
 // (cubic convergence), a few FMAs instead of the library's long sequence.
@@ -269,7 +282,7 @@ __device__ __forceinline__ double rsqrt_fast(double x) {
 __device__ __forceinline__ void wave_lds_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
 
 // workgroup barrier that orders LDS only: it does not wait for outstanding
-// global stores (s_waitcnt lgkmcnt(0), vmcnt/expcnt left at their maximum)
+// global loads or stores (s_waitcnt lgkmcnt(0), vmcnt/expcnt left at their maximum)
 __device__ __forceinline__ void lds_barrier() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_waitcnt(0xC07F);
@@ -277,197 +290,269 @@ __device__ __forceinline__ void lds_barrier() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-__global__ __launch_bounds__(256) void k_chol_band(double* H, int Pp, const int* first, const double* g, double* x,
-                                                   double* dinv, Control* ctl) {
-    // S: the tall panel of block column k (diagonal block and m <= BMAXB
-    // blocks below), staged through LDS so global traffic is row-contiguous.
-    // Wavefront 0 factors it, lane l holding panel rows l and l + 64, with no
-    // barrier inside the column loop.  The right-hand side / solution xs and
-    // the inverse diagonal dv stay in LDS for the whole factorisation.
-    extern __shared__ double xs[];          // Pp
-    double* dv = xs + Pp;                   // Pp
-    __shared__ double S[PR * FST];
-    __shared__ double part[BMAXB][BB];
+__host__ __device__ constexpr int env_window_rows(int D) { return D + 3; }
+__host__ __device__ constexpr int env_window_slots(int D) { return (D + 3) * (D + 4) / 2; }
+
+// window block (i, j), i >= j
+__device__ __forceinline__ double* env_block(double* win, int W, int i, int j) {
+    int a = i % W, b = j % W;
+    if (a < b) {
+        const int t = a;
+        a = b;
+        b = t;
+    }
+    return win + (a * (a + 1) / 2 + b) * EBLK;
+}
+
+// C(bi, bj) -= L(bi, kp) L(bj, kp)^T on one 16x16 MFMA tile (one wavefront)
+__device__ __forceinline__ void env_tile(double* win, int W, int bi, int bj, int kp, int lane) {
+    const double* Ai = env_block(win, W, bi, kp) + (lane & 15) * EST + (lane >> 4);
+    const double* Aj = env_block(win, W, bj, kp) + (lane & 15) * EST + (lane >> 4);
+    double* C = env_block(win, W, bi, bj) + (lane & 15);
+    double av[4], bv[4], cv[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        av[s] = Ai[4 * s];
+        bv[s] = Aj[4 * s];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) cv[v] = C[mfma16_row(0.0, lane, v) * EST];
+    d4 acc = 0.0;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], bv[s], acc, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) C[mfma16_row(0.0, lane, v) * EST] = cv[v] - acc[v];
+}
+
+__global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, const int* first, const double* g,
+                                                  double* x, Control* ctl) {
+    extern __shared__ double lds[];
+    const int W = env_window_rows(D);
+    double* win = lds;                                  // env_window_slots(D) blocks
+    double* xs = win + env_window_slots(D) * EBLK;      // right-hand side, then solution (Pp)
+    double* dv = xs + Pp;                               // 1 / L_ii (Pp)
+    int* mk = (int*)(dv + Pp);                          // blocks below the diagonal per block column
+    int* fst = mk + Pp / EB;                            // first[] staged
     __shared__ int sbad;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int nb = Pp / BB;
+    const int nb = Pp / EB;
+
     for (int i = tid; i < Pp; i += 256) xs[i] = g[i];
+    for (int i = tid; i < nb; i += 256) fst[i] = first[i];
     if (tid == 0) sbad = 0;
     __syncthreads();
-
-    constexpr int PER = PR * BB / 256;      // staged elements per thread (16)
-    for (int k = 0; k < nb; ++k) {
+    for (int k = tid; k < nb; k += 256) {
         int m = 0;
-        while (m < BMAXB && k + 1 + m < nb && first[k + 1 + m] <= k) ++m;
-        const int R = BB * (m + 1);
-        const long long r0 = (long long)BB * k;
-        {
-            // stage the panel: all loads in flight, then into LDS
-            double v[PER];
+        while (m < D && k + 1 + m < nb && fst[k + 1 + m] <= k) ++m;
+        mk[k] = m;
+    }
+    // rows 0 .. D of the window; every thread owns one element of each block
+    {
+        const int r = tid >> 4, c = tid & 15;
+        double v[(EDMAX + 1) * (EDMAX + 2) / 2];
+        int n = 0;
 #pragma unroll
-            for (int i = 0; i < PER; ++i) {
-                const int e = tid + 256 * i, r = e / BB, c = e % BB;
-                v[i] = (r < R) ? H[(r0 + r) * Pp + r0 + c] : 0.0;
-            }
+        for (int i = 0; i <= EDMAX; ++i)
 #pragma unroll
-            for (int i = 0; i < PER; ++i) {
-                const int e = tid + 256 * i, r = e / BB, c = e % BB;
-                S[r * FST + c] = v[i];
+            for (int j = 0; j <= i; ++j) {
+                const bool use = i <= D && i < nb && j >= fst[min(i, nb - 1)];
+                v[n++] = use ? H[(long long)(EB * i + r) * Pp + EB * j + c] : 0.0;
             }
+        n = 0;
+#pragma unroll
+        for (int i = 0; i <= EDMAX; ++i)
+#pragma unroll
+            for (int j = 0; j <= i; ++j) {
+                const bool use = i <= D && i < nb && j >= fst[min(i, nb - 1)];
+                if (use) env_block(win, W, i, j)[r * EST + c] = v[n];
+                ++n;
+            }
+    }
+    // wavefront 3 keeps the next block row in registers until it has a slot
+    double pre[EPRE];
+    auto prefetch = [&](int i) {
+        const int f0 = (i < nb) ? fst[i] : 0, nbl = (i < nb) ? i - f0 + 1 : 0;
+#pragma unroll
+        for (int t = 0; t < EPRE; ++t) {
+            const int e = lane + 64 * t, b = e >> 8, rr = (e >> 4) & 15, cc = e & 15;
+            pre[t] = (b < nbl) ? H[(long long)(EB * i + rr) * Pp + EB * (f0 + b) + cc] : 0.0;
         }
-        lds_barrier();
-        if (w == 0) {
-            // wavefront 0 factors the staged panel in place: lane l owns rows
-            // l and l + 64 (zero rows beyond R are harmless).  Column j is
-            // broadcast with v_readlane; the row updates are LDS
-            // read-modify-writes, so no register arrays are live.
-            double* rowA = S + lane * FST;
-            double* rowB = S + (lane + 64) * FST;
-            double bA = (lane < R) ? xs[r0 + lane] : 0.0;
-            double bB = (lane + 64 < R) ? xs[r0 + 64 + lane] : 0.0;
+    };
+    if (w == 3) prefetch(D + 1);
+    __syncthreads();
+
+    for (int k = 0; k < nb; ++k) {
+        const int m = mk[k];
+        const int nf = m > 3 ? (m + 2) / 3 : 1;         // factoring wavefronts
+        // ---------------- phase A
+        if (w < nf) {
+            const int bi = (lane < EB) ? 0 : 1 + 3 * w + ((lane - EB) >> 4);
+            const int rr = lane & 15;
+            const bool valid = bi <= m;
+            double a[EB];
+            {
+                const double* src = env_block(win, W, k + (valid ? bi : 0), k) + rr * EST;
+#pragma unroll
+                for (int c = 0; c < EB; ++c) a[c] = valid ? src[c] : 0.0;
+            }
+            double b = valid ? xs[EB * (k + bi) + rr] : 0.0;
             bool bad = false;
             double myinv = 0.0;
-#pragma unroll 1
-            for (int j = 0; j < BB; ++j) {
-                const double u = rowA[j];     // unscaled column-j entry (diagonal rows are lanes 0..31)
-                const double u2 = rowB[j];
+            int rl = lane;
+            asm volatile("" : "+v"(rl));
+#pragma unroll
+            for (int j = 0; j < EB; ++j) {
+                const double u = a[j];
                 double piv = readlane_d(u, j);
                 if (!(piv > 0.0) || !isfinite(piv)) {
                     bad = true;
                     piv = 1.0;
                 }
                 const double inv = rsqrt_fast(piv);
-                const double ljj = piv * inv;
-                const double yj = readlane_d(bA, j) * inv;
-                const double lA = (lane == j) ? ljj : u * inv;
-                const double lB = u2 * inv;
-                rowA[j] = lA;
-                rowB[j] = lB;
-                if (lane == j) {
+                const double yj = readlane_d(b, j) * inv;
+                const double l = (rl == j) ? piv * inv : u * inv;
+                a[j] = l;
+                if (rl == j) {
                     myinv = inv;
-                    bA = yj;
-                } else if (lane > j) {
-                    bA -= lA * yj;
+                    b = yj;
+                } else if (rl > j) {
+                    b -= l * yj;
                 }
-                bB -= lB * yj;
-                const double sA = lA * inv, sB = lB * inv;
-#pragma unroll 4
-                for (int q = j + 1; q < BB; ++q) {
-                    const double cq = readlane_d(u, q);
-                    rowA[q] -= sA * cq;
-                    rowB[q] -= sB * cq;
+                const double sc = l * inv;
+#pragma unroll
+                for (int q = j + 1; q < EB; ++q) a[q] -= sc * readlane_d(u, q);
+            }
+            if (valid) {
+                // the factor to global memory (upper triangle of the diagonal block zeroed)
+                double* dst = H + (long long)(EB * (k + bi) + rr) * Pp + EB * k;
+                if (bi == 0) {
+                    if (w == 0) {
+#pragma unroll
+                        for (int c = 0; c < EB; ++c) dst[c] = (c <= rr) ? a[c] : 0.0;
+                        dv[EB * k + rr] = myinv;
+                        xs[EB * k + rr] = b;
+                    }
+                } else {
+                    double* wl = env_block(win, W, k + bi, k) + rr * EST;
+#pragma unroll
+                    for (int c = 0; c < EB; ++c) {
+                        dst[c] = a[c];
+                        wl[c] = a[c];
+                    }
+                    xs[EB * (k + bi) + rr] = b;
                 }
             }
             if (bad && lane == 0) sbad = 1;
-            if (lane < BB) {
-                // zero the strict upper triangle of the diagonal block
-                for (int q = lane + 1; q < BB; ++q) rowA[q] = 0.0;
-                dv[r0 + lane] = myinv;
+        } else {
+            if (w == 3) {
+                // block row k + D + 1 into its slot (row k - 2's, free since the
+                // last phase A), then prefetch row k + D + 2
+                const int i = k + D + 1;
+                if (i < nb) {
+                    const int f0 = fst[i], nbl = i - f0 + 1;
+#pragma unroll
+                    for (int t = 0; t < EPRE; ++t) {
+                        const int e = lane + 64 * t, bb = e >> 8, rr = (e >> 4) & 15, cc = e & 15;
+                        if (bb < nbl) env_block(win, W, i, f0 + bb)[rr * EST + cc] = pre[t];
+                    }
+                    prefetch(i + 1);
+                }
             }
-            if (lane < R) xs[r0 + lane] = bA;
-            if (lane + 64 < R) xs[r0 + 64 + lane] = bB;
+            if (k >= 1) {
+                // rest of column k - 1's trailing update: tiles (kp+i, kp+j), 2 <= j <= i <= mp
+                const int kp = k - 1, mp = mk[kp];
+                const int nt = (mp - 1) * mp / 2;
+                for (int t = w - nf; t < nt; t += 4 - nf) {
+                    int i = 2;
+                    while ((i - 1) * i / 2 <= t) ++i;
+                    const int j = 2 + t - (i - 2) * (i - 1) / 2;
+                    env_tile(win, W, kp + i, kp + j, kp, lane);
+                }
+            }
         }
         lds_barrier();
-        // trailing update of the blocks (k+i, k+j), 1 <= j <= i <= m: four
-        // 16x16 MFMA tiles each; the tile is prefetched before the products
-        const int njobs = (m * (m + 1) / 2) * 4;
-        for (int job = w; job < njobs; job += 4) {
-            const int pr = job >> 2, sub = job & 3;
-            int bi = 0;
-            while ((bi + 1) * (bi + 2) / 2 <= pr) ++bi;
-            const int bj = pr - bi * (bi + 1) / 2;     // 0-based panel indices, bj <= bi
-            const int sr = (sub >> 1) * 16, sc = (sub & 1) * 16;
-            double* Ht[4];
-            double hv[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const long long r = (long long)BB * (k + 1 + bi) + sr + mfma16_row(0.0, lane, v);
-                const long long c = (long long)BB * (k + 1 + bj) + sc + (lane & 15);
-                Ht[v] = H + r * Pp + c;
-                hv[v] = *Ht[v];
-            }
-            const double* Fi = S + ((bi + 1) * BB + sr + (lane & 15)) * FST + (lane >> 4);
-            const double* Fj = S + ((bj + 1) * BB + sc + (lane & 15)) * FST + (lane >> 4);
-            d4 acc = 0.0;
-#pragma unroll
-            for (int kk = 0; kk < BB; kk += 4) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Fi[kk], Fj[kk], acc, 0, 0, 0);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) *Ht[v] = hv[v] - acc[v];
-        }
-        // the factor of block column k back to global memory
-        {
-#pragma unroll
-            for (int i = 0; i < PER; ++i) {
-                const int e = tid + 256 * i, r = e / BB, c = e % BB;
-                if (r < R) H[(r0 + r) * Pp + r0 + c] = S[r * FST + c];
-            }
-        }
-        __syncthreads();   // global: the updated blocks are staged by the next step
+        // ---------------- phase B: tiles (k + i, k + 1), 1 <= i <= m
+        for (int t = w; t < m; t += 4) env_tile(win, W, k + 1 + t, k + 1, k, lane);
+        lds_barrier();
     }
+    __syncthreads();   // the factor in global memory is complete
 
-    // backward solve L^T x = y from the bottom block row up: wavefronts 1..m
-    // form the panel contributions, wavefront 0 the diagonal solve
-    for (int k = nb - 1; k >= 0; --k) {
-        int m = 0;
-        while (m < BMAXB && k + 1 + m < nb && first[k + 1 + m] <= k) ++m;
-        const int c0 = BB * k;
-        if (w >= 1 && w <= m && lane < BB) {
-            const int rb = BB * (k + w);
-            double hv[BB];
+    // backward solve L^T x = y by wavefront 0: lane (c, g) sums rows g, g+4,
+    // g+8, g+12 of column c over the blocks below the diagonal; lanes 0-15
+    // then solve with the diagonal block.  The next block's factor is loaded
+    // while the current one is solved.
+    if (w == 0) {
+        const int c = lane & 15, g4 = lane >> 4;
+        double pl[2][4 * EDMAX], cl[2][EB];
+        auto load = [&](int k, double* P, double* Cc) {
+            const int m = (k >= 0) ? mk[k] : 0;
 #pragma unroll
-            for (int r = 0; r < BB; ++r) hv[r] = H[(long long)(rb + r) * Pp + c0 + lane];
-            double s0 = 0.0, s1 = 0.0;
+            for (int i = 1; i <= EDMAX; ++i)
 #pragma unroll
-            for (int r = 0; r < BB; r += 2) {
-                s0 += hv[r] * xs[rb + r];
-                s1 += hv[r + 1] * xs[rb + r + 1];
+                for (int t = 0; t < 4; ++t)
+                    P[4 * (i - 1) + t] =
+                        (i <= m) ? H[(long long)(EB * (k + i) + g4 + 4 * t) * Pp + EB * k + c] : 0.0;
+#pragma unroll
+            for (int r = 0; r < EB; ++r) Cc[r] = (k >= 0) ? H[(long long)(EB * k + r) * Pp + EB * k + c] : 0.0;
+        };
+        auto solve = [&](int k, const double* P, const double* Cc) {
+            const int m = mk[k];
+            double s = 0.0;
+#pragma unroll
+            for (int i = 1; i <= EDMAX; ++i) {
+                if (i <= m) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) s += P[4 * (i - 1) + t] * xs[EB * (k + i) + g4 + 4 * t];
+                }
             }
-            part[w - 1][lane] = s0 + s1;
-        }
-        double cl[BB];   // column `lane` of the diagonal factor, loaded while the panel sums run
-        if (w == 0 && lane < BB) {
-#pragma unroll
-            for (int r = 0; r < BB; ++r) cl[r] = H[(long long)(c0 + r) * Pp + c0 + lane];
-        }
-        lds_barrier();
-        if (w == 0 && lane < BB) {
-            double s = xs[c0 + lane];
-            for (int i = 0; i < m; ++i) s -= part[i][lane];
-            const double myinv = dv[c0 + lane];
+            s += __shfl_xor(s, 16, 64);
+            s += __shfl_xor(s, 32, 64);
+            double z = xs[EB * k + c] - s;
+            const double myinv = dv[EB * k + c];
             int rl = lane;
             asm volatile("" : "+v"(rl));
-#pragma clang loop unroll(full)
-            for (int r = BB - 1; r >= 0; --r) {
-                const double xr = readlane_d(s, r) * readlane_d(myinv, r);
-                if (rl == r) s = xr;
-                else if (rl < r) s -= cl[r] * xr;
+#pragma unroll
+            for (int r = EB - 1; r >= 0; --r) {
+                const double xr = readlane_d(z, r) * readlane_d(myinv, r);
+                if (rl == r) z = xr;
+                else if (rl < r) z -= Cc[r] * xr;
             }
-            xs[c0 + lane] = s;
+            if (lane < EB) xs[EB * k + c] = z;
+            wave_lds_order();
+        };
+        load(nb - 1, pl[0], cl[0]);
+        int k = nb - 1;
+        for (; k >= 1; k -= 2) {
+            load(k - 1, pl[1], cl[1]);
+            solve(k, pl[0], cl[0]);
+            load(k - 2, pl[0], cl[0]);
+            solve(k - 1, pl[1], cl[1]);
         }
-        lds_barrier();
+        if (k == 0) solve(0, pl[0], cl[0]);
     }
-    for (int i = tid; i < Pp; i += 256) {
-        x[i] = xs[i];
-        dinv[i] = dv[i];
-    }
+    __syncthreads();
+    for (int i = tid; i < Pp; i += 256) x[i] = xs[i];
     if (tid == 0 && sbad) ctl->chol_ok = 0;
 }
 
-size_t chol_band_lds(int);
+size_t chol_band_lds(int Pp, int D) {
+    const int nb = Pp / EB;
+    return (size_t)env_window_slots(D) * EBLK * sizeof(double) + (size_t)2 * Pp * sizeof(double) +
+           (size_t)2 * nb * sizeof(int);
+}
 
-void launch_chol_band(double* H, int Pp, const int* first, const double* g, double* x, double* dinv, Control* ctl,
+void launch_chol_band(double* H, int Pp, int D, const int* first, const double* g, double* x, Control* ctl,
                       hipStream_t s) {
-    hipLaunchKernelGGL(k_chol_band, dim3(1), dim3(256), chol_band_lds(Pp), s, H, Pp, first, g, x, dinv, ctl);
+    hipLaunchKernelGGL(k_chol_env, dim3(1), dim3(256), chol_band_lds(Pp, D), s, H, Pp, D, first, g, x, ctl);
 }
 
-int prepare_chol_band(int Pp) {
-    return (int)hipFuncSetAttribute((const void*)k_chol_band, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)chol_band_lds(Pp));
+int prepare_chol_band(int Pp, int D) {
+    return (int)hipFuncSetAttribute((const void*)k_chol_env, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)chol_band_lds(Pp, D));
 }
 
-int chol_band_max_blocks() { return BMAXB; }
-int chol_band_block() { return BB; }
-size_t chol_band_lds(int Pp) { return (size_t)2 * Pp * sizeof(double); }
+int chol_band_max_blocks() { return EDMAX; }
+int chol_band_block() { return EB; }
 
 // ---------------------------------------------------------------------------
 // L L^T x = g, one workgroup, right-hand side in LDS

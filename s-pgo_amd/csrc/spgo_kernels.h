@@ -24,12 +24,12 @@ void launch_form_system(const Layout& lay, const StateBuf<T>* bufs, Control* ctl
                         double* H, double* g, int Pp, hipStream_t s);
 
 void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t s);
-void launch_chol_band(double* H, int Pp, const int* first, const double* g, double* x, double* dinv, Control* ctl,
+void launch_chol_band(double* H, int Pp, int D, const int* first, const double* g, double* x, Control* ctl,
                       hipStream_t s);
 int chol_band_max_blocks();
-int prepare_chol_band(int Pp);
+int prepare_chol_band(int Pp, int D);
 int chol_band_block();
-size_t chol_band_lds(int Pp);
+size_t chol_band_lds(int Pp, int D);
 void launch_chol_solve(const double* H, const double* Linv, const double* g, double* delta, int Pp,
                        hipStream_t s);
 

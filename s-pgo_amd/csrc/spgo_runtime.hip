@@ -108,6 +108,7 @@ struct spgo_solver {
     int Pp = 0;
     bool band = false;          // envelope Cholesky in one workgroup (narrow profile)
     const int* first = nullptr; // first nonzero block column of every block row
+    int band_depth = 0;         // max over block rows of (row - first)
     Control* ctl = nullptr;
     IterRecord* rec = nullptr;
     int rec_cap = 0;
@@ -390,7 +391,7 @@ int setup_layout(spgo_solver* h) {
     if (rc) return rc;
     const int P = 6 * h->N;
     const int BBk = chol_band_block();
-    h->Pp = ((P + BBk - 1) / BBk) * BBk;
+    h->Pp = ((P + NB - 1) / NB) * NB;   // a whole number of blocks for either factorisation
     {
         // envelope of the reduced system: pose j couples back to the smallest
         // anchor among the groups covering it
@@ -417,12 +418,13 @@ int setup_layout(spgo_solver* h) {
             fb[i] = f;
             bw = std::max(bw, i - f);
         }
-        const bool fits = bw <= chol_band_max_blocks() && chol_band_lds(h->Pp) <= 160 * 1024 - 40 * 1024;
+        const bool fits = bw <= chol_band_max_blocks() && chol_band_lds(h->Pp, bw) <= 160 * 1024 - 1024;
         if (h->opt.factorization == SPGO_FACTOR_ENVELOPE && !fits)
             return h->fail(SPGO_E_LAYOUT, "reduced-system profile too wide for the envelope factorisation");
         h->band = fits && h->opt.factorization != SPGO_FACTOR_DENSE;
         if ((rc = upload(h, fb, &h->first))) return rc;
-        if (h->band && prepare_chol_band(h->Pp) != 0)
+        h->band_depth = bw;
+        if (h->band && prepare_chol_band(h->Pp, bw) != 0)
             return h->fail(SPGO_E_DEVICE, "cannot reserve LDS for the envelope Cholesky");
     }
     h->H = h->mem.alloc<double>((size_t)h->Pp * h->Pp);
@@ -483,7 +485,7 @@ void launch_iteration(spgo_solver* h, StateBuf<T>* b) {
     hipStream_t s = h->stream;
     launch_form_system<T>(h->lay, b, h->ctl, h->H, h->gvec, h->Pp, s);
     if (h->band) {
-        launch_chol_band(h->H, h->Pp, h->first, h->gvec, h->delta, h->Linv, h->ctl, s);
+        launch_chol_band(h->H, h->Pp, h->band_depth, h->first, h->gvec, h->delta, h->ctl, s);
     } else {
         launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
         launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, s);
@@ -1014,7 +1016,7 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
         else launch_form_system<double>(h->lay, h->bd, h->ctl, h->H, h->gvec, h->Pp, s);
         (void)hipEventRecord(ev[1], s);
         if (h->band) {
-            launch_chol_band(h->H, h->Pp, h->first, h->gvec, h->delta, h->Linv, h->ctl, s);
+            launch_chol_band(h->H, h->Pp, h->band_depth, h->first, h->gvec, h->delta, h->ctl, s);
             (void)hipEventRecord(ev[2], s);
         } else {
             launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);

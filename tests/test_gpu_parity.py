@@ -84,6 +84,19 @@ def test_medium_parity_huber_and_options(factor):
     _check_run(pa, s, 8, oo, p.X0.astype(np.float64), p.alpha0.astype(np.float64), "medium-huber")
 
 
+@pytest.mark.parametrize("n_poses,n_lm,obs,seed", [(20, 1500, 12, 41), (26, 2000, 16, 42)])
+def test_deep_profile_envelope(n_poses, n_lm, obs, seed):
+    # landmarks seen over 12 and 16 frames: the reduced system's profile is 4 to
+    # 7 blocks of 16 deep, so the envelope factorisation (forced here) runs the
+    # panel on two and three wavefronts
+    p = make_problem(n_poses, n_lm, obs, seed=seed)
+    pa = O.from_problem(p)
+    go, oo = _opts_pair(factorization=2)
+    s = solver_from_problem(p, go)
+    print(f"[deep-profile] poses={n_poses} span={s.layout_info()['max_span']}", flush=True)
+    _check_run(pa, s, 5, oo, p.X0.astype(np.float64), p.alpha0.astype(np.float64), f"deep-{obs}")
+
+
 def test_reference_mode_step_size_and_split_steps():
     p = make_problem(8, 400, 4, seed=5)
     pa = O.from_problem(p)
