@@ -253,21 +253,23 @@ void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t 
 //            0-15 the diagonal block, lanes 16-63 three blocks below it (a
 //            second or third wavefront repeats the diagonal block for its own
 //            three blocks, so no wavefront waits on another inside the column
-//            loop).  The right-hand side rides along, so the forward solve is
-//            done with the factorisation.  Meanwhile the other wavefronts apply
-//            the part of the previous column's trailing update that the next
-//            panel does not need, and wavefront 3 moves the block row it
-//            prefetched into the window and prefetches the next one.
+//            loop).  The right-hand side rides along as one more column, so
+//            the forward solve is done with the factorisation.  Meanwhile the
+//            other wavefronts apply the part of the previous column's
+//            trailing update that the next panel does not need, invert the
+//            previous diagonal factor (for the backward solve), and
+//            wavefront 3 moves the block row it prefetched into the window
+//            and prefetches the next one.
 //   phase B  the trailing update of block column k + 1 (the next panel) on
 //            MFMA, one 16x16 tile per wavefront.
 // The factor goes back to global memory as it is produced; the backward solve
-// then runs block by block from the bottom, its loads one block ahead.
+// x_k = L_kk^-T (y_k - sum_i L_(k+i),k^T x_(k+i)) then runs block by block from
+// the bottom as products with the stored inverses, its loads one block ahead.
 // ---------------------------------------------------------------------------
 constexpr int EB = 16;            // block size: one MFMA tile
 constexpr int EDMAX = 7;          // deepest profile: blocks below the diagonal
 constexpr int EST = EB + 1;       // LDS row stride of a window block
 constexpr int EBLK = EB * EST;    // doubles per window block
-constexpr int EPRE = 4 * (EDMAX + 1);   // prefetched doubles per lane of one block row
 
 // 1/sqrt(x) in double: the hardware estimate refined by one Halley step
 // (cubic convergence), a few FMAs instead of the library's long sequence.
@@ -290,25 +292,20 @@ __device__ __forceinline__ void lds_barrier() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-__host__ __device__ constexpr int env_window_rows(int D) { return D + 3; }
 __host__ __device__ constexpr int env_window_slots(int D) { return (D + 3) * (D + 4) / 2; }
 
-// window block (i, j), i >= j
-__device__ __forceinline__ double* env_block(double* win, int W, int i, int j) {
-    int a = i % W, b = j % W;
-    if (a < b) {
-        const int t = a;
-        a = b;
-        b = t;
-    }
-    return win + (a * (a + 1) / 2 + b) * EBLK;
+// window slot of block (i, j) from the residues a = i mod W, b = j mod W
+__device__ __forceinline__ double* env_slot(double* win, int a, int b) {
+    const int hi = a > b ? a : b, lo = a > b ? b : a;
+    return win + (hi * (hi + 1) / 2 + lo) * EBLK;
 }
 
-// C(bi, bj) -= L(bi, kp) L(bj, kp)^T on one 16x16 MFMA tile (one wavefront)
-__device__ __forceinline__ void env_tile(double* win, int W, int bi, int bj, int kp, int lane) {
-    const double* Ai = env_block(win, W, bi, kp) + (lane & 15) * EST + (lane >> 4);
-    const double* Aj = env_block(win, W, bj, kp) + (lane & 15) * EST + (lane >> 4);
-    double* C = env_block(win, W, bi, bj) + (lane & 15);
+// C(bi, bj) -= L(bi, kp) L(bj, kp)^T on one 16x16 MFMA tile (one wavefront);
+// arguments are window residues
+__device__ __forceinline__ void env_tile(double* win, int ri, int rj, int rk, int lane) {
+    const double* Ai = env_slot(win, ri, rk) + (lane & 15) * EST + (lane >> 4);
+    const double* Aj = env_slot(win, rj, rk) + (lane & 15) * EST + (lane >> 4);
+    double* C = env_slot(win, ri, rj) + (lane & 15);
     double av[4], bv[4], cv[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -324,18 +321,42 @@ __device__ __forceinline__ void env_tile(double* win, int W, int bi, int bj, int
     for (int v = 0; v < 4; ++v) C[mfma16_row(0.0, lane, v) * EST] = cv[v] - acc[v];
 }
 
+// Inverse of the lower-triangular diagonal factor held column-major (stride
+// EST) in LDS, with its inverse diagonal dinv: lane c < 16 forms column c of
+// L^-1 by forward substitution and stores it row-major to dst (16 x 16).
+__device__ __forceinline__ void env_invert(const double* Lc, const double* dinv, double* dst, int lane) {
+    const int c = lane & 15;
+    double s[EB];
+#pragma unroll
+    for (int r = 0; r < EB; ++r) s[r] = (r == c) ? 1.0 : 0.0;
+#pragma unroll
+    for (int q = 0; q < EB; ++q) {
+        s[q] *= dinv[q];
+#pragma unroll
+        for (int r = q + 1; r < EB; ++r) s[r] -= Lc[q * EST + r] * s[q];
+    }
+    if (lane < EB) {
+#pragma unroll
+        for (int r = 0; r < EB; ++r) dst[r * EB + c] = s[r];
+    }
+}
+
+template <int DT>
 __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, const int* first, const double* g,
-                                                  double* x, Control* ctl) {
+                                                  double* x, double* winv, Control* ctl) {
+    constexpr int EPRE = 4 * (DT + 1);      // prefetched doubles per lane of one block row
     extern __shared__ double lds[];
-    const int W = env_window_rows(D);
+    const int W = D + 3;
+    const int nb = Pp / EB;
     double* win = lds;                                  // env_window_slots(D) blocks
-    double* xs = win + env_window_slots(D) * EBLK;      // right-hand side, then solution (Pp)
+    double* ring = win + env_window_slots(D) * EBLK;    // last two diagonal factors, column-major
+    double* xs = ring + 2 * EBLK;                       // right-hand side, then solution (Pp)
     double* dv = xs + Pp;                               // 1 / L_ii (Pp)
     int* mk = (int*)(dv + Pp);                          // blocks below the diagonal per block column
-    int* fst = mk + Pp / EB;                            // first[] staged
+    int* fst = mk + nb;                                 // first[] staged
     __shared__ int sbad;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int nb = Pp / EB;
+    auto wrap = [W](int r) { return r >= W ? r - W : r; };
 
     for (int i = tid; i < Pp; i += 256) xs[i] = g[i];
     for (int i = tid; i < nb; i += 256) fst[i] = first[i];
@@ -349,10 +370,10 @@ __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, cons
     // rows 0 .. D of the window; every thread owns one element of each block
     {
         const int r = tid >> 4, c = tid & 15;
-        double v[(EDMAX + 1) * (EDMAX + 2) / 2];
+        double v[(DT + 1) * (DT + 2) / 2];
         int n = 0;
 #pragma unroll
-        for (int i = 0; i <= EDMAX; ++i)
+        for (int i = 0; i <= DT; ++i)
 #pragma unroll
             for (int j = 0; j <= i; ++j) {
                 const bool use = i <= D && i < nb && j >= fst[min(i, nb - 1)];
@@ -360,28 +381,32 @@ __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, cons
             }
         n = 0;
 #pragma unroll
-        for (int i = 0; i <= EDMAX; ++i)
+        for (int i = 0; i <= DT; ++i)
 #pragma unroll
             for (int j = 0; j <= i; ++j) {
                 const bool use = i <= D && i < nb && j >= fst[min(i, nb - 1)];
-                if (use) env_block(win, W, i, j)[r * EST + c] = v[n];
+                if (use) env_slot(win, i, j)[r * EST + c] = v[n];
                 ++n;
             }
     }
-    // wavefront 3 keeps the next block row in registers until it has a slot
+    // wavefront 3 keeps the next block row in registers until it has a slot.
+    // Every lane loads unconditionally (the block index is clamped to the row)
+    // and only the stores to the window are predicated.
     double pre[EPRE];
     auto prefetch = [&](int i) {
-        const int f0 = (i < nb) ? fst[i] : 0, nbl = (i < nb) ? i - f0 + 1 : 0;
+        const int ic = min(i, nb - 1);
+        const int f0 = fst[ic];
 #pragma unroll
         for (int t = 0; t < EPRE; ++t) {
-            const int e = lane + 64 * t, b = e >> 8, rr = (e >> 4) & 15, cc = e & 15;
-            pre[t] = (b < nbl) ? H[(long long)(EB * i + rr) * Pp + EB * (f0 + b) + cc] : 0.0;
+            const int e = lane + 64 * t, b = min(e >> 8, ic - f0), rr = (e >> 4) & 15, cc = e & 15;
+            pre[t] = H[(long long)(EB * ic + rr) * Pp + EB * (f0 + b) + cc];
         }
     };
     if (w == 3) prefetch(D + 1);
     __syncthreads();
 
-    for (int k = 0; k < nb; ++k) {
+    int rk = 0;   // k mod W
+    for (int k = 0; k < nb; ++k, rk = wrap(rk + 1)) {
         const int m = mk[k];
         const int nf = m > 3 ? (m + 2) / 3 : 1;         // factoring wavefronts
         // ---------------- phase A
@@ -389,36 +414,37 @@ __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, cons
             const int bi = (lane < EB) ? 0 : 1 + 3 * w + ((lane - EB) >> 4);
             const int rr = lane & 15;
             const bool valid = bi <= m;
+            const int bic = valid ? bi : 0;
             double a[EB];
             {
-                const double* src = env_block(win, W, k + (valid ? bi : 0), k) + rr * EST;
+                const double* src = env_slot(win, wrap(rk + bic), rk) + rr * EST;
 #pragma unroll
                 for (int c = 0; c < EB; ++c) a[c] = valid ? src[c] : 0.0;
             }
-            double b = valid ? xs[EB * (k + bi) + rr] : 0.0;
+            double b = valid ? xs[EB * (k + bic) + rr] : 0.0;
+            double yv = 0.0, iv = 0.0;
             bool bad = false;
-            double myinv = 0.0;
             int rl = lane;
             asm volatile("" : "+v"(rl));
 #pragma unroll
             for (int j = 0; j < EB; ++j) {
+                // column j: u holds A(:, j) after the updates of columns < j.
+                // For lane j, u * inv = sqrt(pivot) and sc = 1, so the update
+                // clobbers only entries of rows <= j that are already final
+                // (their y_j and 1/L_jj are kept in yv, iv).
                 const double u = a[j];
-                double piv = readlane_d(u, j);
-                if (!(piv > 0.0) || !isfinite(piv)) {
-                    bad = true;
-                    piv = 1.0;
-                }
+                const double piv = readlane_d(u, j);
+                bad |= !(piv > 0.0 && piv < __builtin_inf());
                 const double inv = rsqrt_fast(piv);
-                const double yj = readlane_d(b, j) * inv;
-                const double l = (rl == j) ? piv * inv : u * inv;
-                a[j] = l;
-                if (rl == j) {
-                    myinv = inv;
-                    b = yj;
-                } else if (rl > j) {
-                    b -= l * yj;
-                }
+                const double bj = readlane_d(b, j);
+                const double l = u * inv;
                 const double sc = l * inv;
+                if (rl == j) {
+                    yv = bj * inv;
+                    iv = inv;
+                }
+                a[j] = l;
+                b -= sc * bj;
 #pragma unroll
                 for (int q = j + 1; q < EB; ++q) a[q] -= sc * readlane_d(u, q);
             }
@@ -427,13 +453,18 @@ __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, cons
                 double* dst = H + (long long)(EB * (k + bi) + rr) * Pp + EB * k;
                 if (bi == 0) {
                     if (w == 0) {
+                        double* rg = ring + (k & 1) * EBLK + rr;
 #pragma unroll
-                        for (int c = 0; c < EB; ++c) dst[c] = (c <= rr) ? a[c] : 0.0;
-                        dv[EB * k + rr] = myinv;
-                        xs[EB * k + rr] = b;
+                        for (int c = 0; c < EB; ++c) {
+                            const double lv = (c <= rr) ? a[c] : 0.0;
+                            dst[c] = lv;
+                            rg[c * EST] = lv;
+                        }
+                        dv[EB * k + rr] = iv;
+                        xs[EB * k + rr] = yv;
                     }
                 } else {
-                    double* wl = env_block(win, W, k + bi, k) + rr * EST;
+                    double* wl = env_slot(win, wrap(rk + bi), rk) + rr * EST;
 #pragma unroll
                     for (int c = 0; c < EB; ++c) {
                         dst[c] = a[c];
@@ -449,57 +480,61 @@ __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, cons
                 // last phase A), then prefetch row k + D + 2
                 const int i = k + D + 1;
                 if (i < nb) {
-                    const int f0 = fst[i], nbl = i - f0 + 1;
+                    const int f0 = fst[i], nbl = i - f0 + 1, ri = wrap(rk + D + 1);
 #pragma unroll
                     for (int t = 0; t < EPRE; ++t) {
                         const int e = lane + 64 * t, bb = e >> 8, rr = (e >> 4) & 15, cc = e & 15;
-                        if (bb < nbl) env_block(win, W, i, f0 + bb)[rr * EST + cc] = pre[t];
+                        if (bb < nbl) env_slot(win, ri, wrap(rk + f0 + bb - k))[rr * EST + cc] = pre[t];
                     }
-                    prefetch(i + 1);
+                    if (i + 1 < nb) prefetch(i + 1);
                 }
             }
             if (k >= 1) {
-                // rest of column k - 1's trailing update: tiles (kp+i, kp+j), 2 <= j <= i <= mp
-                const int kp = k - 1, mp = mk[kp];
+                const int kp = k - 1, mp = mk[kp], rkp = rk == 0 ? W - 1 : rk - 1;
+                // the previous diagonal factor's inverse, for the backward solve
+                if (w == nf) env_invert(ring + (kp & 1) * EBLK, dv + EB * kp, winv + (long long)kp * EB * EB, lane);
+                // rest of column kp's trailing update: tiles (kp+i, kp+j), 2 <= j <= i <= mp
                 const int nt = (mp - 1) * mp / 2;
                 for (int t = w - nf; t < nt; t += 4 - nf) {
                     int i = 2;
                     while ((i - 1) * i / 2 <= t) ++i;
                     const int j = 2 + t - (i - 2) * (i - 1) / 2;
-                    env_tile(win, W, kp + i, kp + j, kp, lane);
+                    env_tile(win, wrap(rkp + i), wrap(rkp + j), rkp, lane);
                 }
             }
         }
         lds_barrier();
         // ---------------- phase B: tiles (k + i, k + 1), 1 <= i <= m
-        for (int t = w; t < m; t += 4) env_tile(win, W, k + 1 + t, k + 1, k, lane);
+        for (int t = w; t < m; t += 4) env_tile(win, wrap(rk + 1 + t), wrap(rk + 1), rk, lane);
         lds_barrier();
     }
-    __syncthreads();   // the factor in global memory is complete
+    if (w == 1) env_invert(ring + ((nb - 1) & 1) * EBLK, dv + EB * (nb - 1), winv + (long long)(nb - 1) * EB * EB, lane);
+    __syncthreads();   // the factor and the inverses in global memory are complete
 
-    // backward solve L^T x = y by wavefront 0: lane (c, g) sums rows g, g+4,
-    // g+8, g+12 of column c over the blocks below the diagonal; lanes 0-15
-    // then solve with the diagonal block.  The next block's factor is loaded
-    // while the current one is solved.
+    // backward solve by wavefront 0.  Lane (c, g) sums rows g, g+4, g+8, g+12
+    // of column c over the blocks below the diagonal; z = y_k - that sum goes
+    // through LDS so each lane can form its share of x_k = L_kk^-T z from its
+    // four entries of column c of L_kk^-1.  The next block's loads are issued
+    // before the current block is solved.
     if (w == 0) {
         const int c = lane & 15, g4 = lane >> 4;
-        double pl[2][4 * EDMAX], cl[2][EB];
-        auto load = [&](int k, double* P, double* Cc) {
-            const int m = (k >= 0) ? mk[k] : 0;
+        double pl[2][4 * DT], wl[2][4];
+        auto load = [&](int k, double* P, double* Wc) {
+            const int kc = max(k, 0);
+            const int m = mk[kc];
 #pragma unroll
-            for (int i = 1; i <= EDMAX; ++i)
+            for (int i = 1; i <= DT; ++i)
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
-                    P[4 * (i - 1) + t] =
-                        (i <= m) ? H[(long long)(EB * (k + i) + g4 + 4 * t) * Pp + EB * k + c] : 0.0;
+                    P[4 * (i - 1) + t] = H[(long long)(EB * (kc + min(i, m)) + g4 + 4 * t) * Pp + EB * kc + c];
 #pragma unroll
-            for (int r = 0; r < EB; ++r) Cc[r] = (k >= 0) ? H[(long long)(EB * k + r) * Pp + EB * k + c] : 0.0;
+            for (int t = 0; t < 4; ++t) Wc[t] = winv[(long long)kc * EB * EB + (g4 + 4 * t) * EB + c];
         };
-        auto solve = [&](int k, const double* P, const double* Cc) {
+        auto solve = [&](int k, const double* P, const double* Wc) {
             const int m = mk[k];
             double s = 0.0;
 #pragma unroll
-            for (int i = 1; i <= EDMAX; ++i) {
+            for (int i = 1; i <= DT; ++i) {
                 if (i <= m) {
 #pragma unroll
                     for (int t = 0; t < 4; ++t) s += P[4 * (i - 1) + t] * xs[EB * (k + i) + g4 + 4 * t];
@@ -507,28 +542,28 @@ __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, cons
             }
             s += __shfl_xor(s, 16, 64);
             s += __shfl_xor(s, 32, 64);
-            double z = xs[EB * k + c] - s;
-            const double myinv = dv[EB * k + c];
-            int rl = lane;
-            asm volatile("" : "+v"(rl));
-#pragma unroll
-            for (int r = EB - 1; r >= 0; --r) {
-                const double xr = readlane_d(z, r) * readlane_d(myinv, r);
-                if (rl == r) z = xr;
-                else if (rl < r) z -= Cc[r] * xr;
-            }
+            const double z = xs[EB * k + c] - s;
+            wave_lds_order();
             if (lane < EB) xs[EB * k + c] = z;
             wave_lds_order();
+            double xv = 0.0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) xv += Wc[t] * xs[EB * k + g4 + 4 * t];
+            xv += __shfl_xor(xv, 16, 64);
+            xv += __shfl_xor(xv, 32, 64);
+            wave_lds_order();
+            if (lane < EB) xs[EB * k + c] = xv;
+            wave_lds_order();
         };
-        load(nb - 1, pl[0], cl[0]);
+        load(nb - 1, pl[0], wl[0]);
         int k = nb - 1;
         for (; k >= 1; k -= 2) {
-            load(k - 1, pl[1], cl[1]);
-            solve(k, pl[0], cl[0]);
-            load(k - 2, pl[0], cl[0]);
-            solve(k - 1, pl[1], cl[1]);
+            load(k - 1, pl[1], wl[1]);
+            solve(k, pl[0], wl[0]);
+            load(k - 2, pl[0], wl[0]);
+            solve(k - 1, pl[1], wl[1]);
         }
-        if (k == 0) solve(0, pl[0], cl[0]);
+        if (k == 0) solve(0, pl[0], wl[0]);
     }
     __syncthreads();
     for (int i = tid; i < Pp; i += 256) x[i] = xs[i];
@@ -537,18 +572,24 @@ __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, cons
 
 size_t chol_band_lds(int Pp, int D) {
     const int nb = Pp / EB;
-    return (size_t)env_window_slots(D) * EBLK * sizeof(double) + (size_t)2 * Pp * sizeof(double) +
+    return (size_t)(env_window_slots(D) + 2) * EBLK * sizeof(double) + (size_t)2 * Pp * sizeof(double) +
            (size_t)2 * nb * sizeof(int);
 }
 
-void launch_chol_band(double* H, int Pp, int D, const int* first, const double* g, double* x, Control* ctl,
-                      hipStream_t s) {
-    hipLaunchKernelGGL(k_chol_env, dim3(1), dim3(256), chol_band_lds(Pp, D), s, H, Pp, D, first, g, x, ctl);
+void launch_chol_band(double* H, int Pp, int D, const int* first, const double* g, double* x, double* winv,
+                      Control* ctl, hipStream_t s) {
+    if (D <= 3)
+        hipLaunchKernelGGL(k_chol_env<3>, dim3(1), dim3(256), chol_band_lds(Pp, D), s, H, Pp, D, first, g, x, winv,
+                           ctl);
+    else
+        hipLaunchKernelGGL(k_chol_env<EDMAX>, dim3(1), dim3(256), chol_band_lds(Pp, D), s, H, Pp, D, first, g, x,
+                           winv, ctl);
 }
 
 int prepare_chol_band(int Pp, int D) {
-    return (int)hipFuncSetAttribute((const void*)k_chol_env, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)chol_band_lds(Pp, D));
+    const int bytes = (int)chol_band_lds(Pp, D);
+    const void* f = D <= 3 ? (const void*)k_chol_env<3> : (const void*)k_chol_env<EDMAX>;
+    return (int)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
 int chol_band_max_blocks() { return EDMAX; }

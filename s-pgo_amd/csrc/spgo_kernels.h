@@ -24,8 +24,8 @@ void launch_form_system(const Layout& lay, const StateBuf<T>* bufs, Control* ctl
                         double* H, double* g, int Pp, hipStream_t s);
 
 void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t s);
-void launch_chol_band(double* H, int Pp, int D, const int* first, const double* g, double* x, Control* ctl,
-                      hipStream_t s);
+void launch_chol_band(double* H, int Pp, int D, const int* first, const double* g, double* x, double* winv,
+                      Control* ctl, hipStream_t s);
 int chol_band_max_blocks();
 int prepare_chol_band(int Pp, int D);
 int chol_band_block();

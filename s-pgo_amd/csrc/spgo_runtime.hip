@@ -485,7 +485,7 @@ void launch_iteration(spgo_solver* h, StateBuf<T>* b) {
     hipStream_t s = h->stream;
     launch_form_system<T>(h->lay, b, h->ctl, h->H, h->gvec, h->Pp, s);
     if (h->band) {
-        launch_chol_band(h->H, h->Pp, h->band_depth, h->first, h->gvec, h->delta, h->ctl, s);
+        launch_chol_band(h->H, h->Pp, h->band_depth, h->first, h->gvec, h->delta, h->Linv, h->ctl, s);
     } else {
         launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
         launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, s);
@@ -1016,7 +1016,7 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
         else launch_form_system<double>(h->lay, h->bd, h->ctl, h->H, h->gvec, h->Pp, s);
         (void)hipEventRecord(ev[1], s);
         if (h->band) {
-            launch_chol_band(h->H, h->Pp, h->band_depth, h->first, h->gvec, h->delta, h->ctl, s);
+            launch_chol_band(h->H, h->Pp, h->band_depth, h->first, h->gvec, h->delta, h->Linv, h->ctl, s);
             (void)hipEventRecord(ev[2], s);
         } else {
             launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
