@@ -8,11 +8,14 @@ iteration: reduced-system formation, Cholesky, solves, pose update and the
 linearisation of the candidate with accept/reject -- all on the GPU, inputs
 resident in HBM.  Prints one JSON line.
 
-The roofline object describes the phase that takes the most device time,
-timed with HIP events on the solver's stream over the timed iterations
-(a second pass of the same iterations launched one phase at a time).  The CPU
-baseline is the FP64 oracle timed on this host on a bounded sample (two LM
-iterations of the same problem).
+The roofline object describes the kernel that takes the most device time,
+timed with HIP events on the solver's stream over the timed iterations (a
+second pass of the same iterations launched one phase at a time); the
+assembly kernel's HBM roofline is reported beside it.  HBM traffic per launch
+comes from the rocprofv3 counter summaries under profiles/ (FETCH_SIZE and
+WRITE_SIZE, scaled by the reading of a plain copy of known size; see
+--calibrate-copy).  The CPU baseline is the FP64 oracle timed on this host on
+a bounded sample (two LM iterations of the same problem).
 """
 from __future__ import annotations
 
@@ -57,9 +60,48 @@ def assembly_bytes(p, real_bytes: int = 8) -> int:
     return int(b)
 
 
-def cholesky_flops(n_poses: int) -> float:
+def scalar_profile(p) -> np.ndarray:
+    """First coupled column of every row of the reduced pose system: pose j
+    couples back to the first frame of every landmark whose observations span
+    it (the frames of one landmark are consecutive)."""
+    a, t, l = p.meas_a, p.meas_t, p.meas_l
+    lo = np.full(p.n_slots, 1 << 30)
+    hi = np.full(p.n_slots, -1)
+    np.minimum.at(lo, l, np.minimum(a, t))
+    np.maximum.at(hi, l, np.maximum(a, t))
+    first_pose = np.arange(p.n_poses)
+    for f0, f1 in zip(lo[hi >= 0], hi[hi >= 0]):
+        first_pose[f0:f1] = np.minimum(first_pose[f0:f1], f0)
+    return 6 * first_pose[np.arange(6 * p.n_poses) // 6]
+
+
+def envelope_flops(p) -> float:
+    """Algorithmic flops of the envelope Cholesky and both triangular solves:
+    column j with n_j entries below the diagonal inside the profile costs one
+    square root, n_j scalings and n_j (n_j + 1) / 2 multiply-adds of the
+    update; each solve costs 2 n_j + 1 per column."""
+    first = scalar_profile(p)
+    P = first.size
+    n = np.array([np.count_nonzero(first[j + 1:] <= j) for j in range(P)], dtype=np.float64)
+    return float(np.sum(1 + n + n * (n + 1)) + 2 * np.sum(2 * n + 1))
+
+
+def dense_flops(n_poses: int) -> float:
     P = 6 * n_poses
-    return P ** 3 / 3.0
+    return P ** 3 / 3.0 + 2 * P ** 2
+
+
+def profile_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    profiles/traffic_<kernel>.txt (a line 'traffic_bytes_per_launch: N')."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(p, n_iter: int, gpu_recs):
@@ -93,7 +135,18 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--chunk-landmarks", type=int, default=0)
     ap.add_argument("--group-chunks", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device copy of 256 MiB five times (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8}))
+        return 0
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the single-problem LM step is measured on one MI355X)", file=sys.stderr)
@@ -126,17 +179,28 @@ def main() -> int:
     # second pass: the same iterations, one launch per phase, HIP events per phase
     s.set_state(X0, p.alpha0.astype(np.float64))
     phases = s.profile_phases(args.steps)
+    info = s.layout_info()
+    envelope = info["factorization"] == 2
+    if envelope:
+        # one kernel factors and solves; the "solve" phase holds no kernel
+        phases["cholesky"] += phases.pop("solve")
+    nbytes = assembly_bytes(p)
+    ach_a = nbytes / (phases["assemble"] * 1e-3) / 1e9
+    roof_asm = {"kernel": "k_assemble<double>", "bound": "hbm", "achieved": ach_a, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach_a / HBM_PEAK_GBS, "traffic": profile_traffic("k_assemble"),
+                "algorithmic_bytes": nbytes}
     dom = max(phases, key=phases.get)
     if dom == "cholesky":
-        flops = cholesky_flops(p.n_poses)
+        flops = envelope_flops(p) if envelope else dense_flops(p.n_poses)
         ach = flops / (phases[dom] * 1e-3) / 1e12
-        roof = {"kernel": "cholesky (k_chol_panel + k_chol_update)", "bound": "mfma", "achieved": ach,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None}
+        name = "k_chol_env" if envelope else "k_chol_panel + k_chol_update + k_chol_solve"
+        roof = {"kernel": name, "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / FP64_PEAK_TFLOPS, "traffic": profile_traffic("k_chol_env") if envelope else None,
+                "algorithmic_flops": flops,
+                "note": "one workgroup walks the reduced system's profile column by column: latency-bound "
+                        "(the sequential column chain), FP64 peak is the whole chip's"}
     else:
-        nbytes = assembly_bytes(p)
-        ach = nbytes / (phases["assemble"] * 1e-3) / 1e9
-        roof = {"kernel": "k_assemble<double>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": nbytes}
+        roof = dict(roof_asm)
         if dom != "assemble":
             roof["note"] = f"dominant phase is {dom}; roofline given for the assembly kernel"
 
@@ -171,6 +235,8 @@ def main() -> int:
         "oracle_rel_err": err,
         "phases_ms": phases,
         "roofline": roof,
+        "roofline_assemble": roof_asm,
+        "factorization": "envelope" if envelope else "dense",
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -81,6 +81,10 @@ typedef struct spgo_iter_stats {
 typedef struct spgo_layout_info {
     int32_t n_poses, n_slots, n_measurements, n_landmarks;
     int32_t n_groups, n_chunks, max_span, n_anchors_used;
+    int32_t factorization;   /* SPGO_FACTOR_DENSE or SPGO_FACTOR_ENVELOPE: the path spgo_step takes */
+    int32_t profile_depth;   /* blocks of 16 below the diagonal in the reduced system's profile */
+    int32_t reduced_size;    /* rows of the padded reduced pose system                          */
+    int32_t reserved;
 } spgo_layout_info;
 
 typedef struct spgo_solver spgo_solver;

@@ -254,15 +254,16 @@ void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t 
 //            second or third wavefront repeats the diagonal block for its own
 //            three blocks, so no wavefront waits on another inside the column
 //            loop).  The right-hand side rides along as one more column, so
-//            the forward solve is done with the factorisation.  Meanwhile the
-//            other wavefronts apply the part of the previous column's
-//            trailing update that the next panel does not need, invert the
-//            previous diagonal factor (for the backward solve), and
-//            wavefront 3 moves the block row it prefetched into the window
-//            and prefetches the next one.
+//            the forward solve is done with the factorisation.  The panel is
+//            written to LDS only.  Meanwhile the other wavefronts apply the
+//            part of the previous column's trailing update that the next
+//            panel does not need, and wavefront 3 moves the block row it
+//            prefetched into the window, inverts the previous diagonal
+//            factor, copies the previous panel to global memory (both for
+//            the backward solve) and prefetches the next block row.
 //   phase B  the trailing update of block column k + 1 (the next panel) on
 //            MFMA, one 16x16 tile per wavefront.
-// The factor goes back to global memory as it is produced; the backward solve
+// The backward solve
 // x_k = L_kk^-T (y_k - sum_i L_(k+i),k^T x_(k+i)) then runs block by block from
 // the bottom as products with the stored inverses, its loads one block ahead.
 // ---------------------------------------------------------------------------
@@ -448,51 +449,40 @@ __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, cons
 #pragma unroll
                 for (int q = j + 1; q < EB; ++q) a[q] -= sc * readlane_d(u, q);
             }
+            // the panel stays in LDS (the diagonal factor column-major in the
+            // ring); wavefront 3 copies it to global memory in the next phase A,
+            // so this wavefront never waits on global stores
             if (valid) {
-                // the factor to global memory (upper triangle of the diagonal block zeroed)
-                double* dst = H + (long long)(EB * (k + bi) + rr) * Pp + EB * k;
                 if (bi == 0) {
                     if (w == 0) {
                         double* rg = ring + (k & 1) * EBLK + rr;
 #pragma unroll
-                        for (int c = 0; c < EB; ++c) {
-                            const double lv = (c <= rr) ? a[c] : 0.0;
-                            dst[c] = lv;
-                            rg[c * EST] = lv;
-                        }
+                        for (int c = 0; c < EB; ++c) rg[c * EST] = (c <= rr) ? a[c] : 0.0;
                         dv[EB * k + rr] = iv;
                         xs[EB * k + rr] = yv;
                     }
                 } else {
                     double* wl = env_slot(win, wrap(rk + bi), rk) + rr * EST;
 #pragma unroll
-                    for (int c = 0; c < EB; ++c) {
-                        dst[c] = a[c];
-                        wl[c] = a[c];
-                    }
+                    for (int c = 0; c < EB; ++c) wl[c] = a[c];
                     xs[EB * (k + bi) + rr] = b;
                 }
             }
             if (bad && lane == 0) sbad = 1;
         } else {
-            if (w == 3) {
-                // block row k + D + 1 into its slot (row k - 2's, free since the
-                // last phase A), then prefetch row k + D + 2
-                const int i = k + D + 1;
-                if (i < nb) {
-                    const int f0 = fst[i], nbl = i - f0 + 1, ri = wrap(rk + D + 1);
+            // wavefront 3: the prefetched block row k + D + 1 into its slot (row
+            // k - 2's, free since the last phase A)
+            const int inew = k + D + 1;
+            if (w == 3 && inew < nb) {
+                const int f0 = fst[inew], nbl = inew - f0 + 1, ri = wrap(rk + D + 1);
 #pragma unroll
-                    for (int t = 0; t < EPRE; ++t) {
-                        const int e = lane + 64 * t, bb = e >> 8, rr = (e >> 4) & 15, cc = e & 15;
-                        if (bb < nbl) env_slot(win, ri, wrap(rk + f0 + bb - k))[rr * EST + cc] = pre[t];
-                    }
-                    if (i + 1 < nb) prefetch(i + 1);
+                for (int t = 0; t < EPRE; ++t) {
+                    const int e = lane + 64 * t, bb = e >> 8, rr = (e >> 4) & 15, cc = e & 15;
+                    if (bb < nbl) env_slot(win, ri, wrap(rk + f0 + bb - k))[rr * EST + cc] = pre[t];
                 }
             }
             if (k >= 1) {
                 const int kp = k - 1, mp = mk[kp], rkp = rk == 0 ? W - 1 : rk - 1;
-                // the previous diagonal factor's inverse, for the backward solve
-                if (w == nf) env_invert(ring + (kp & 1) * EBLK, dv + EB * kp, winv + (long long)kp * EB * EB, lane);
                 // rest of column kp's trailing update: tiles (kp+i, kp+j), 2 <= j <= i <= mp
                 const int nt = (mp - 1) * mp / 2;
                 for (int t = w - nf; t < nt; t += 4 - nf) {
@@ -501,14 +491,31 @@ __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, cons
                     const int j = 2 + t - (i - 2) * (i - 1) / 2;
                     env_tile(win, wrap(rkp + i), wrap(rkp + j), rkp, lane);
                 }
+                if (w == 3) {
+                    // panel kp to global memory for the backward solve: the
+                    // inverse of its diagonal factor and the blocks below it
+                    env_invert(ring + (kp & 1) * EBLK, dv + EB * kp, winv + (long long)kp * EB * EB, lane);
+                    for (int i = 1; i <= mp; ++i) {
+                        const double* src = env_slot(win, wrap(rkp + i), rkp);
+                        double* dst = H + (long long)EB * (kp + i) * Pp + EB * kp;
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            const int e = lane + 64 * t, rr = e >> 4, cc = e & 15;
+                            dst[(long long)rr * Pp + cc] = src[rr * EST + cc];
+                        }
+                    }
+                }
             }
+            // wavefront 3: prefetch block row k + D + 2 (global stores and
+            // loads are issued last, so no wavefront waits on them in this phase)
+            if (w == 3 && inew + 1 < nb) prefetch(inew + 1);
         }
         lds_barrier();
         // ---------------- phase B: tiles (k + i, k + 1), 1 <= i <= m
         for (int t = w; t < m; t += 4) env_tile(win, wrap(rk + 1 + t), wrap(rk + 1), rk, lane);
         lds_barrier();
     }
-    if (w == 1) env_invert(ring + ((nb - 1) & 1) * EBLK, dv + EB * (nb - 1), winv + (long long)(nb - 1) * EB * EB, lane);
+    if (w == 3) env_invert(ring + ((nb - 1) & 1) * EBLK, dv + EB * (nb - 1), winv + (long long)(nb - 1) * EB * EB, lane);
     __syncthreads();   // the factor and the inverses in global memory are complete
 
     // backward solve by wavefront 0.  Lane (c, g) sums rows g, g+4, g+8, g+12

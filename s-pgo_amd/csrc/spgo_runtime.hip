@@ -303,6 +303,36 @@ int build_layout(spgo_solver* h, HostLayout& out) {
     return SPGO_OK;
 }
 
+// Size of the reduced system with padding: a whole number of blocks for
+// either factorisation.
+static int reduced_size(int n_poses) { return ((6 * n_poses + NB - 1) / NB) * NB; }
+
+// Block profile of the reduced system for the envelope factorisation: pose j
+// couples back to the smallest anchor among the landmark groups covering it,
+// fb[i] is the first nonzero block column of block row i and depth the
+// largest i - fb[i].  Returns whether the envelope path is used.
+static bool use_envelope(const spgo_solver* h, const HostLayout& L, std::vector<int>& fb, int& depth) {
+    const int P = 6 * h->N, Pp = reduced_size(h->N), BBk = chol_band_block();
+    std::vector<int> fpose(h->N);
+    for (int j = 0; j < h->N; ++j) fpose[j] = j;
+    for (size_t g = 0; g < L.grp_anchor.size(); ++g)
+        for (int k = 0; k < L.grp_K[g]; ++k) {
+            const int j = L.grp_anchor[g] + k;
+            if (j < h->N) fpose[j] = std::min(fpose[j], L.grp_anchor[g]);
+        }
+    const int nbk = Pp / BBk;
+    fb.assign(nbk, 0);
+    depth = 0;
+    for (int i = 0; i < nbk; ++i) {
+        int f = i;
+        for (int r = i * BBk; r < (i + 1) * BBk && r < P; ++r) f = std::min(f, (6 * fpose[r / 6]) / BBk);
+        fb[i] = f;
+        depth = std::max(depth, i - f);
+    }
+    const bool fits = depth <= chol_band_max_blocks() && chol_band_lds(Pp, depth) <= 160 * 1024 - 1024;
+    return fits && h->opt.factorization != SPGO_FACTOR_DENSE;
+}
+
 template <typename T>
 int alloc_state(spgo_solver* h, StateBuf<T>* b, const HostLayout& hl, int n_act) {
     for (int i = 0; i < 2; ++i) {
@@ -390,18 +420,8 @@ int setup_layout(spgo_solver* h) {
         rc = alloc_state<double>(h, h->bd, L, lay.n_act);
     if (rc) return rc;
     const int P = 6 * h->N;
-    const int BBk = chol_band_block();
-    h->Pp = ((P + NB - 1) / NB) * NB;   // a whole number of blocks for either factorisation
+    h->Pp = reduced_size(h->N);
     {
-        // envelope of the reduced system: pose j couples back to the smallest
-        // anchor among the groups covering it
-        std::vector<int> fpose(h->N);
-        for (int j = 0; j < h->N; ++j) fpose[j] = j;
-        for (size_t g = 0; g < L.grp_anchor.size(); ++g)
-            for (int k = 0; k < L.grp_K[g]; ++k) {
-                const int j = L.grp_anchor[g] + k;
-                if (j < h->N) fpose[j] = std::min(fpose[j], L.grp_anchor[g]);
-            }
         // k_form_system stages the groups of the anchors that can reach a pose row
         for (int j = 0; j < h->N; ++j) {
             const int amin = std::max(0, j - L.kmax + 1);
@@ -409,19 +429,11 @@ int setup_layout(spgo_solver* h) {
                 return h->fail(SPGO_E_LAYOUT, "too many landmark groups reach pose " + std::to_string(j) +
                                                   "; raise chunk_landmarks or group_chunks");
         }
-        const int nbk = h->Pp / BBk;
-        std::vector<int> fb(nbk);
+        std::vector<int> fb;
         int bw = 0;
-        for (int i = 0; i < nbk; ++i) {
-            int f = i;
-            for (int r = i * BBk; r < (i + 1) * BBk && r < P; ++r) f = std::min(f, (6 * fpose[r / 6]) / BBk);
-            fb[i] = f;
-            bw = std::max(bw, i - f);
-        }
-        const bool fits = bw <= chol_band_max_blocks() && chol_band_lds(h->Pp, bw) <= 160 * 1024 - 1024;
-        if (h->opt.factorization == SPGO_FACTOR_ENVELOPE && !fits)
+        h->band = use_envelope(h, L, fb, bw);
+        if (h->opt.factorization == SPGO_FACTOR_ENVELOPE && !h->band)
             return h->fail(SPGO_E_LAYOUT, "reduced-system profile too wide for the envelope factorisation");
-        h->band = fits && h->opt.factorization != SPGO_FACTOR_DENSE;
         if ((rc = upload(h, fb, &h->first))) return rc;
         h->band_depth = bw;
         if (h->band && prepare_chol_band(h->Pp, bw) != 0)
@@ -1066,6 +1078,11 @@ int spgo_problem_info(spgo_solver* h, spgo_layout_info* out) {
     out->n_chunks = (int)hl.chunk.size();
     out->max_span = hl.grp_K.empty() ? 0 : *std::max_element(hl.grp_K.begin(), hl.grp_K.end());
     out->n_anchors_used = hl.anchors_used;
+    std::vector<int> fb;
+    int depth = 0;
+    out->factorization = use_envelope(h, hl, fb, depth) ? SPGO_FACTOR_ENVELOPE : SPGO_FACTOR_DENSE;
+    out->profile_depth = depth;
+    out->reduced_size = reduced_size(h->N);
     return SPGO_OK;
 }
 

@@ -46,7 +46,8 @@ class IterStats(ctypes.Structure):
 class LayoutInfo(ctypes.Structure):
     _fields_ = [("n_poses", c_int32), ("n_slots", c_int32), ("n_measurements", c_int32),
                 ("n_landmarks", c_int32), ("n_groups", c_int32), ("n_chunks", c_int32),
-                ("max_span", c_int32), ("n_anchors_used", c_int32)]
+                ("max_span", c_int32), ("n_anchors_used", c_int32), ("factorization", c_int32),
+                ("profile_depth", c_int32), ("reduced_size", c_int32), ("reserved", c_int32)]
 
 
 F32P = POINTER(c_float)

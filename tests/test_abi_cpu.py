@@ -98,6 +98,36 @@ def test_layout_of_generated_problem():
     per_anchor = np.bincount(p.meas_a[p.meas_t == p.meas_a])
     assert info["n_groups"] == int(np.sum((per_anchor + 63) // 64))
     assert info["n_anchors_used"] == int(np.count_nonzero(per_anchor))
+    # 6 x 10 pose unknowns padded to whole 32-row blocks; the profile of a
+    # landmark span of 3 frames fits the envelope factorisation
+    assert info["reduced_size"] == 64
+    assert info["factorization"] == _lib.SPGO_FACTOR_ENVELOPE
+    s2 = Solver(p.n_poses, p.m_per_frame, make_options(factorization=_lib.SPGO_FACTOR_DENSE))
+    s2.write_observations(p.meas_a, p.meas_t, p.meas_l, p.uv)
+    assert s2.layout_info()["factorization"] == _lib.SPGO_FACTOR_DENSE
+
+
+def test_envelope_profile_depth_of_the_64_keyframe_problem():
+    # 8 observations per landmark over consecutive frames: pose rows couple 6
+    # poses back (36 scalars), at most 3 blocks of 16 below the diagonal
+    from spgo.problems import make_config
+    p = make_config("c2_fp64")
+    s = Solver(p.n_poses, p.m_per_frame)
+    s.write_observations(p.meas_a, p.meas_t, p.meas_l, p.uv)
+    info = s.layout_info()
+    a, t, l = p.meas_a, p.meas_t, p.meas_l
+    lo = np.full(p.n_slots, 1 << 30)
+    hi = np.full(p.n_slots, -1)
+    np.minimum.at(lo, l, np.minimum(a, t))
+    np.maximum.at(hi, l, np.maximum(a, t))
+    first_pose = np.arange(p.n_poses)
+    for f0, f1 in zip(lo[hi >= 0], hi[hi >= 0]):
+        first_pose[f0:f1] = np.minimum(first_pose[f0:f1], f0)
+    rows = np.arange(6 * p.n_poses)
+    first_row = 6 * first_pose[rows // 6]
+    depth = max(b - first_row[16 * b:16 * b + 16].min() // 16 for b in range(6 * p.n_poses // 16))
+    assert info["profile_depth"] == depth == 3
+    assert info["factorization"] == _lib.SPGO_FACTOR_ENVELOPE and info["reduced_size"] == 384
 
 
 def test_layout_rejects_mixed_anchors_and_wide_spans():
