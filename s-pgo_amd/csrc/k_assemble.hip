@@ -24,12 +24,22 @@ namespace spgo {
 
 namespace {
 
-constexpr int VS = 66;                 // LDS row stride of the V tile (doubles)
-constexpr int VR = 6 * KMAX;           // rows of the V tile (96)
-constexpr int SCR = 27;                // per-lane stride of the pair scratch
+constexpr int VS = 66;                 // LDS row stride of the V tile (landmarks + padding)
 constexpr int NWAVE = WG / 64;
-constexpr int VSH = (VR * VR > NWAVE * 64 * SCR) ? VR * VR : NWAVE * 64 * SCR;   // >= VR * VS
-constexpr int MAXTILE = ((VR / 16) * (VR / 16 + 1) / 2 + NWAVE - 1) / NWAVE;   // tiles per wave
+constexpr int SCRW = 32;               // pair-sum scratch rows per wavefront
+
+// LDS shapes for groups whose pose span is at most KT
+template <int KT>
+struct AsmShape {
+    static constexpr int VROWS = 16 * ((6 * KT + 16) / 16);   // B columns + the gradient row, whole tiles
+    static constexpr int NTILE = (VROWS / 16) * (VROWS / 16 + 1) / 2;
+    static constexpr int MAXTILE = (NTILE + NWAVE - 1) / NWAVE;   // MFMA tiles per wavefront
+    static constexpr int A0 = VROWS * VS, A1 = NWAVE * SCRW * 27, A2 = (6 * KT) * (6 * KT);
+    static constexpr int VSH = A0 > A1 ? (A0 > A2 ? A0 : A2) : (A1 > A2 ? A1 : A2);
+};
+
+// keep the compiler from reordering one wavefront's LDS accesses
+__device__ __forceinline__ void wave_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
 
 template <typename T> struct Vec4;
 template <> struct Vec4<double> { typedef double __attribute__((ext_vector_type(4))) type; };
@@ -69,8 +79,25 @@ __device__ __forceinline__ void robust_eval(const Robust& rb, T s, T kappa, T& w
 
 }  // namespace
 
-template <typename T>
-__global__ __launch_bounds__(WG) void k_assemble(AsmArgs A, StateBuf<T> B0, StateBuf<T> B1) {
+// One wavefront's measurement: ids and pixels, loaded ahead of use.
+struct MeasIn {
+    int t;
+    uint32_t meta;
+    float4 uv;
+};
+
+__device__ __forceinline__ MeasIn meas_load(const Layout& lay, int ob0, int o, bool valid) {
+    MeasIn m;
+    const int oc = valid ? ob0 + o : ob0;
+    m.t = lay.ob_t[oc];
+    m.meta = lay.ob_meta[oc];
+    m.uv = lay.ob_uv[oc];
+    return m;
+}
+
+template <typename T, int KT>
+__global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, StateBuf<T> B0, StateBuf<T> B1) {
+    using SH = AsmShape<KT>;
     const Layout& lay = A.lay;
     const int g = blockIdx.x;
     const int tid = threadIdx.x;
@@ -86,24 +113,23 @@ __global__ __launch_bounds__(WG) void k_assemble(AsmArgs A, StateBuf<T> B0, Stat
     const int a = lay.grp_anchor[g];
     const int K = lay.grp_K[g];
     const int K6 = 6 * K;
-    const int nT = (K6 + 15) >> 4;
+    const int nT = (K6 + 16) >> 4;             // tile rows: B columns and the gradient row K6
     const int nTiles = nT * (nT + 1) / 2;
 
-    __shared__ T S_w[NWAVE][KMAX + 1][27];      // per-wave pair partials, slot = t - a
-    __shared__ T Trel[2][KMAX + 1][12];         // (G^c_{a+s})^-1 G_a of the state being linearised
-    __shared__ T dl[KMAX][6];                   // Ad(G_a^-1 G_{a+k}) delta_{a+k} (back-substitution)
+    __shared__ T S[KT + 1][27];                 // pair sums per slot t - a
+    __shared__ T Trel[2][KT + 1][12];           // (G^c_{a+s})^-1 G_a of the state being linearised
+    __shared__ T dl[KT][6];                     // Ad(G_a^-1 G_{a+k}) delta_{a+k} (back-substitution)
     __shared__ T stage[CHUNK_OBS][8];           // per measurement: J_a^T j_alpha (6), j_alpha^2, j_alpha r
     __shared__ int stslot[CHUNK_OBS];
-    __shared__ T Vsh[VSH];
-    __shared__ int sslot[NWAVE][64];
-    __shared__ T gt[CHUNK_LM];
+    __shared__ T Vsh[SH::VSH];                  // B columns in / pair scratch / V tile / D
     __shared__ T lmP[CHUNK_LM][6];              // P_a and -P_a/alpha in the anchor frame
+    __shared__ int2 lpos[CHUNK_LM];
+    __shared__ T dly[6 * KT];
     __shared__ double red[NWAVE];
-    __shared__ double Adk[KMAX][36];            // Ad(G_a^-1 G_{a+k}) of the state being linearised
+    __shared__ double Adk[KT][36];              // Ad(G_a^-1 G_{a+k}) of the state being linearised
 
     // ---- group set-up ------------------------------------------------------
-    for (int i = tid; i < NWAVE * (KMAX + 1) * 27; i += WG) (&S_w[0][0][0])[i] = (T)0;
-    for (int i = tid; i < VR * VS; i += WG) Vsh[i] = (T)0;
+    for (int i = tid; i < (KT + 1) * 27; i += WG) (&S[0][0])[i] = (T)0;
     if (tid < 2 * (K + 1)) {
         const int c = tid / (K + 1), s = tid % (K + 1);
         double Ga[12], Gti[12], Tr[12];
@@ -121,43 +147,54 @@ __global__ __launch_bounds__(WG) void k_assemble(AsmArgs A, StateBuf<T> B0, Stat
         pose_mul(Gai, Gi, E);
         for (int e = 0; e < 36; ++e) Adk[k][e] = adjoint_entry(E, e / 6, e % 6);
     }
-    if (update && tid < K) {
+    if (update && tid >= 128 && tid < 128 + K) {
+        const int k = tid - 128;
         double Gai[12], Gi[12], E[12], x[6], out[6];
         pose_load(bin.GinvL + 12 * a, Gai);
-        pose_load(bin.G + 12 * (a + tid), Gi);
+        pose_load(bin.G + 12 * (a + k), Gi);
         pose_mul(Gai, Gi, E);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) x[q] = A.delta[6 * (a + tid) + q];
+        for (int q = 0; q < 6; ++q) x[q] = A.delta[6 * (a + k) + q];
         adjoint_apply(E, x, out);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) dl[tid][q] = (T)out[q];
+        for (int q = 0; q < 6; ++q) dl[k][q] = (T)out[q];
     }
 
-    typename Vec4<T>::type acc[MAXTILE];
+    typename Vec4<T>::type acc[SH::MAXTILE];
 #pragma unroll
-    for (int i = 0; i < MAXTILE; ++i) acc[i] = (T)0;
-    T yr = (T)0;
+    for (int i = 0; i < SH::MAXTILE; ++i) acc[i] = (T)0;
     double cost = 0.0;
 
     const T fx0 = (T)A.cam.fx[0], fy0 = (T)A.cam.fy[0], cx0 = (T)A.cam.cx[0], cy0 = (T)A.cam.cy[0];
     const int2 cr = lay.grp_chunks[g];
     const int glm0 = lay.grp_lm_begin[g];
     const long long voff = lay.grp_v_off[g];
-    __syncthreads();
 
     for (int ci = cr.x; ci < cr.y; ++ci) {
         const int4 ch = lay.chunk[ci];
-        const int lm0 = ch.x, nl = ch.y, ob0 = ch.z, nob = ch.w;
-
-        // ---- phase 0: per landmark: back-substitution and anchor-frame point ----
+        const int lm0 = ch.x, nl = ch.y, ob0 = ch.z;
+        const int* soff = lay.chunk_soff + (long long)ci * (KMAX + 2);
+        // this wavefront's first slot segment is loaded now and evaluated after phase 0
+        MeasIn pre;
+        {
+            const int sb = wv <= K ? soff[wv] : 0, se = wv <= K ? soff[wv + 1] : 0;
+            pre = meas_load(lay, ob0, sb + lane, sb + lane < se);
+        }
+        // ---- phase 0: back-substitution and anchor-frame point per landmark ----
+        // the chunk's B columns (nl x 6K, contiguous) into LDS with coalesced loads
+        const T* vin = bin.v + voff + (long long)(lm0 - glm0) * K6;
+        if (update && chol_ok)
+            for (int i = tid; i < nl * K6; i += WG) Vsh[i] = vin[i];
+        __syncthreads();   // also orders the set-up above
         if (tid < nl) {
             const int act = lm0 + tid;
             const int id = lay.lm_id[act];
+            lpos[tid] = lay.lm_pos[act];
             double al = bin.alpha[id];
             if (update) {
                 double dal = 0.0;
                 if (chol_ok) {
-                    const T* v = bin.v + voff + (long long)(act - glm0) * K6;
+                    const T* v = Vsh + tid * K6;
                     double s = (double)bin.g[act];
                     for (int k = 0; k < K; ++k)
 #pragma unroll
@@ -180,157 +217,160 @@ __global__ __launch_bounds__(WG) void k_assemble(AsmArgs A, StateBuf<T> B0, Stat
         }
         __syncthreads();
 
-        // ---- phase 1: per measurement ----------------------------------------
-        for (int base = 0; base < nob; base += WG) {
-            const int o = base + tid;
-            const bool valid = o < nob;
+        // ---- phase 1: per measurement, one slot segment per wavefront ----------
+        // Measurements are in slot-major order, so slot s = t - a (0 for t <= a)
+        // is the contiguous segment soff[s] .. soff[s+1]; its pair sums are one
+        // reduction over the wavefront (no atomics, fixed order).
+        for (int seg = wv; seg <= K; seg += NWAVE) {
+            const int sb = soff[seg], se = soff[seg + 1];
             T pr[27];
 #pragma unroll
             for (int q = 0; q < 27; ++q) pr[q] = (T)0;
-            int slot = 0;
-            if (valid) {
-                const int t = lay.ob_t[ob0 + o];
-                const uint32_t meta = lay.ob_meta[ob0 + o];
-                const int pos = (int)(meta & 0xffffu);
-                const int j = (int)(meta >> 16);
-                const float4 uv = lay.ob_uv[ob0 + o];
-                slot = t > a ? t - a : 0;
-                const T Pa[3] = {lmP[j][0], lmP[j][1], lmP[j][2]};
-                const T dv[3] = {lmP[j][3], lmP[j][4], lmP[j][5]};
-                const T kappa = (A.rb.near_boost && abs(t - a) < 2) ? (T)16 : (T)1;
-                T u[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
-                T cc = (T)0, gg = (T)0;
+            for (int o = sb + lane; o - lane < se; o += 64) {
+                const bool valid = o < se;
+                MeasIn mi = pre;
+                if (o != sb + lane || seg != wv) mi = meas_load(lay, ob0, o, valid);
+                if (valid) {
+                    const int t = mi.t;
+                    const int pos = (int)(mi.meta & 0xffffu);
+                    const int j = (int)(mi.meta >> 16);
+                    const float4 uv = mi.uv;
+                    const T Pa[3] = {lmP[j][0], lmP[j][1], lmP[j][2]};
+                    const T dv[3] = {lmP[j][3], lmP[j][4], lmP[j][5]};
+                    const T kappa = (A.rb.near_boost && abs(t - a) < 2) ? (T)16 : (T)1;
+                    T u[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
+                    T cc = (T)0, gg = (T)0;
 #pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    T R[12];
-                    if (t >= a) {
+                    for (int c = 0; c < 2; ++c) {
+                        T R[12];
+                        if (t >= a) {
 #pragma unroll
-                        for (int i = 0; i < 12; ++i) R[i] = Trel[c][t - a][i];
-                    } else {
-                        double Ga[12], Gti[12], Tr[12];
-                        pose_load(bout.G + 12 * a, Ga);
-                        pose_load((c ? bout.GinvR : bout.GinvL) + 12 * t, Gti);
-                        pose_mul(Gti, Ga, Tr);
-#pragma unroll
-                        for (int i = 0; i < 12; ++i) R[i] = (T)Tr[i];
-                    }
-                    const T fx = c ? (T)A.cam.fx[1] : fx0, fy = c ? (T)A.cam.fy[1] : fy0;
-                    const T cx = c ? (T)A.cam.cx[1] : cx0, cy = c ? (T)A.cam.cy[1] : cy0;
-                    const T Px = R[0] * Pa[0] + R[1] * Pa[1] + R[2] * Pa[2] + R[3];
-                    const T Py = R[4] * Pa[0] + R[5] * Pa[1] + R[6] * Pa[2] + R[7];
-                    const T Pz = R[8] * Pa[0] + R[9] * Pa[1] + R[10] * Pa[2] + R[11];
-                    const T iz = (T)1 / Pz;
-                    const T ex = (c ? (T)uv.z : (T)uv.x) - (fx * Px * iz + cx);
-                    const T ey = (c ? (T)uv.w : (T)uv.y) - (fy * Py * iz + cy);
-                    T w;
-                    robust_eval(A.rb, ex * ex + ey * ey, kappa, w, cost);
-                    // rows of M R_{a->t}: M = K_c Pi'(P_t) (LMutils.cu:615-628)
-                    const T m0a = fx * iz, m0c = -fx * Px * iz * iz;
-                    const T m1b = fy * iz, m1c = -fy * Py * iz * iz;
-#pragma unroll
-                    for (int k = 0; k < 2; ++k) {
-                        T n[3];
-                        if (k == 0) {
-                            n[0] = m0a * R[0] + m0c * R[8];
-                            n[1] = m0a * R[1] + m0c * R[9];
-                            n[2] = m0a * R[2] + m0c * R[10];
+                            for (int i = 0; i < 12; ++i) R[i] = Trel[c][t - a][i];
                         } else {
-                            n[0] = m1b * R[4] + m1c * R[8];
-                            n[1] = m1b * R[5] + m1c * R[9];
-                            n[2] = m1b * R[6] + m1c * R[10];
+                            double Ga[12], Gti[12], Tr[12];
+                            pose_load(bout.G + 12 * a, Ga);
+                            pose_load((c ? bout.GinvR : bout.GinvL) + 12 * t, Gti);
+                            pose_mul(Gti, Ga, Tr);
+#pragma unroll
+                            for (int i = 0; i < 12; ++i) R[i] = (T)Tr[i];
                         }
-                        // weighted row [P_a x n, n] (the -[P]x of compute_del_tn_del_xi)
-                        T J[6];
-                        J[0] = w * (Pa[1] * n[2] - Pa[2] * n[1]);
-                        J[1] = w * (Pa[2] * n[0] - Pa[0] * n[2]);
-                        J[2] = w * (Pa[0] * n[1] - Pa[1] * n[0]);
-                        J[3] = w * n[0];
-                        J[4] = w * n[1];
-                        J[5] = w * n[2];
-                        const T ja = w * (n[0] * dv[0] + n[1] * dv[1] + n[2] * dv[2]);
-                        const T rk = w * (k == 0 ? ex : ey);
-                        int idx = 0;
+                        const T fx = c ? (T)A.cam.fx[1] : fx0, fy = c ? (T)A.cam.fy[1] : fy0;
+                        const T cx = c ? (T)A.cam.cx[1] : cx0, cy = c ? (T)A.cam.cy[1] : cy0;
+                        const T Px = R[0] * Pa[0] + R[1] * Pa[1] + R[2] * Pa[2] + R[3];
+                        const T Py = R[4] * Pa[0] + R[5] * Pa[1] + R[6] * Pa[2] + R[7];
+                        const T Pz = R[8] * Pa[0] + R[9] * Pa[1] + R[10] * Pa[2] + R[11];
+                        const T iz = (T)1 / Pz;
+                        const T ex = (c ? (T)uv.z : (T)uv.x) - (fx * Px * iz + cx);
+                        const T ey = (c ? (T)uv.w : (T)uv.y) - (fy * Py * iz + cy);
+                        T w;
+                        robust_eval(A.rb, ex * ex + ey * ey, kappa, w, cost);
+                        // rows of M R_{a->t}: M = K_c Pi'(P_t) (LMutils.cu:615-628)
+                        const T m0a = fx * iz, m0c = -fx * Px * iz * iz;
+                        const T m1b = fy * iz, m1c = -fy * Py * iz * iz;
 #pragma unroll
-                        for (int p = 0; p < 6; ++p) {
+                        for (int k = 0; k < 2; ++k) {
+                            T n[3];
+                            if (k == 0) {
+                                n[0] = m0a * R[0] + m0c * R[8];
+                                n[1] = m0a * R[1] + m0c * R[9];
+                                n[2] = m0a * R[2] + m0c * R[10];
+                            } else {
+                                n[0] = m1b * R[4] + m1c * R[8];
+                                n[1] = m1b * R[5] + m1c * R[9];
+                                n[2] = m1b * R[6] + m1c * R[10];
+                            }
+                            // weighted row [P_a x n, n] (the -[P]x of compute_del_tn_del_xi)
+                            T J[6];
+                            J[0] = w * (Pa[1] * n[2] - Pa[2] * n[1]);
+                            J[1] = w * (Pa[2] * n[0] - Pa[0] * n[2]);
+                            J[2] = w * (Pa[0] * n[1] - Pa[1] * n[0]);
+                            J[3] = w * n[0];
+                            J[4] = w * n[1];
+                            J[5] = w * n[2];
+                            const T ja = w * (n[0] * dv[0] + n[1] * dv[1] + n[2] * dv[2]);
+                            const T rk = w * (k == 0 ? ex : ey);
+                            int idx = 0;
 #pragma unroll
-                            for (int q = p; q < 6; ++q) pr[idx++] += J[p] * J[q];
-                            pr[21 + p] += J[p] * rk;
-                            u[p] += J[p] * ja;
+                            for (int p = 0; p < 6; ++p) {
+#pragma unroll
+                                for (int q = p; q < 6; ++q) pr[idx++] += J[p] * J[q];
+                                pr[21 + p] += J[p] * rk;
+                                u[p] += J[p] * ja;
+                            }
+                            cc += ja * ja;
+                            gg += ja * rk;
                         }
-                        cc += ja * ja;
-                        gg += ja * rk;
                     }
-                }
 #pragma unroll
-                for (int q = 0; q < 6; ++q) stage[pos][q] = u[q];
-                stage[pos][6] = cc;
-                stage[pos][7] = gg;
-                stslot[pos] = slot;
-            }
-            // pair sums: per-wave segmented reduction over lanes (deterministic)
-            T* scr = Vsh + (wv * 64 + lane) * SCR;
-#pragma unroll
-            for (int q = 0; q < 27; ++q) scr[q] = pr[q];
-            sslot[wv][lane] = slot;
-            __syncthreads();
-            if (lane < 27) {
-                const T* col = Vsh + wv * 64 * SCR + lane;
-                T s = (T)0;
-                int cs = sslot[wv][0];
-                for (int jj = 0; jj < 64; ++jj) {
-                    const int sj = sslot[wv][jj];
-                    if (sj != cs) {
-                        if (cs > 0) S_w[wv][cs][lane] += s;
-                        s = (T)0;
-                        cs = sj;
-                    }
-                    s += col[jj * SCR];
+                    for (int q = 0; q < 6; ++q) stage[pos][q] = u[q];
+                    stage[pos][6] = cc;
+                    stage[pos][7] = gg;
+                    stslot[pos] = seg;
                 }
-                if (cs > 0) S_w[wv][cs][lane] += s;
             }
-            __syncthreads();
+            if (seg > 0) {
+                // pair sums of the segment: fold the two halves of the wavefront,
+                // then 27 lanes each add 32 partials
+                T* scr = Vsh + wv * SCRW * 27;
+#pragma unroll
+                for (int q = 0; q < 27; ++q) pr[q] += __shfl_xor(pr[q], 32, 64);
+                if (lane < 32) {
+#pragma unroll
+                    for (int q = 0; q < 27; ++q) scr[lane * 27 + q] = pr[q];
+                }
+                wave_order();
+                if (lane < 27) {
+                    T s = (T)0;
+#pragma unroll
+                    for (int jj = 0; jj < SCRW; ++jj) s += scr[jj * 27 + lane];
+                    S[seg][lane] += s;
+                }
+                wave_order();
+            }
         }
-
-        // ---- phase 2: per landmark: C, g_alpha, B column v_l -------------------
-        for (int i = tid; i < VR * VS; i += WG) Vsh[i] = (T)0;   // scratch -> V tile
         __syncthreads();
-        if (tid < nl) {
-            const int act = lm0 + tid;
-            const int2 lp = lay.lm_pos[act];
-            T C = (T)0, G = (T)0;
+
+        // ---- phase 2: per (landmark, pose block): C, g_alpha, B column v_l ------
+        for (int i = tid; i < SH::VROWS * VS; i += WG) Vsh[i] = (T)0;   // scratch -> V tile
+        __syncthreads();
+        // (at least one item per landmark: groups with K = 0 still have C and g_alpha)
+        const int KK = K > 0 ? K : 1;
+        for (int it = tid; it < nl * KK; it += WG) {
+            const int j = it / KK, k = it - j * KK;
+            const int act = lm0 + j;
+            const int2 lp = lpos[j];
+            T C = (T)0;
+            T s6[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
             for (int p = lp.x; p < lp.x + lp.y; ++p) {
                 C += stage[p][6];
-                G += stage[p][7];
+                const bool in = stslot[p] > k;   // v_l(k) sums the measurements of slots > k
+#pragma unroll
+                for (int q = 0; q < 6; ++q) s6[q] += in ? stage[p][q] : (T)0;
             }
             const T Ct = C + (T)A.eps;
             const T isc = (T)1 / sqrt(Ct);
-            T* vo = bout.v + voff + (long long)(act - glm0) * K6;
-            T s6[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
-            int p = lp.x + lp.y - 1;
-            for (int k = K - 1; k >= 0; --k) {
-                while (p >= lp.x && stslot[p] > k) {
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) s6[q] += stage[p][q];
-                    --p;
-                }
+            if (k < K) {
+                T* vo = bout.v + voff + (long long)(act - glm0) * K6 + 6 * k;
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
-                    vo[6 * k + q] = s6[q];
-                    Vsh[(6 * k + q) * VS + tid] = s6[q] * isc;
+                    vo[q] = s6[q];
+                    Vsh[(6 * k + q) * VS + j] = s6[q] * isc;
                 }
             }
-            bout.Ct[act] = Ct;
-            bout.g[act] = G;
-            gt[tid] = G * isc;
-        } else if (tid < CHUNK_LM) {
-            gt[tid] = (T)0;
+            if (k == 0) {
+                T G = (T)0;
+                for (int p = lp.x; p < lp.x + lp.y; ++p) G += stage[p][7];
+                bout.Ct[act] = Ct;
+                bout.g[act] = G;
+                Vsh[K6 * VS + j] = G * isc;     // the gradient row: y comes out of the same products
+            }
         }
         __syncthreads();
 
-        // ---- phase 3: Y += V V^T (MFMA), y += V gt -----------------------------
+        // ---- phase 3: [V; g]^T-products on MFMA: Y = V V^T and y = V g -----------
         const int kend = (nl + 3) & ~3;
 #pragma unroll
-        for (int ti = 0; ti < MAXTILE; ++ti) {
+        for (int ti = 0; ti < SH::MAXTILE; ++ti) {
             const int tile = wv + ti * NWAVE;
             if (tile < nTiles) {
                 int r = 0;
@@ -341,29 +381,22 @@ __global__ __launch_bounds__(WG) void k_assemble(AsmArgs A, StateBuf<T> B0, Stat
                 for (int kk = 0; kk < kend; kk += 4) acc[ti] = mfma16x16x4(Ar[kk], Br[kk], acc[ti]);
             }
         }
-        if (tid < K6) {
-            const T* row = Vsh + tid * VS;
-            for (int jj = 0; jj < nl; ++jj) yr += row[jj] * gt[jj];
-        }
         __syncthreads();
     }
 
     // ---- finalize: D = Q - Y, d = q - y ----------------------------------------
-    T* Rsuf = &stage[0][0];            // [KMAX][27] suffix sums of the pair partials
-    if (tid < 27) {
+    // suffix sums over slots: Q(i, j) = sum_{s > max(i, j)} S(s)
+    T* Rsuf = &stage[0][0];            // [K][27]
+    for (int it = tid; it < K * 27; it += WG) {
+        const int s0 = it / 27 + 1, q = it - (s0 - 1) * 27;
         T run = (T)0;
-        for (int s = K; s >= 1; --s) {
-#pragma unroll
-            for (int w = 0; w < NWAVE; ++w) run += S_w[w][s][tid];
-            Rsuf[(s - 1) * 27 + tid] = run;
-        }
+        for (int s = K; s >= s0; --s) run += S[s][q];
+        Rsuf[(s0 - 1) * 27 + q] = run;
     }
     __syncthreads();
-    // D = Q - Y into LDS (row stride 6K), d = q - y beside it
-    T* Dl = Vsh;
-    T* dlv = &stage[0][0] + KMAX * 27;
+    T* Dl = Vsh;   // K6 x K6, full
 #pragma unroll
-    for (int ti = 0; ti < MAXTILE; ++ti) {
+    for (int ti = 0; ti < SH::MAXTILE; ++ti) {
         const int tile = wv + ti * NWAVE;
         if (tile < nTiles) {
             int r = 0;
@@ -375,48 +408,59 @@ __global__ __launch_bounds__(WG) void k_assemble(AsmArgs A, StateBuf<T> B0, Stat
                 const int col = c * 16 + (lane & 15);
                 if (row < K6 && col < K6) {
                     const int kb = max(row / 6, col / 6);
-                    const T q = Rsuf[kb * 27 + sym6(row % 6, col % 6)];
-                    const T val = q - acc[ti][v];
+                    const T val = Rsuf[kb * 27 + sym6(row % 6, col % 6)] - acc[ti][v];
                     Dl[row * K6 + col] = val;
                     Dl[col * K6 + row] = val;
+                } else if (row == K6 && col < K6) {
+                    dly[col] = Rsuf[(col / 6) * 27 + 21 + (col % 6)] - acc[ti][v];
                 }
             }
         }
     }
-    if (tid < K6) dlv[tid] = Rsuf[(tid / 6) * 27 + 21 + (tid % 6)] - yr;
     __syncthreads();
-    // to incremental-pose coordinates: block (i, j) <- Ad_i^T D(i, j) Ad_j, d_i <- Ad_i^T d_i,
-    // Ad_k = Ad(G_a^-1 G_{a+k}); lower blocks only (the reduced system reads i >= j)
+    // to incremental-pose coordinates, in place: D(i, j) <- Ad_i^T D(i, j) Ad_j with
+    // Ad_k = Ad(G_a^-1 G_{a+k}); first every row times the block-diagonal Ad ...
+    for (int it = tid; it < K6 * K; it += WG) {
+        const int row = it / K, bj = it - row * K;
+        T* d6 = Dl + row * K6 + 6 * bj;
+        double in[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) in[q] = (double)d6[q];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) s += in[q] * Adk[bj][6 * q + c];
+            d6[c] = (T)s;
+        }
+    }
+    __syncthreads();
+    // ... then the transposed block-diagonal Ad times every column
+    for (int it = tid; it < K * K6; it += WG) {
+        const int bi = it / K6, col = it - bi * K6;
+        double in[6];
+#pragma unroll
+        for (int p = 0; p < 6; ++p) in[p] = (double)Dl[(6 * bi + p) * K6 + col];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            double s = 0.0;
+#pragma unroll
+            for (int p = 0; p < 6; ++p) s += Adk[bi][6 * p + r] * in[p];
+            Dl[(6 * bi + r) * K6 + col] = (T)s;
+        }
+    }
+    __syncthreads();
+    // lower blocks out (the reduced system reads block rows i >= j)
     T* Dg = bout.D + lay.grp_D_off[g];
-    for (int pr = tid; pr < K * (K + 1) / 2; pr += WG) {
-        int bi = 0;
-        while ((bi + 1) * (bi + 2) / 2 <= pr) ++bi;
-        const int bj = pr - bi * (bi + 1) / 2;
-        double M1[36];
-#pragma unroll
-        for (int r = 0; r < 6; ++r)
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                double s = 0.0;
-#pragma unroll
-                for (int q = 0; q < 6; ++q) s += (double)Dl[(6 * bi + r) * K6 + 6 * bj + q] * Adk[bj][6 * q + c];
-                M1[6 * r + c] = s;
-            }
-#pragma unroll
-        for (int r = 0; r < 6; ++r)
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                double s = 0.0;
-#pragma unroll
-                for (int q = 0; q < 6; ++q) s += Adk[bi][6 * q + r] * M1[6 * q + c];
-                Dg[(long long)(6 * bi + r) * K6 + 6 * bj + c] = (T)s;
-            }
+    for (int it = tid; it < K6 * K6; it += WG) {
+        const int row = it / K6, col = it - row * K6;
+        if (row / 6 >= col / 6) Dg[it] = Dl[it];
     }
     if (tid < K6) {
         const int k = tid / 6, r = tid % 6;
         double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) s += Adk[k][6 * q + r] * (double)dlv[6 * k + q];
+        for (int q = 0; q < 6; ++q) s += Adk[k][6 * q + r] * (double)dly[6 * k + q];
         bout.d[lay.grp_d_off[g] + tid] = (T)s;
     }
     cost = wave_sum(cost);
@@ -432,7 +476,10 @@ __global__ __launch_bounds__(WG) void k_assemble(AsmArgs A, StateBuf<T> B0, Stat
 template <typename T>
 void launch_assemble(const AsmArgs& a, const StateBuf<T>* bufs, hipStream_t s) {
     if (a.lay.n_groups == 0) return;
-    hipLaunchKernelGGL(k_assemble<T>, dim3(a.lay.n_groups), dim3(WG), 0, s, a, bufs[0], bufs[1]);
+    if (a.lay.kmax <= 8)
+        hipLaunchKernelGGL((k_assemble<T, 8>), dim3(a.lay.n_groups), dim3(WG), 0, s, a, bufs[0], bufs[1]);
+    else
+        hipLaunchKernelGGL((k_assemble<T, KMAX>), dim3(a.lay.n_groups), dim3(WG), 0, s, a, bufs[0], bufs[1]);
 }
 
 template void launch_assemble<double>(const AsmArgs&, const StateBuf<double>*, hipStream_t);

@@ -56,6 +56,7 @@ struct Layout {
     const int2* lm_pos;           // (first landmark-major position in chunk, count)
     // chunks
     const int4* chunk;            // (first active landmark, n landmarks, first obs, n obs)
+    const int* chunk_soff;        // per chunk KMAX + 2 entries: first position of slot s (slot-major order)
     // groups
     const int* grp_anchor;
     const int* grp_K;             // pose span of the group

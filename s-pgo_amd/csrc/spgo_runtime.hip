@@ -56,6 +56,7 @@ struct HostLayout {
     std::vector<float2> lm_ray;
     std::vector<int2> lm_pos;
     std::vector<int4> chunk;
+    std::vector<int> chunk_soff;    // per chunk KMAX + 2 offsets: first position of each slot
     std::vector<int> grp_anchor, grp_K, grp_lm_begin, grp_d_off;
     std::vector<int2> grp_chunks;
     std::vector<long long> grp_v_off, grp_D_off;
@@ -271,6 +272,16 @@ int build_layout(spgo_solver* h, HostLayout& out) {
                 if (x.slot != y.slot) return x.slot < y.slot;
                 return x.j < y.j;
             });
+            {
+                // slot offsets inside the chunk (slot-major order), padded to KMAX + 2 entries
+                const size_t base = hl.chunk_soff.size();
+                hl.chunk_soff.resize(base + KMAX + 2, nobs);
+                int q = 0;
+                for (int sl = 0; sl <= KMAX + 1; ++sl) {
+                    while (q < nobs && obs[q].slot < sl) ++q;
+                    hl.chunk_soff[base + sl] = q;
+                }
+            }
             for (const Ob& o : obs) {
                 const int m = o.m;
                 hl.ob_t.push_back(h->mt[m]);
@@ -407,7 +418,7 @@ int setup_layout(spgo_solver* h) {
     if ((rc = upload(h, L.ob_t, &lay.ob_t)) || (rc = upload(h, L.ob_meta, &lay.ob_meta)) ||
         (rc = upload(h, L.ob_uv, &lay.ob_uv)) || (rc = upload(h, L.lm_id, &lay.lm_id)) ||
         (rc = upload(h, L.lm_ray, &lay.lm_ray)) || (rc = upload(h, L.lm_pos, &lay.lm_pos)) ||
-        (rc = upload(h, L.chunk, &lay.chunk)) || (rc = upload(h, L.grp_anchor, &lay.grp_anchor)) ||
+        (rc = upload(h, L.chunk, &lay.chunk)) || (rc = upload(h, L.chunk_soff, &lay.chunk_soff)) || (rc = upload(h, L.grp_anchor, &lay.grp_anchor)) ||
         (rc = upload(h, L.grp_K, &lay.grp_K)) || (rc = upload(h, L.grp_chunks, &lay.grp_chunks)) ||
         (rc = upload(h, L.grp_lm_begin, &lay.grp_lm_begin)) || (rc = upload(h, L.grp_v_off, &lay.grp_v_off)) ||
         (rc = upload(h, L.grp_D_off, &lay.grp_D_off)) || (rc = upload(h, L.grp_d_off, &lay.grp_d_off)) ||
