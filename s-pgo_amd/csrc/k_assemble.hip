@@ -41,6 +41,18 @@ struct AsmShape {
 // keep the compiler from reordering one wavefront's LDS accesses
 __device__ __forceinline__ void wave_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
 
+// workgroup barrier that orders LDS only: the kernel never reads back what it
+// stores to global memory, so it does not wait for those stores
+// (s_waitcnt lgkmcnt(0) with vmcnt/expcnt at their maximum)
+__device__ __forceinline__ void lds_barrier() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+constexpr int PRE_SEG = 2;   // slot segments per wavefront whose measurements are loaded up front
+
 template <typename T> struct Vec4;
 template <> struct Vec4<double> { typedef double __attribute__((ext_vector_type(4))) type; };
 template <> struct Vec4<float> { typedef float __attribute__((ext_vector_type(4))) type; };
@@ -52,13 +64,38 @@ __device__ __forceinline__ Vec4<float>::type mfma16x16x4(float a, float b, Vec4<
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// 1/x: the hardware estimate refined by two Newton steps (FP64); plain
+// division in FP32
+__device__ __forceinline__ double rcp_refined(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+__device__ __forceinline__ float rcp_refined(float x) { return 1.0f / x; }
+
+// 1/sqrt(x): the hardware estimate refined by one Halley step (FP64)
+__device__ __forceinline__ double rsqrt_refined(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(0.375, e, 0.5), y);
+}
+__device__ __forceinline__ float rsqrt_refined(float x) { return 1.0f / sqrtf(x); }
+
+// Robust weight w = sqrt(kappa * rho'(s)) and cost 1/2 kappa rho(s) of the
+// squared residual norm s.  inv_param = 1 / param.  Cauchy:
+// rho = c^2 log(1 + s / c^2), w = sqrt(kappa) / sqrt(1 + s / c^2).
 template <typename T>
-__device__ __forceinline__ void robust_eval(const Robust& rb, T s, T kappa, T& w, double& cost) {
+__device__ __forceinline__ void robust_eval(const Robust& rb, T inv_param, T s, T kappa, T& w, double& cost) {
     T rho, drho;
     if (rb.kind == 0) {
         const T c2 = (T)rb.param;
-        rho = c2 * log1p(s / c2);
-        drho = (T)1 / ((T)1 + s / c2);
+        const T x = s * inv_param;
+        rho = c2 * log1p(x);
+        w = (kappa == (T)16 ? (T)4 : (T)1) * rsqrt_refined((T)1 + x);
+        cost += 0.5 * (double)kappa * (double)rho;
+        return;
     } else if (rb.kind == 1) {
         const T d = (T)rb.param;
         const T rs = sqrt(s);
@@ -110,8 +147,9 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
     const bool update = !A.init;
     const bool chol_ok = update ? (A.ctl->chol_ok != 0) : true;
 
-    const int a = lay.grp_anchor[g];
-    const int K = lay.grp_K[g];
+    const GroupDesc& gd = lay.grp_desc[g];      // one record: no chain of dependent loads
+    const int a = gd.anchor;
+    const int K = gd.K;
     const int K6 = 6 * K;
     const int nT = (K6 + 16) >> 4;             // tile rows: B columns and the gradient row K6
     const int nTiles = nT * (nT + 1) / 2;
@@ -119,8 +157,11 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
     __shared__ T S[KT + 1][27];                 // pair sums per slot t - a
     __shared__ T Trel[2][KT + 1][12];           // (G^c_{a+s})^-1 G_a of the state being linearised
     __shared__ T dl[KT][6];                     // Ad(G_a^-1 G_{a+k}) delta_{a+k} (back-substitution)
-    __shared__ T stage[CHUNK_OBS][8];           // per measurement: J_a^T j_alpha (6), j_alpha^2, j_alpha r
-    __shared__ int stslot[CHUNK_OBS];
+    // per (landmark, slot s >= 1): u = J_a^T j_alpha of its measurements (zero if none);
+    // row stride 6 KT + 1 keeps the landmarks of one wavefront in different banks
+    __shared__ T U[CHUNK_LM * (6 * KT + 1)];
+    __shared__ T ccp[CHUNK_OBS], ggp[CHUNK_OBS];   // per measurement: j_alpha^2, j_alpha r
+    __shared__ T iscl[CHUNK_LM];                // 1 / sqrt(C_l + eps)
     __shared__ T Vsh[SH::VSH];                  // B columns in / pair scratch / V tile / D
     __shared__ T lmP[CHUNK_LM][6];              // P_a and -P_a/alpha in the anchor frame
     __shared__ int2 lpos[CHUNK_LM];
@@ -164,32 +205,66 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
 #pragma unroll
     for (int i = 0; i < SH::MAXTILE; ++i) acc[i] = (T)0;
     double cost = 0.0;
+    T gl_grad = (T)0;
 
     const T fx0 = (T)A.cam.fx[0], fy0 = (T)A.cam.fy[0], cx0 = (T)A.cam.cx[0], cy0 = (T)A.cam.cy[0];
-    const int2 cr = lay.grp_chunks[g];
-    const int glm0 = lay.grp_lm_begin[g];
-    const long long voff = lay.grp_v_off[g];
+    const T inv_param = A.rb.param != 0.0 ? (T)(1.0 / A.rb.param) : (T)0;
+    const int glm0 = gd.lm_begin;
+    const long long voff = gd.v_off;
 
-    for (int ci = cr.x; ci < cr.y; ++ci) {
-        const int4 ch = lay.chunk[ci];
+    for (int ci = gd.chunk_begin; ci < gd.chunk_end; ++ci) {
+        const bool first = ci == gd.chunk_begin;
+        const int4 ch = first ? gd.chunk0 : lay.chunk[ci];
         const int lm0 = ch.x, nl = ch.y, ob0 = ch.z;
-        const int* soff = lay.chunk_soff + (long long)ci * (KMAX + 2);
-        // this wavefront's first slot segment is loaded now and evaluated after phase 0
-        MeasIn pre;
-        {
-            const int sb = wv <= K ? soff[wv] : 0, se = wv <= K ? soff[wv + 1] : 0;
-            pre = meas_load(lay, ob0, sb + lane, sb + lane < se);
+        const int* soff = first ? gd.soff0 : lay.chunk_soff + (long long)ci * (KMAX + 2);
+        // everything phases 0 and 1 read from global memory is requested now:
+        // the measurements of this wavefront's first slot segments and the
+        // landmark records
+        MeasIn pre[PRE_SEG];
+#pragma unroll
+        for (int r = 0; r < PRE_SEG; ++r) {
+            const int seg = wv + r * NWAVE;
+            const int sb = seg <= K ? soff[seg] : 0, se = seg <= K ? soff[seg + 1] : 0;
+            pre[r] = meas_load(lay, ob0, sb + lane, sb + lane < se);
+        }
+        int lid = 0;
+        float2 lray = make_float2(0.f, 0.f);
+        int2 lps = make_int2(0, 0);
+        if (tid < nl) {
+            lid = lay.lm_id[lm0 + tid];
+            lray = lay.lm_ray[lm0 + tid];
+            lps = lay.lm_pos[lm0 + tid];
         }
         // ---- phase 0: back-substitution and anchor-frame point per landmark ----
         // the chunk's B columns (nl x 6K, contiguous) into LDS with coalesced loads
         const T* vin = bin.v + voff + (long long)(lm0 - glm0) * K6;
-        if (update && chol_ok)
-            for (int i = tid; i < nl * K6; i += WG) Vsh[i] = vin[i];
-        __syncthreads();   // also orders the set-up above
+        if (update && chol_ok) {
+            // loads in flight in batches of NB before their stores (two round trips
+            // for spans up to 8)
+            constexpr int NV = (CHUNK_LM * 6 * KT + WG - 1) / WG;
+            constexpr int NBV = 6;
+            const int nv = nl * K6;
+#pragma unroll
+            for (int r0 = 0; r0 < NV; r0 += NBV) {
+                T vv[NBV];
+#pragma unroll
+                for (int r = 0; r < NBV; ++r) {
+                    const int i = tid + (r0 + r) * WG;
+                    vv[r] = (r0 + r < NV && i < nv) ? vin[i] : (T)0;
+                }
+#pragma unroll
+                for (int r = 0; r < NBV; ++r) {
+                    const int i = tid + (r0 + r) * WG;
+                    if (r0 + r < NV && i < nv) Vsh[i] = vv[r];
+                }
+            }
+        }
+        for (int i = tid; i < CHUNK_LM * (6 * KT + 1); i += WG) U[i] = (T)0;
+        lds_barrier();   // also orders the set-up above
         if (tid < nl) {
             const int act = lm0 + tid;
-            const int id = lay.lm_id[act];
-            lpos[tid] = lay.lm_pos[act];
+            const int id = lid;
+            lpos[tid] = lps;
             double al = bin.alpha[id];
             if (update) {
                 double dal = 0.0;
@@ -204,7 +279,7 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
                 al = fmin(fmax(al + dal, A.alpha_min), A.alpha_max);
                 bout.alpha[id] = al;
             }
-            const float2 ray = lay.lm_ray[act];
+            const float2 ray = lray;
             const double x = ((double)ray.x - A.cam.cx[0]) / A.cam.fx[0];
             const double y = ((double)ray.y - A.cam.cy[0]) / A.cam.fy[0];
             const double P0 = x / al, P1 = y / al, P2 = 1.0 / al;
@@ -215,7 +290,7 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
             lmP[tid][4] = (T)(-P1 / al);
             lmP[tid][5] = (T)(-P2 / al);
         }
-        __syncthreads();
+        lds_barrier();
 
         // ---- phase 1: per measurement, one slot segment per wavefront ----------
         // Measurements are in slot-major order, so slot s = t - a (0 for t <= a)
@@ -226,10 +301,19 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
             T pr[27];
 #pragma unroll
             for (int q = 0; q < 27; ++q) pr[q] = (T)0;
+            const int round = (seg - wv) / NWAVE;
             for (int o = sb + lane; o - lane < se; o += 64) {
                 const bool valid = o < se;
-                MeasIn mi = pre;
-                if (o != sb + lane || seg != wv) mi = meas_load(lay, ob0, o, valid);
+                MeasIn mi;
+                if (o == sb + lane && round < PRE_SEG) {
+                    // a register array indexed by a wavefront-uniform value: select, no scratch
+                    mi = pre[0];
+#pragma unroll
+                    for (int r = 1; r < PRE_SEG; ++r)
+                        if (round == r) mi = pre[r];
+                } else {
+                    mi = meas_load(lay, ob0, o, valid);
+                }
                 if (valid) {
                     const int t = mi.t;
                     const int pos = (int)(mi.meta & 0xffffu);
@@ -259,11 +343,11 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
                         const T Px = R[0] * Pa[0] + R[1] * Pa[1] + R[2] * Pa[2] + R[3];
                         const T Py = R[4] * Pa[0] + R[5] * Pa[1] + R[6] * Pa[2] + R[7];
                         const T Pz = R[8] * Pa[0] + R[9] * Pa[1] + R[10] * Pa[2] + R[11];
-                        const T iz = (T)1 / Pz;
+                        const T iz = rcp_refined(Pz);
                         const T ex = (c ? (T)uv.z : (T)uv.x) - (fx * Px * iz + cx);
                         const T ey = (c ? (T)uv.w : (T)uv.y) - (fy * Py * iz + cy);
                         T w;
-                        robust_eval(A.rb, ex * ex + ey * ey, kappa, w, cost);
+                        robust_eval(A.rb, inv_param, ex * ex + ey * ey, kappa, w, cost);
                         // rows of M R_{a->t}: M = K_c Pi'(P_t) (LMutils.cu:615-628)
                         const T m0a = fx * iz, m0c = -fx * Px * iz * iz;
                         const T m1b = fy * iz, m1c = -fy * Py * iz * iz;
@@ -301,11 +385,16 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
                             gg += ja * rk;
                         }
                     }
+                    if (seg > 0) {
+                        // a cell written twice is two measurements of one landmark in one
+                        // slot (the reference counts both): add, do not overwrite.  The
+                        // slot starts at zero, so one or two contributions give the
+                        // same sum in either order.
 #pragma unroll
-                    for (int q = 0; q < 6; ++q) stage[pos][q] = u[q];
-                    stage[pos][6] = cc;
-                    stage[pos][7] = gg;
-                    stslot[pos] = seg;
+                        for (int q = 0; q < 6; ++q) atomicAdd(&U[j * (6 * KT + 1) + 6 * (seg - 1) + q], u[q]);
+                    }
+                    ccp[pos] = cc;
+                    ggp[pos] = gg;
                 }
             }
             if (seg > 0) {
@@ -328,44 +417,45 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
                 wave_order();
             }
         }
-        __syncthreads();
+        lds_barrier();
 
-        // ---- phase 2: per (landmark, pose block): C, g_alpha, B column v_l ------
+        // ---- phase 2: per landmark C, g_alpha and its B column v_l ---------------
         for (int i = tid; i < SH::VROWS * VS; i += WG) Vsh[i] = (T)0;   // scratch -> V tile
-        __syncthreads();
-        // (at least one item per landmark: groups with K = 0 still have C and g_alpha)
-        const int KK = K > 0 ? K : 1;
-        for (int it = tid; it < nl * KK; it += WG) {
-            const int j = it / KK, k = it - j * KK;
-            const int act = lm0 + j;
-            const int2 lp = lpos[j];
-            T C = (T)0;
-            T s6[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
-            for (int p = lp.x; p < lp.x + lp.y; ++p) {
-                C += stage[p][6];
-                const bool in = stslot[p] > k;   // v_l(k) sums the measurements of slots > k
-#pragma unroll
-                for (int q = 0; q < 6; ++q) s6[q] += in ? stage[p][q] : (T)0;
+        if (tid < nl) {
+            // sums over the landmark's measurements, starting at a landmark-dependent
+            // position so the lanes of a wavefront read different banks
+            const int2 lp = lpos[tid];
+            T C = (T)0, G = (T)0;
+            int p = lp.x + (lp.y > 0 ? tid % lp.y : 0);
+            for (int i = 0; i < lp.y; ++i) {
+                C += ccp[p];
+                G += ggp[p];
+                if (++p == lp.x + lp.y) p = lp.x;
             }
+            const int act = lm0 + tid;
             const T Ct = C + (T)A.eps;
             const T isc = (T)1 / sqrt(Ct);
-            if (k < K) {
-                T* vo = bout.v + voff + (long long)(act - glm0) * K6 + 6 * k;
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    vo[q] = s6[q];
-                    Vsh[(6 * k + q) * VS + j] = s6[q] * isc;
-                }
-            }
-            if (k == 0) {
-                T G = (T)0;
-                for (int p = lp.x; p < lp.x + lp.y; ++p) G += stage[p][7];
-                bout.Ct[act] = Ct;
-                bout.g[act] = G;
-                Vsh[K6 * VS + j] = G * isc;     // the gradient row: y comes out of the same products
+            bout.Ct[act] = Ct;
+            bout.g[act] = G;
+            iscl[tid] = isc;
+            gl_grad = G * isc;
+        }
+        lds_barrier();
+        if (tid < nl) Vsh[K6 * VS + tid] = gl_grad;   // the gradient row: y comes out of the same products
+        // v_l(k) = sum of u over the slots s > k: a suffix scan per (landmark, component)
+        for (int it = tid; it < nl * 6; it += WG) {
+            const int j = it / 6, q = it - j * 6;
+            const T isc = iscl[j];
+            T* vo = bout.v + voff + (long long)(lm0 + j - glm0) * K6 + q;
+            const T* uj = U + j * (6 * KT + 1) + q;
+            T run = (T)0;
+            for (int k = K - 1; k >= 0; --k) {
+                run += uj[6 * k];
+                vo[6 * k] = run;
+                Vsh[(6 * k + q) * VS + j] = run * isc;
             }
         }
-        __syncthreads();
+        lds_barrier();
 
         // ---- phase 3: [V; g]^T-products on MFMA: Y = V V^T and y = V g -----------
         const int kend = (nl + 3) & ~3;
@@ -381,19 +471,19 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
                 for (int kk = 0; kk < kend; kk += 4) acc[ti] = mfma16x16x4(Ar[kk], Br[kk], acc[ti]);
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
 
     // ---- finalize: D = Q - Y, d = q - y ----------------------------------------
     // suffix sums over slots: Q(i, j) = sum_{s > max(i, j)} S(s)
-    T* Rsuf = &stage[0][0];            // [K][27]
+    T* Rsuf = U;                       // [K][27]
     for (int it = tid; it < K * 27; it += WG) {
         const int s0 = it / 27 + 1, q = it - (s0 - 1) * 27;
         T run = (T)0;
         for (int s = K; s >= s0; --s) run += S[s][q];
         Rsuf[(s0 - 1) * 27 + q] = run;
     }
-    __syncthreads();
+    lds_barrier();
     T* Dl = Vsh;   // K6 x K6, full
 #pragma unroll
     for (int ti = 0; ti < SH::MAXTILE; ++ti) {
@@ -417,7 +507,7 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
     // to incremental-pose coordinates, in place: D(i, j) <- Ad_i^T D(i, j) Ad_j with
     // Ad_k = Ad(G_a^-1 G_{a+k}); first every row times the block-diagonal Ad ...
     for (int it = tid; it < K6 * K; it += WG) {
@@ -434,7 +524,7 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
             d6[c] = (T)s;
         }
     }
-    __syncthreads();
+    lds_barrier();
     // ... then the transposed block-diagonal Ad times every column
     for (int it = tid; it < K * K6; it += WG) {
         const int bi = it / K6, col = it - bi * K6;
@@ -449,9 +539,9 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
             Dl[(6 * bi + r) * K6 + col] = (T)s;
         }
     }
-    __syncthreads();
+    lds_barrier();
     // lower blocks out (the reduced system reads block rows i >= j)
-    T* Dg = bout.D + lay.grp_D_off[g];
+    T* Dg = bout.D + gd.D_off;
     for (int it = tid; it < K6 * K6; it += WG) {
         const int row = it / K6, col = it - row * K6;
         if (row / 6 >= col / 6) Dg[it] = Dl[it];
@@ -461,11 +551,11 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
         double s = 0.0;
 #pragma unroll
         for (int q = 0; q < 6; ++q) s += Adk[k][6 * q + r] * (double)dly[6 * k + q];
-        bout.d[lay.grp_d_off[g] + tid] = (T)s;
+        bout.d[gd.d_off + tid] = (T)s;
     }
     cost = wave_sum(cost);
     if (lane == 0) red[wv] = cost;
-    __syncthreads();
+    lds_barrier();
     if (tid == 0) {
         double c = 0.0;
         for (int w = 0; w < NWAVE; ++w) c += red[w];

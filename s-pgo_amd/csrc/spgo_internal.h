@@ -13,6 +13,16 @@ constexpr int CHUNK_OBS = 512;      // measurements per LDS chunk (upper bound)
 constexpr int NB = 32;              // Cholesky block size
 constexpr int WG = 256;             // threads per workgroup of the wide kernels
 
+// Everything the linearisation needs to start a landmark group, in one record
+// (one memory round trip): the group's fields and those of its first chunk.
+struct alignas(16) GroupDesc {
+    int anchor, K, chunk_begin, chunk_end;
+    int lm_begin, d_off, pad0, pad1;
+    long long v_off, D_off;
+    int4 chunk0;                  // (first active landmark, n landmarks, first obs, n obs)
+    int soff0[KMAX + 2];          // slot offsets of the first chunk
+};
+
 // Device control block: which state buffer is current, damping, cost.
 struct Control {
     int cur;                // index (0/1) of the accepted state buffers
@@ -66,6 +76,7 @@ struct Layout {
     const long long* grp_D_off;   // offset of the group's (6K)^2 block
     const int* grp_d_off;         // offset of the group's 6K vector
     const int* anchor_grp;        // CSR over anchor frames, F+1 entries
+    const GroupDesc* grp_desc;    // per group, for k_assemble
 };
 
 // One of the two state/linearisation buffer sets (current and candidate).

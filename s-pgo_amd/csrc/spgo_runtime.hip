@@ -61,6 +61,7 @@ struct HostLayout {
     std::vector<int2> grp_chunks;
     std::vector<long long> grp_v_off, grp_D_off;
     std::vector<int> anchor_grp;
+    std::vector<GroupDesc> grp_desc;
     int kmax = 0;
     long long v_total = 0, D_total = 0, d_total = 0;
     int anchors_used = 0;
@@ -310,6 +311,21 @@ int build_layout(spgo_solver* h, HostLayout& out) {
         if (grp_of_anchor_count[a]) hl.anchors_used++;
     }
     hl.kmax = std::max(hl.kmax, 1);
+    hl.grp_desc.resize(hl.grp_anchor.size());
+    for (size_t g = 0; g < hl.grp_anchor.size(); ++g) {
+        GroupDesc& d = hl.grp_desc[g];
+        d = GroupDesc{};
+        d.anchor = hl.grp_anchor[g];
+        d.K = hl.grp_K[g];
+        d.chunk_begin = hl.grp_chunks[g].x;
+        d.chunk_end = hl.grp_chunks[g].y;
+        d.lm_begin = hl.grp_lm_begin[g];
+        d.d_off = hl.grp_d_off[g];
+        d.v_off = hl.grp_v_off[g];
+        d.D_off = hl.grp_D_off[g];
+        d.chunk0 = hl.chunk[d.chunk_begin];
+        for (int sl = 0; sl < KMAX + 2; ++sl) d.soff0[sl] = hl.chunk_soff[(size_t)d.chunk_begin * (KMAX + 2) + sl];
+    }
     out = std::move(hl);
     return SPGO_OK;
 }
@@ -418,7 +434,7 @@ int setup_layout(spgo_solver* h) {
     if ((rc = upload(h, L.ob_t, &lay.ob_t)) || (rc = upload(h, L.ob_meta, &lay.ob_meta)) ||
         (rc = upload(h, L.ob_uv, &lay.ob_uv)) || (rc = upload(h, L.lm_id, &lay.lm_id)) ||
         (rc = upload(h, L.lm_ray, &lay.lm_ray)) || (rc = upload(h, L.lm_pos, &lay.lm_pos)) ||
-        (rc = upload(h, L.chunk, &lay.chunk)) || (rc = upload(h, L.chunk_soff, &lay.chunk_soff)) || (rc = upload(h, L.grp_anchor, &lay.grp_anchor)) ||
+        (rc = upload(h, L.chunk, &lay.chunk)) || (rc = upload(h, L.chunk_soff, &lay.chunk_soff)) || (rc = upload(h, L.grp_desc, &lay.grp_desc)) || (rc = upload(h, L.grp_anchor, &lay.grp_anchor)) ||
         (rc = upload(h, L.grp_K, &lay.grp_K)) || (rc = upload(h, L.grp_chunks, &lay.grp_chunks)) ||
         (rc = upload(h, L.grp_lm_begin, &lay.grp_lm_begin)) || (rc = upload(h, L.grp_v_off, &lay.grp_v_off)) ||
         (rc = upload(h, L.grp_D_off, &lay.grp_D_off)) || (rc = upload(h, L.grp_d_off, &lay.grp_d_off)) ||
