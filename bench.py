@@ -180,7 +180,8 @@ def main() -> int:
     s.set_state(X0, p.alpha0.astype(np.float64))
     phases = s.profile_phases(args.steps)
     info = s.layout_info()
-    envelope = info["factorization"] == 2
+    envelope = info["factorization"] in (2, 3)   # one workgroup, or split at a separator
+    split = info["factorization"] == 3
     if envelope:
         # one kernel factors and solves; the "solve" phase holds no kernel
         phases["cholesky"] += phases.pop("solve")
@@ -193,12 +194,14 @@ def main() -> int:
     if dom == "cholesky":
         flops = envelope_flops(p) if envelope else dense_flops(p.n_poses)
         ach = flops / (phases[dom] * 1e-3) / 1e12
-        name = "k_chol_env" if envelope else "k_chol_panel + k_chol_update + k_chol_solve"
+        name = ("k_chol_split1 + k_chol_split2" if split else "k_chol_env") if envelope else \
+            "k_chol_panel + k_chol_update + k_chol_solve"
         roof = {"kernel": name, "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / FP64_PEAK_TFLOPS, "traffic": profile_traffic("k_chol_env") if envelope else None,
+                "frac": ach / FP64_PEAK_TFLOPS,
+                "traffic": profile_traffic("k_chol_split" if split else "k_chol_env") if envelope else None,
                 "algorithmic_flops": flops,
-                "note": "one workgroup walks the reduced system's profile column by column: latency-bound "
-                        "(the sequential column chain), FP64 peak is the whole chip's"}
+                "note": "the profile is walked column by column by one workgroup (two, split at a separator): "
+                        "latency-bound by the sequential column chain; FP64 peak is the whole chip's"}
     else:
         roof = dict(roof_asm)
         if dom != "assemble":
@@ -236,7 +239,7 @@ def main() -> int:
         "phases_ms": phases,
         "roofline": roof,
         "roofline_assemble": roof_asm,
-        "factorization": "envelope" if envelope else "dense",
+        "factorization": ("envelope, split" if split else "envelope") if envelope else "dense",
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -46,11 +46,16 @@ extern "C" {
 #define SPGO_PREC_FP64 0
 #define SPGO_PREC_MIXED 1
 
-/* Reduced-system factorisation: AUTO picks ENVELOPE when the block profile of
- * the reduced system is narrow, DENSE (multi-kernel blocked) otherwise. */
+/* Reduced-system factorisation.  ENVELOPE: the profile Cholesky in one
+ * workgroup; ENVELOPE_SPLIT: the same, split at a separator in the middle of
+ * the profile into a top and a bottom part factored concurrently; DENSE: the
+ * multi-kernel blocked Cholesky.  AUTO picks ENVELOPE_SPLIT for long narrow
+ * profiles (at least 2 D + 6 blocks of 16 rows, D the profile depth),
+ * ENVELOPE for short narrow ones and DENSE for wide ones. */
 #define SPGO_FACTOR_AUTO 0
 #define SPGO_FACTOR_DENSE 1
 #define SPGO_FACTOR_ENVELOPE 2
+#define SPGO_FACTOR_ENVELOPE_SPLIT 3
 
 typedef struct spgo_options {
     int32_t mode;            /* SPGO_MODE_*                                    */
@@ -81,7 +86,7 @@ typedef struct spgo_iter_stats {
 typedef struct spgo_layout_info {
     int32_t n_poses, n_slots, n_measurements, n_landmarks;
     int32_t n_groups, n_chunks, max_span, n_anchors_used;
-    int32_t factorization;   /* SPGO_FACTOR_DENSE or SPGO_FACTOR_ENVELOPE: the path spgo_step takes */
+    int32_t factorization;   /* SPGO_FACTOR_DENSE, _ENVELOPE or _ENVELOPE_SPLIT: the path spgo_step takes */
     int32_t profile_depth;   /* blocks of 16 below the diagonal in the reduced system's profile */
     int32_t reduced_size;    /* rows of the padded reduced pose system                          */
     int32_t reserved;

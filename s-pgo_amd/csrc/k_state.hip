@@ -26,8 +26,10 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
     const bool ok = init ? false : (ctl->chol_ok != 0);
 
     const int chunk = (N + WG - 1) / WG;
+    const int nact = (N + chunk - 1) / chunk;   // threads holding poses
     const int k0 = min(N, tid * chunk), k1 = min(N, k0 + chunk);
     double P[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    double Xkeep[12];   // with one pose per thread it stays in registers for the second pass
     for (int k = k0; k < k1; ++k) {
         double X[12], Xi[12], tmp[12];
         pose_load(bin.X + 12 * k, X);
@@ -43,6 +45,8 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
                 bout.X[12 * k + q] = tmp[q];
             }
         }
+#pragma unroll
+        for (int q = 0; q < 12; ++q) Xkeep[q] = X[q];
         pose_inv(X, Xi);
         pose_mul(P, Xi, tmp);
 #pragma unroll
@@ -52,7 +56,7 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
     for (int q = 0; q < 12; ++q) scan[0][tid][q] = P[q];
     __syncthreads();
     int src = 0;
-    for (int d = 1; d < WG; d <<= 1) {
+    for (int d = 1; d < nact; d <<= 1) {
         double out[12];
         if (tid >= d) {
             pose_mul(scan[src][tid - d], scan[src][tid], out);
@@ -89,7 +93,12 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
     if (tid == 0) emit(0, G);
     for (int k = k0; k < k1; ++k) {
         double X[12], Xi[12], tmp[12];
-        pose_load(bout.X + 12 * k, X);
+        if (chunk == 1) {
+#pragma unroll
+            for (int q = 0; q < 12; ++q) X[q] = Xkeep[q];
+        } else {
+            pose_load(bout.X + 12 * k, X);   // this thread's own stores: program order
+        }
         pose_inv(X, Xi);
         pose_mul(G, Xi, tmp);
 #pragma unroll

@@ -35,70 +35,59 @@ constexpr int FS_MAXG = 512;   // groups whose span can reach one pose row
 template <typename T>
 __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0, StateBuf<T> B1, Control* ctl,
                                                      double* H, double* gvec, int Pp) {
-    __shared__ long long goff[FS_MAXG];
-    __shared__ int ganc[FS_MAXG], gK[FS_MAXG], gdoff[FS_MAXG];
+    // the covering groups of this row's pose block (precomputed per layout, anchors
+    // ascending) with the offset of row `row` inside each group's block
+    __shared__ long long rowoff[FS_MAXG];
+    __shared__ int canc[FS_MAXG], gdo[FS_MAXG];
     const int row = blockIdx.x;
     const int tid = threadIdx.x;
-    const int P = 6 * lay.N;
     if (row == 0 && tid == 0) ctl->chol_ok = 1;   // reset for this iteration's factorisation
     const StateBuf<T>& bs = ctl->cur ? B1 : B0;
     const int bi = row / 6;
-    const int amin = max(0, bi - lay.kmax + 1);
-    // groups of anchors amin..bi (contiguous, sorted by anchor); K = 0 marks
-    // groups whose span does not reach pose bi.  The host keeps their number
-    // within FS_MAXG.
-    const int g0 = lay.anchor_grp[amin];
-    const int n = min(lay.anchor_grp[bi + 1] - g0, FS_MAXG);
-    for (int e = tid; e < n; e += 256) {
-        const int g = g0 + e;
-        const int a = lay.grp_anchor[g], K = lay.grp_K[g];
-        ganc[e] = a;
-        gK[e] = (bi - a < K) ? K : 0;
-        goff[e] = lay.grp_D_off[g];
-        gdoff[e] = lay.grp_d_off[g];
+    const int e0 = lay.fs_begin[bi];
+    const int nc = min(lay.fs_begin[bi + 1] - e0, FS_MAXG);
+    for (int e = tid; e < nc; e += 256) {
+        const FsEntry f = lay.fs_list[e0 + e];
+        canc[e] = f.anchor;
+        rowoff[e] = f.D_off + (long long)(row - 6 * f.anchor) * f.K6 - 6 * f.anchor;
+        gdo[e] = f.d_off + row - 6 * f.anchor;
     }
     __syncthreads();
-    // compact the covering groups in group order (anchors ascending), with the
-    // base offset of row `row` inside each group's block
-    __shared__ long long rowoff[FS_MAXG];
-    __shared__ int canc[FS_MAXG], cK6[FS_MAXG];
-    __shared__ int ncov;
-    if (tid == 0) {
-        int c = 0;
-        for (int e = 0; e < n; ++e) {
-            if (gK[e] == 0) continue;
-            const int a = ganc[e], K6 = 6 * gK[e];
-            canc[c] = a;
-            cK6[c] = K6;
-            rowoff[c] = goff[e] + (long long)(row - 6 * a) * K6 - 6 * a;
-            gdoff[c] = gdoff[e] + row - 6 * a;
-            ++c;
-        }
-        ncov = c;
-    }
-    __syncthreads();
-    const int nc = ncov;
+    // every lower entry of the row is written (zeros outside the profile), so a
+    // failed factorisation cannot leave values behind for the next one
     for (int col = tid; col <= row; col += 256) {
         const int bj = col / 6;
-        int ne = 0;
-        while (ne < nc && canc[ne] <= bj) ++ne;   // covering groups with anchor <= bj: a prefix
-        double h0 = 0.0, h1 = 0.0;
-        int e = 0;
-        for (; e + 1 < ne; e += 2) {
-            h0 += (double)bs.D[rowoff[e] + col];
-            h1 += (double)bs.D[rowoff[e + 1] + col];
+        // covering groups with anchor <= bj form a prefix (anchors ascending):
+        // their blocks reach column col
+        int lo = 0, hi = nc;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (canc[mid] <= bj) lo = mid + 1;
+            else hi = mid;
         }
-        if (e < ne) h0 += (double)bs.D[rowoff[e] + col];
+        const int ne = lo;
+        double h0 = 0.0, h1 = 0.0;
+        // loads in batches of 16 before their adds, so a long list is a few round
+        // trips; even entries into h0, odd into h1
+        for (int eb = 0; eb < ne; eb += 16) {
+            double v[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) v[i] = (eb + i < ne) ? (double)bs.D[rowoff[eb + i] + col] : 0.0;
+#pragma unroll
+            for (int i = 0; i < 16; i += 2) {
+                h0 += v[i];
+                h1 += v[i + 1];
+            }
+        }
         double h = h0 + h1;
         if (col == row) {
             h += ctl->lambda;
             double gs = 0.0;
-            for (int q = 0; q < ne; ++q) gs += (double)bs.d[gdoff[q]];
+            for (int q = 0; q < nc; ++q) gs += (double)bs.d[gdo[q]];
             gvec[row] = gs;
         }
         H[(long long)row * Pp + col] = h;
     }
-    (void)P;
 }
 
 template <typename T>
@@ -239,15 +228,15 @@ void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t 
 }
 
 // ---------------------------------------------------------------------------
-// Envelope (profile) Cholesky and both triangular solves in one workgroup.
+// Envelope (profile) Cholesky and both triangular solves.
 //
 // H is block-banded with EB x EB blocks: block row i has nonzero blocks only
 // in columns first[i] .. i (first[] nondecreasing, i - first[i] <= D), and
-// the Cholesky factor keeps that envelope.  The blocks that are still to be
-// updated live in a sliding LDS window of W = D + 3 block rows; block (i, j)
-// sits in the slot of the unordered pair (i mod W, j mod W), which is unique
-// among the resident rows.  Right-looking over block columns k, two barriers
-// per column:
+// the Cholesky factor keeps that envelope.  One workgroup factors a block
+// system right-looking over block columns k; the blocks still to be updated
+// live in a sliding LDS window of W = D + 3 block rows, block (i, j) in the
+// slot of the unordered pair (i mod W, j mod W), which is unique among the
+// resident rows.  Two barriers per column:
 //   phase A  wavefront 0 (and 1, 2 for profiles deeper than 3 blocks) factors
 //            the panel of column k from registers, one row per lane: lanes
 //            0-15 the diagonal block, lanes 16-63 three blocks below it (a
@@ -263,9 +252,17 @@ void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t 
 //            the backward solve) and prefetches the next block row.
 //   phase B  the trailing update of block column k + 1 (the next panel) on
 //            MFMA, one 16x16 tile per wavefront.
-// The backward solve
-// x_k = L_kk^-T (y_k - sum_i L_(k+i),k^T x_(k+i)) then runs block by block from
-// the bottom as products with the stored inverses, its loads one block ahead.
+// The backward solve x_k = L_kk^-T (y_k - sum_i L_(k+i),k^T x_(k+i)) runs block
+// by block from the bottom as products with the stored inverses.
+//
+// Long profiles are split in two (a twisted factorisation): a separator S of
+// D block rows in the middle leaves a top part T and a bottom part B that do
+// not couple.  One kernel factors T top-down and B bottom-up at the same time
+// in two workgroups (B as the reversed, transposed system J H J, read from
+// H's lower triangle); each leaves its Schur update of S behind.  A second
+// kernel sums the two updates, factors S, and runs the backward solves of T
+// and B concurrently.  The sequential chain drops from nb to about nb/2 + D
+// block columns.
 // ---------------------------------------------------------------------------
 constexpr int EB = 16;            // block size: one MFMA tile
 constexpr int EDMAX = 7;          // deepest profile: blocks below the diagonal
@@ -342,51 +339,86 @@ __device__ __forceinline__ void env_invert(const double* Lc, const double* dinv,
     }
 }
 
-template <int DT>
-__global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, const int* first, const double* g,
-                                                  double* x, double* winv, Control* ctl) {
-    constexpr int EPRE = 4 * (DT + 1);      // prefetched doubles per lane of one block row
-    extern __shared__ double lds[];
-    const int W = D + 3;
-    const int nb = Pp / EB;
-    double* win = lds;                                  // env_window_slots(D) blocks
-    double* ring = win + env_window_slots(D) * EBLK;    // last two diagonal factors, column-major
-    double* xs = ring + 2 * EBLK;                       // right-hand side, then solution (Pp)
-    double* dv = xs + Pp;                               // 1 / L_ii (Pp)
-    int* mk = (int*)(dv + Pp);                          // blocks below the diagonal per block column
-    int* fst = mk + nb;                                 // first[] staged
-    __shared__ int sbad;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    auto wrap = [W](int r) { return r >= W ? r - W : r; };
+// Where a local block system lives in global memory: local block row i is
+// block row i of H (rev = 0), or block row nb - 1 - i with the system
+// reversed and transposed (rev = 1, J H J read from H's lower triangle).
+struct EnvSrc {
+    double* H;    // lower triangle, row-major, row stride Pp
+    int Pp, nb, rev;
+};
 
-    for (int i = tid; i < Pp; i += 256) xs[i] = g[i];
-    for (int i = tid; i < nb; i += 256) fst[i] = first[i];
-    if (tid == 0) sbad = 0;
-    __syncthreads();
-    for (int k = tid; k < nb; k += 256) {
+__device__ __forceinline__ long long env_gaddr(const EnvSrc& s, int i, int j, int r, int c) {
+    if (!s.rev) return (long long)(EB * i + r) * s.Pp + EB * j + c;
+    return (long long)(EB * (s.nb - 1 - j) + (EB - 1 - c)) * s.Pp + EB * (s.nb - 1 - i) + (EB - 1 - r);
+}
+
+// One factorisation context in LDS, and the local system it works on.
+struct EnvCtx {
+    double* win;    // env_window_slots(D) blocks
+    double* ring;   // the last two diagonal factors, column-major
+    double* xs;     // right-hand side, then forward solution (EB * nloc)
+    double* dv;     // 1 / L_ii (EB * nloc)
+    int* mk;        // blocks below the diagonal per block column (nloc)
+    int* fst;       // local profile (nloc), staged by the caller
+    int nloc;       // block rows of the local system
+    int steps;      // block columns factored (the rest stay in the window)
+    int D;          // profile depth of the local system
+    int zero_from;  // blocks (i, j) with i, j >= zero_from load as zero
+    int wrev, wn;   // inverse diagonal block of column k goes to slot k, or wn - 1 - k
+};
+
+__device__ __forceinline__ int env_wslot(const EnvCtx& c, int k) { return c.wrev ? c.wn - 1 - k : k; }
+
+// blocks below the diagonal of every block column: the rows k+1.. with first <= k
+__device__ __forceinline__ void env_counts(const int* fst, int nloc, int D, int* mk) {
+    for (int k = threadIdx.x; k < nloc; k += 256) {
         int m = 0;
-        while (m < D && k + 1 + m < nb && fst[k + 1 + m] <= k) ++m;
+        while (m < D && k + 1 + m < nloc && fst[k + 1 + m] <= k) ++m;
         mk[k] = m;
     }
+}
+
+// Factor the first c.steps block columns of the local system and solve
+// forward.  Called by all 256 threads after c.fst and c.xs are staged and a
+// barrier.  On return (after a barrier) the window holds the blocks of rows
+// and columns >= c.steps with every update applied, c.xs the forward
+// solution of rows < EB * steps and the updated right-hand side of the rest,
+// and L below the diagonal and the inverse diagonal blocks are stored to
+// global memory (not yet waited for).
+template <int DT>
+__device__ void env_forward(const EnvSrc& src, const EnvCtx& c, double* winv, int* sbad) {
+    constexpr int EPRE = 4 * (DT + 1);      // prefetched doubles per lane of one block row
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int D = c.D, W = D + 3, nloc = c.nloc;
+    double* win = c.win;
+    double* ring = c.ring;
+    double* xs = c.xs;
+    double* dv = c.dv;
+    int* mk = c.mk;
+    const int* fst = c.fst;
+    auto wrap = [W](int r) { return r >= W ? r - W : r; };
+    auto zero_blk = [&](int i, int j) { return i >= c.zero_from && j >= c.zero_from; };
+
+    env_counts(fst, nloc, D, mk);
     // rows 0 .. D of the window; every thread owns one element of each block
     {
-        const int r = tid >> 4, c = tid & 15;
+        const int r = tid >> 4, cc = tid & 15;
         double v[(DT + 1) * (DT + 2) / 2];
         int n = 0;
 #pragma unroll
         for (int i = 0; i <= DT; ++i)
 #pragma unroll
             for (int j = 0; j <= i; ++j) {
-                const bool use = i <= D && i < nb && j >= fst[min(i, nb - 1)];
-                v[n++] = use ? H[(long long)(EB * i + r) * Pp + EB * j + c] : 0.0;
+                const bool use = i <= D && i < nloc && j >= fst[min(i, nloc - 1)] && !zero_blk(i, j);
+                v[n++] = use ? src.H[env_gaddr(src, i, j, r, cc)] : 0.0;
             }
         n = 0;
 #pragma unroll
         for (int i = 0; i <= DT; ++i)
 #pragma unroll
             for (int j = 0; j <= i; ++j) {
-                const bool use = i <= D && i < nb && j >= fst[min(i, nb - 1)];
-                if (use) env_slot(win, i, j)[r * EST + c] = v[n];
+                const bool use = i <= D && i < nloc && j >= fst[min(i, nloc - 1)];
+                if (use) env_slot(win, i, j)[r * EST + cc] = v[n];
                 ++n;
             }
     }
@@ -395,19 +427,46 @@ __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, cons
     // and only the stores to the window are predicated.
     double pre[EPRE];
     auto prefetch = [&](int i) {
-        const int ic = min(i, nb - 1);
+        const int ic = min(i, nloc - 1);
         const int f0 = fst[ic];
 #pragma unroll
         for (int t = 0; t < EPRE; ++t) {
             const int e = lane + 64 * t, b = min(e >> 8, ic - f0), rr = (e >> 4) & 15, cc = e & 15;
-            pre[t] = H[(long long)(EB * ic + rr) * Pp + EB * (f0 + b) + cc];
+            pre[t] = src.H[env_gaddr(src, ic, f0 + b, rr, cc)];
         }
     };
-    if (w == 3) prefetch(D + 1);
+    if (w == 3 && D + 1 < nloc) prefetch(D + 1);
     __syncthreads();
 
+    // panel kp to global memory for the backward solve: the inverse of its
+    // diagonal factor and the blocks below it (one wavefront)
+    auto copy_out = [&](int kp, int rkp) {
+        env_invert(ring + (kp & 1) * EBLK, dv + EB * kp, winv + (long long)env_wslot(c, kp) * EB * EB, lane);
+        const int mp = mk[kp];
+        for (int i = 1; i <= mp; ++i) {
+            const double* s = env_slot(win, wrap(rkp + i), rkp);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int e = lane + 64 * t, rr = e >> 4, cc = e & 15;
+                src.H[env_gaddr(src, kp + i, kp, rr, cc)] = s[rr * EST + cc];
+            }
+        }
+    };
+    // rest of column kp's trailing update: tiles (kp+i, kp+j), 2 <= j <= i <= mp,
+    // on wavefronts w0 .. 3
+    auto rest_tiles = [&](int kp, int rkp, int w0) {
+        const int mp = mk[kp];
+        const int nt = (mp - 1) * mp / 2;
+        for (int t = w - w0; t < nt; t += 4 - w0) {
+            int i = 2;
+            while ((i - 1) * i / 2 <= t) ++i;
+            const int j = 2 + t - (i - 2) * (i - 1) / 2;
+            env_tile(win, wrap(rkp + i), wrap(rkp + j), rkp, lane);
+        }
+    };
+
     int rk = 0;   // k mod W
-    for (int k = 0; k < nb; ++k, rk = wrap(rk + 1)) {
+    for (int k = 0; k < c.steps; ++k, rk = wrap(rk + 1)) {
         const int m = mk[k];
         const int nf = m > 3 ? (m + 2) / 3 : 1;         // factoring wavefronts
         // ---------------- phase A
@@ -418,9 +477,9 @@ __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, cons
             const int bic = valid ? bi : 0;
             double a[EB];
             {
-                const double* src = env_slot(win, wrap(rk + bic), rk) + rr * EST;
+                const double* s = env_slot(win, wrap(rk + bic), rk) + rr * EST;
 #pragma unroll
-                for (int c = 0; c < EB; ++c) a[c] = valid ? src[c] : 0.0;
+                for (int cc = 0; cc < EB; ++cc) a[cc] = valid ? s[cc] : 0.0;
             }
             double b = valid ? xs[EB * (k + bic) + rr] : 0.0;
             double yv = 0.0, iv = 0.0;
@@ -457,130 +516,278 @@ __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, cons
                     if (w == 0) {
                         double* rg = ring + (k & 1) * EBLK + rr;
 #pragma unroll
-                        for (int c = 0; c < EB; ++c) rg[c * EST] = (c <= rr) ? a[c] : 0.0;
+                        for (int cc = 0; cc < EB; ++cc) rg[cc * EST] = (cc <= rr) ? a[cc] : 0.0;
                         dv[EB * k + rr] = iv;
                         xs[EB * k + rr] = yv;
                     }
                 } else {
                     double* wl = env_slot(win, wrap(rk + bi), rk) + rr * EST;
 #pragma unroll
-                    for (int c = 0; c < EB; ++c) wl[c] = a[c];
+                    for (int cc = 0; cc < EB; ++cc) wl[cc] = a[cc];
                     xs[EB * (k + bi) + rr] = b;
                 }
             }
-            if (bad && lane == 0) sbad = 1;
+            if (bad && lane == 0) *sbad = 1;
         } else {
             // wavefront 3: the prefetched block row k + D + 1 into its slot (row
             // k - 2's, free since the last phase A)
             const int inew = k + D + 1;
-            if (w == 3 && inew < nb) {
+            if (w == 3 && inew < nloc) {
                 const int f0 = fst[inew], nbl = inew - f0 + 1, ri = wrap(rk + D + 1);
 #pragma unroll
                 for (int t = 0; t < EPRE; ++t) {
                     const int e = lane + 64 * t, bb = e >> 8, rr = (e >> 4) & 15, cc = e & 15;
-                    if (bb < nbl) env_slot(win, ri, wrap(rk + f0 + bb - k))[rr * EST + cc] = pre[t];
+                    if (bb < nbl)
+                        env_slot(win, ri, wrap(rk + f0 + bb - k))[rr * EST + cc] =
+                            zero_blk(inew, f0 + bb) ? 0.0 : pre[t];
                 }
             }
             if (k >= 1) {
-                const int kp = k - 1, mp = mk[kp], rkp = rk == 0 ? W - 1 : rk - 1;
-                // rest of column kp's trailing update: tiles (kp+i, kp+j), 2 <= j <= i <= mp
-                const int nt = (mp - 1) * mp / 2;
-                for (int t = w - nf; t < nt; t += 4 - nf) {
-                    int i = 2;
-                    while ((i - 1) * i / 2 <= t) ++i;
-                    const int j = 2 + t - (i - 2) * (i - 1) / 2;
-                    env_tile(win, wrap(rkp + i), wrap(rkp + j), rkp, lane);
-                }
-                if (w == 3) {
-                    // panel kp to global memory for the backward solve: the
-                    // inverse of its diagonal factor and the blocks below it
-                    env_invert(ring + (kp & 1) * EBLK, dv + EB * kp, winv + (long long)kp * EB * EB, lane);
-                    for (int i = 1; i <= mp; ++i) {
-                        const double* src = env_slot(win, wrap(rkp + i), rkp);
-                        double* dst = H + (long long)EB * (kp + i) * Pp + EB * kp;
-#pragma unroll
-                        for (int t = 0; t < 4; ++t) {
-                            const int e = lane + 64 * t, rr = e >> 4, cc = e & 15;
-                            dst[(long long)rr * Pp + cc] = src[rr * EST + cc];
-                        }
-                    }
-                }
+                const int rkp = rk == 0 ? W - 1 : rk - 1;
+                rest_tiles(k - 1, rkp, nf);
+                if (w == 3) copy_out(k - 1, rkp);
             }
             // wavefront 3: prefetch block row k + D + 2 (global stores and
             // loads are issued last, so no wavefront waits on them in this phase)
-            if (w == 3 && inew + 1 < nb) prefetch(inew + 1);
+            if (w == 3 && inew + 1 < nloc) prefetch(inew + 1);
         }
         lds_barrier();
         // ---------------- phase B: tiles (k + i, k + 1), 1 <= i <= m
         for (int t = w; t < m; t += 4) env_tile(win, wrap(rk + 1 + t), wrap(rk + 1), rk, lane);
         lds_barrier();
     }
-    if (w == 3) env_invert(ring + ((nb - 1) & 1) * EBLK, dv + EB * (nb - 1), winv + (long long)(nb - 1) * EB * EB, lane);
-    __syncthreads();   // the factor and the inverses in global memory are complete
-
-    // backward solve by wavefront 0.  Lane (c, g) sums rows g, g+4, g+8, g+12
-    // of column c over the blocks below the diagonal; z = y_k - that sum goes
-    // through LDS so each lane can form its share of x_k = L_kk^-T z from its
-    // four entries of column c of L_kk^-1.  The next block's loads are issued
-    // before the current block is solved.
-    if (w == 0) {
-        const int c = lane & 15, g4 = lane >> 4;
-        double pl[2][4 * DT], wl[2][4];
-        auto load = [&](int k, double* P, double* Wc) {
-            const int kc = max(k, 0);
-            const int m = mk[kc];
-#pragma unroll
-            for (int i = 1; i <= DT; ++i)
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    P[4 * (i - 1) + t] = H[(long long)(EB * (kc + min(i, m)) + g4 + 4 * t) * Pp + EB * kc + c];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) Wc[t] = winv[(long long)kc * EB * EB + (g4 + 4 * t) * EB + c];
-        };
-        auto solve = [&](int k, const double* P, const double* Wc) {
-            const int m = mk[k];
-            double s = 0.0;
-#pragma unroll
-            for (int i = 1; i <= DT; ++i) {
-                if (i <= m) {
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) s += P[4 * (i - 1) + t] * xs[EB * (k + i) + g4 + 4 * t];
-                }
-            }
-            s += __shfl_xor(s, 16, 64);
-            s += __shfl_xor(s, 32, 64);
-            const double z = xs[EB * k + c] - s;
-            wave_lds_order();
-            if (lane < EB) xs[EB * k + c] = z;
-            wave_lds_order();
-            double xv = 0.0;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) xv += Wc[t] * xs[EB * k + g4 + 4 * t];
-            xv += __shfl_xor(xv, 16, 64);
-            xv += __shfl_xor(xv, 32, 64);
-            wave_lds_order();
-            if (lane < EB) xs[EB * k + c] = xv;
-            wave_lds_order();
-        };
-        load(nb - 1, pl[0], wl[0]);
-        int k = nb - 1;
-        for (; k >= 1; k -= 2) {
-            load(k - 1, pl[1], wl[1]);
-            solve(k, pl[0], wl[0]);
-            load(k - 2, pl[0], wl[0]);
-            solve(k - 1, pl[1], wl[1]);
-        }
-        if (k == 0) solve(0, pl[0], wl[0]);
+    if (c.steps > 0) {
+        // the last factored column: its remaining trailing update and its copy
+        const int kp = c.steps - 1, rkp = rk == 0 ? W - 1 : rk - 1;
+        rest_tiles(kp, rkp, 0);
+        lds_barrier();
+        if (w == 3) copy_out(kp, rkp);
     }
+    lds_barrier();
+}
+
+// Backward solve of the first `steps` block rows of a local system by one
+// wavefront, from the bottom: xs holds the forward solution of those rows and
+// the final solution of the rows after them.  Lane (c, g) sums rows g, g+4,
+// g+8, g+12 of column c over the blocks below the diagonal; z = y_k - that
+// sum goes through LDS so each lane can form its share of x_k = L_kk^-T z
+// from its four entries of column c of L_kk^-1.  The next block's loads are
+// issued before the current block is solved.
+template <int DT>
+__device__ void env_backward(const EnvSrc& src, const EnvCtx& c, const double* winv, int lane) {
+    const int cl = lane & 15, g4 = lane >> 4;
+    const int* mk = c.mk;
+    double* xs = c.xs;
+    double pl[2][4 * DT], wl[2][4];
+    auto load = [&](int k, double* P, double* Wc) {
+        const int kc = max(k, 0);
+        const int m = mk[kc];
+#pragma unroll
+        for (int i = 1; i <= DT; ++i)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) P[4 * (i - 1) + t] = src.H[env_gaddr(src, kc + min(i, m), kc, g4 + 4 * t, cl)];
+        const double* Wk = winv + (long long)env_wslot(c, kc) * EB * EB;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) Wc[t] = Wk[(g4 + 4 * t) * EB + cl];
+    };
+    auto solve = [&](int k, const double* P, const double* Wc) {
+        const int m = mk[k];
+        double s = 0.0;
+#pragma unroll
+        for (int i = 1; i <= DT; ++i) {
+            if (i <= m) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) s += P[4 * (i - 1) + t] * xs[EB * (k + i) + g4 + 4 * t];
+            }
+        }
+        s += __shfl_xor(s, 16, 64);
+        s += __shfl_xor(s, 32, 64);
+        const double z = xs[EB * k + cl] - s;
+        wave_lds_order();
+        if (lane < EB) xs[EB * k + cl] = z;
+        wave_lds_order();
+        double xv = 0.0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) xv += Wc[t] * xs[EB * k + g4 + 4 * t];
+        xv += __shfl_xor(xv, 16, 64);
+        xv += __shfl_xor(xv, 32, 64);
+        wave_lds_order();
+        if (lane < EB) xs[EB * k + cl] = xv;
+        wave_lds_order();
+    };
+    if (c.steps <= 0) return;
+    load(c.steps - 1, pl[0], wl[0]);
+    int k = c.steps - 1;
+    for (; k >= 1; k -= 2) {
+        load(k - 1, pl[1], wl[1]);
+        solve(k, pl[0], wl[0]);
+        load(k - 2, pl[0], wl[0]);
+        solve(k - 1, pl[1], wl[1]);
+    }
+    if (k == 0) solve(0, pl[0], wl[0]);
+}
+
+// LDS carve-up of one context (window for depth D, nloc block rows)
+__device__ __forceinline__ EnvCtx env_ctx(double* base, int D, int nloc, double** end) {
+    EnvCtx c;
+    c.win = base;
+    c.ring = c.win + env_window_slots(D) * EBLK;
+    c.xs = c.ring + 2 * EBLK;
+    c.dv = c.xs + EB * nloc;
+    c.mk = (int*)(c.dv + EB * nloc);
+    c.fst = c.mk + nloc;
+    *end = (double*)(c.fst + ((nloc + 1) & ~1));
+    c.nloc = nloc;
+    c.steps = nloc;
+    c.D = D;
+    c.zero_from = nloc;
+    c.wrev = 0;
+    c.wn = nloc;
+    return c;
+}
+
+__host__ __device__ inline size_t env_ctx_bytes(int D, int nloc) {
+    return (size_t)(env_window_slots(D) + 2) * EBLK * sizeof(double) + (size_t)2 * EB * nloc * sizeof(double) +
+           (size_t)((nloc + 1) & ~1) * 2 * sizeof(int);
+}
+
+// ---- one workgroup: the whole system ---------------------------------------
+template <int DT>
+__global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, const int* first, const double* g,
+                                                  double* x, double* winv, Control* ctl) {
+    extern __shared__ double lds[];
+    __shared__ int sbad;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int nb = Pp / EB;
+    double* end;
+    EnvCtx c = env_ctx(lds, D, nb, &end);
+    for (int i = tid; i < Pp; i += 256) c.xs[i] = g[i];
+    for (int i = tid; i < nb; i += 256) c.fst[i] = first[i];
+    if (tid == 0) sbad = 0;
     __syncthreads();
-    for (int i = tid; i < Pp; i += 256) x[i] = xs[i];
+    const EnvSrc src{H, Pp, nb, 0};
+    env_forward<DT>(src, c, winv, &sbad);
+    __syncthreads();   // the factor and the inverses in global memory are complete
+    if (w == 0) env_backward<DT>(src, c, winv, lane);
+    __syncthreads();
+    for (int i = tid; i < Pp; i += 256) x[i] = c.xs[i];
     if (tid == 0 && sbad) ctl->chol_ok = 0;
 }
 
-size_t chol_band_lds(int Pp, int D) {
-    const int nb = Pp / EB;
-    return (size_t)(env_window_slots(D) + 2) * EBLK * sizeof(double) + (size_t)2 * Pp * sizeof(double) +
-           (size_t)2 * nb * sizeof(int);
+// ---- split factorisation, kernel 1: top part top-down (workgroup 0) and bottom
+// part bottom-up (workgroup 1), each up to the separator [s0, s1) ------------
+// Separator export (per half h): blocks (I, J), J <= I < nS, of the local
+// system's rows steps + I in S + h nS^2 EB^2, and its right-hand side in
+// S + 2 nS^2 EB^2 + h EB nS.
+template <int DT>
+__global__ __launch_bounds__(256) void k_chol_split1(double* H, int Pp, int D, int s0, int s1, const int* firstT,
+                                                     const int* firstB, const double* g, double* x, double* winv,
+                                                     double* S, Control* ctl) {
+    extern __shared__ double lds[];
+    __shared__ int sbad;
+    const int tid = threadIdx.x;
+    const int half = blockIdx.x;
+    const int nb = Pp / EB, nS = s1 - s0;
+    const int nloc = half ? nb - s0 : s1;
+    double* end;
+    EnvCtx c = env_ctx(lds, D, nloc, &end);
+    c.steps = half ? nb - s1 : s0;
+    c.zero_from = half ? c.steps : nloc;   // the bottom half adds only its own update of S
+    c.wrev = half;
+    c.wn = nb;
+    const int* fsrc = half ? firstB : firstT;
+    for (int i = tid; i < nloc; i += 256) c.fst[i] = fsrc[i];
+    for (int i = tid; i < EB * nloc; i += 256) {
+        if (!half) c.xs[i] = g[i];
+        else c.xs[i] = i < EB * c.steps ? g[Pp - 1 - i] : 0.0;
+    }
+    if (tid == 0) sbad = 0;
+    __syncthreads();
+    const EnvSrc src{H, Pp, nb, half};
+    env_forward<DT>(src, c, winv, &sbad);
+    // forward solution of the factored rows, and the separator's blocks and right-hand side
+    for (int i = tid; i < EB * c.steps; i += 256) x[half ? Pp - 1 - i : i] = c.xs[i];
+    const int W = D + 3;
+    double* Sh = S + (long long)half * nS * nS * EB * EB;
+    for (int e = tid; e < nS * nS * EB * EB; e += 256) {
+        const int blk = e / (EB * EB), I = blk / nS, J = blk % nS, rc = e % (EB * EB);
+        if (J <= I) {
+            const int i = c.steps + I, j = c.steps + J;
+            Sh[e] = env_slot(c.win, i % W, j % W)[(rc / EB) * EST + rc % EB];
+        }
+    }
+    for (int t = tid; t < EB * nS; t += 256) S[(long long)2 * nS * nS * EB * EB + half * EB * nS + t] = c.xs[EB * c.steps + t];
+    if (tid == 0 && sbad) ctl->chol_ok = 0;
+}
+
+// ---- split factorisation, kernel 2: the separator (sum of both updates,
+// factored and solved), then the backward solves of both parts ---------------
+template <int DT>
+__global__ __launch_bounds__(256) void k_chol_split2(double* H, int Pp, int D, int s0, int s1, const int* firstT,
+                                                     const int* firstB, double* x, double* winv, const double* S,
+                                                     double* Sm, double* winvS, Control* ctl) {
+    extern __shared__ double lds[];
+    __shared__ int sbad;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int nb = Pp / EB, nS = s1 - s0, PS = EB * nS;
+    const int nT = s1, nB = nb - s0;   // local systems of the two parts, separator included
+    double* end;
+    EnvCtx cs = env_ctx(lds, nS - 1, nS, &end);
+    EnvCtx ct = env_ctx(end, 0, nT, &end);      // only xs, mk, fst of these two are used
+    EnvCtx cb = env_ctx(end, 0, nB, &end);
+    ct.steps = s0;
+    cb.steps = nb - s1;
+    cb.wrev = 1;
+    cb.wn = nb;
+    // separator = top update + bottom update (reversed and transposed), lower blocks
+    const double* St = S;
+    const double* Sb = S + (long long)nS * nS * EB * EB;
+    for (int e = tid; e < PS * PS; e += 256) {
+        const int row = e / PS, col = e % PS;
+        if (col / EB <= row / EB) {
+            const int I = row / EB, J = col / EB, r = row % EB, cc = col % EB;
+            const int Ib = nS - 1 - J, Jb = nS - 1 - I;   // the bottom system's block (Ib >= Jb)
+            Sm[e] = St[((long long)(I * nS + J) * EB + r) * EB + cc] +
+                    Sb[((long long)(Ib * nS + Jb) * EB + (EB - 1 - cc)) * EB + (EB - 1 - r)];
+        }
+    }
+    const double* Sr = S + (long long)2 * nS * nS * EB * EB;
+    for (int t = tid; t < PS; t += 256) cs.xs[t] = Sr[t] + Sr[PS + PS - 1 - t];
+    for (int i = tid; i < nS; i += 256) cs.fst[i] = 0;
+    for (int i = tid; i < nT; i += 256) ct.fst[i] = firstT[i];
+    for (int i = tid; i < nB; i += 256) cb.fst[i] = firstB[i];
+    if (tid == 0) sbad = 0;
+    __syncthreads();   // also makes Sm visible to the workgroup
+    const EnvSrc ss{Sm, PS, nS, 0};
+    env_forward<DT>(ss, cs, winvS, &sbad);
+    env_counts(ct.fst, nT, D, ct.mk);
+    env_counts(cb.fst, nB, D, cb.mk);
+    __syncthreads();
+    if (w == 0) env_backward<DT>(ss, cs, winvS, lane);
+    __syncthreads();
+    // the two parts: forward solutions of their rows, the separator's solution after them
+    for (int i = tid; i < EB * s0; i += 256) ct.xs[i] = x[i];
+    for (int i = tid; i < EB * (nb - s1); i += 256) cb.xs[i] = x[Pp - 1 - i];
+    for (int t = tid; t < PS; t += 256) {
+        ct.xs[EB * s0 + t] = cs.xs[t];
+        cb.xs[EB * (nb - s1) + t] = cs.xs[PS - 1 - t];
+    }
+    __syncthreads();
+    if (w == 0) env_backward<DT>(EnvSrc{H, Pp, nb, 0}, ct, winv, lane);
+    if (w == 1) env_backward<DT>(EnvSrc{H, Pp, nb, 1}, cb, winv, lane);
+    __syncthreads();
+    for (int i = tid; i < EB * s0; i += 256) x[i] = ct.xs[i];
+    for (int t = tid; t < PS; t += 256) x[EB * s0 + t] = cs.xs[t];
+    for (int i = tid; i < EB * (nb - s1); i += 256) x[Pp - 1 - i] = cb.xs[i];
+    if (tid == 0 && sbad) ctl->chol_ok = 0;
+}
+
+size_t chol_band_lds(int Pp, int D) { return env_ctx_bytes(D, Pp / EB); }
+
+size_t chol_split_lds(int Pp, int D, int part) {
+    const int nb = Pp / EB, nS = D > 1 ? D : 1;   // the separator has max(D, 1) block rows
+    if (part == 1) return env_ctx_bytes(D, nb);   // either half has fewer than nb block rows
+    return env_ctx_bytes(nS - 1, nS) + env_ctx_bytes(0, nb) * 2;
 }
 
 void launch_chol_band(double* H, int Pp, int D, const int* first, const double* g, double* x, double* winv,
@@ -593,10 +800,34 @@ void launch_chol_band(double* H, int Pp, int D, const int* first, const double* 
                            winv, ctl);
 }
 
+void launch_chol_split(double* H, int Pp, int D, int s0, int s1, const int* firstT, const int* firstB,
+                       const double* g, double* x, double* winv, double* S, double* Sm, double* winvS, Control* ctl,
+                       hipStream_t s) {
+    if (D <= 3) {
+        hipLaunchKernelGGL(k_chol_split1<3>, dim3(2), dim3(256), chol_split_lds(Pp, D, 1), s, H, Pp, D, s0, s1,
+                           firstT, firstB, g, x, winv, S, ctl);
+        hipLaunchKernelGGL(k_chol_split2<3>, dim3(1), dim3(256), chol_split_lds(Pp, D, 2), s, H, Pp, D, s0, s1,
+                           firstT, firstB, x, winv, S, Sm, winvS, ctl);
+    } else {
+        hipLaunchKernelGGL(k_chol_split1<EDMAX>, dim3(2), dim3(256), chol_split_lds(Pp, D, 1), s, H, Pp, D, s0, s1,
+                           firstT, firstB, g, x, winv, S, ctl);
+        hipLaunchKernelGGL(k_chol_split2<EDMAX>, dim3(1), dim3(256), chol_split_lds(Pp, D, 2), s, H, Pp, D, s0, s1,
+                           firstT, firstB, x, winv, S, Sm, winvS, ctl);
+    }
+}
+
 int prepare_chol_band(int Pp, int D) {
     const int bytes = (int)chol_band_lds(Pp, D);
     const void* f = D <= 3 ? (const void*)k_chol_env<3> : (const void*)k_chol_env<EDMAX>;
     return (int)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+int prepare_chol_split(int Pp, int D) {
+    const void* f1 = D <= 3 ? (const void*)k_chol_split1<3> : (const void*)k_chol_split1<EDMAX>;
+    const void* f2 = D <= 3 ? (const void*)k_chol_split2<3> : (const void*)k_chol_split2<EDMAX>;
+    int rc = (int)hipFuncSetAttribute(f1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)chol_split_lds(Pp, D, 1));
+    if (rc) return rc;
+    return (int)hipFuncSetAttribute(f2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)chol_split_lds(Pp, D, 2));
 }
 
 int chol_band_max_blocks() { return EDMAX; }

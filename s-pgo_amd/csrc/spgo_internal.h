@@ -23,6 +23,13 @@ struct alignas(16) GroupDesc {
     int soff0[KMAX + 2];          // slot offsets of the first chunk
 };
 
+// One landmark group whose pose span covers a given pose row block (the
+// reduced system's row sums run over these; built once per layout).
+struct FsEntry {
+    int anchor, K6, d_off, pad;
+    long long D_off;
+};
+
 // Device control block: which state buffer is current, damping, cost.
 struct Control {
     int cur;                // index (0/1) of the accepted state buffers
@@ -77,6 +84,8 @@ struct Layout {
     const int* grp_d_off;         // offset of the group's 6K vector
     const int* anchor_grp;        // CSR over anchor frames, F+1 entries
     const GroupDesc* grp_desc;    // per group, for k_assemble
+    const int* fs_begin;          // per pose row block (N + 1): range in fs_list
+    const FsEntry* fs_list;       // covering groups per pose row block, anchors ascending
 };
 
 // One of the two state/linearisation buffer sets (current and candidate).

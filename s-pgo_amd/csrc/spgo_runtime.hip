@@ -62,6 +62,8 @@ struct HostLayout {
     std::vector<long long> grp_v_off, grp_D_off;
     std::vector<int> anchor_grp;
     std::vector<GroupDesc> grp_desc;
+    std::vector<int> fs_begin;
+    std::vector<FsEntry> fs_list;
     int kmax = 0;
     long long v_total = 0, D_total = 0, d_total = 0;
     int anchors_used = 0;
@@ -111,6 +113,10 @@ struct spgo_solver {
     bool band = false;          // envelope Cholesky in one workgroup (narrow profile)
     const int* first = nullptr; // first nonzero block column of every block row
     int band_depth = 0;         // max over block rows of (row - first)
+    int path = SPGO_FACTOR_DENSE;   // factorisation taken: SPGO_FACTOR_DENSE, _ENVELOPE or _ENVELOPE_SPLIT
+    int s0 = 0, s1 = 0;         // split: separator block rows [s0, s1)
+    const int* firstB = nullptr;   // split: profile of the reversed bottom part
+    double *Ssep = nullptr, *Smrg = nullptr, *winvS = nullptr;   // split: separator scratch
     Control* ctl = nullptr;
     IterRecord* rec = nullptr;
     int rec_cap = 0;
@@ -326,6 +332,16 @@ int build_layout(spgo_solver* h, HostLayout& out) {
         d.chunk0 = hl.chunk[d.chunk_begin];
         for (int sl = 0; sl < KMAX + 2; ++sl) d.soff0[sl] = hl.chunk_soff[(size_t)d.chunk_begin * (KMAX + 2) + sl];
     }
+    // covering groups of every pose row block, anchors ascending
+    hl.fs_begin.assign(h->N + 1, 0);
+    for (int bi = 0; bi < h->N; ++bi) {
+        hl.fs_begin[bi] = (int)hl.fs_list.size();
+        for (int a = std::max(0, bi - hl.kmax + 1); a <= bi; ++a)
+            for (int g = hl.anchor_grp[a]; g < hl.anchor_grp[a + 1]; ++g)
+                if (bi - a < hl.grp_K[g])
+                    hl.fs_list.push_back(FsEntry{a, 6 * hl.grp_K[g], hl.grp_d_off[g], 0, hl.grp_D_off[g]});
+    }
+    hl.fs_begin[h->N] = (int)hl.fs_list.size();
     out = std::move(hl);
     return SPGO_OK;
 }
@@ -337,8 +353,9 @@ static int reduced_size(int n_poses) { return ((6 * n_poses + NB - 1) / NB) * NB
 // Block profile of the reduced system for the envelope factorisation: pose j
 // couples back to the smallest anchor among the landmark groups covering it,
 // fb[i] is the first nonzero block column of block row i and depth the
-// largest i - fb[i].  Returns whether the envelope path is used.
-static bool use_envelope(const spgo_solver* h, const HostLayout& L, std::vector<int>& fb, int& depth) {
+// largest i - fb[i].  Returns the factorisation path (SPGO_FACTOR_DENSE,
+// _ENVELOPE or _ENVELOPE_SPLIT), or -1 if the requested one cannot be used.
+static int factor_path(const spgo_solver* h, const HostLayout& L, std::vector<int>& fb, int& depth) {
     const int P = 6 * h->N, Pp = reduced_size(h->N), BBk = chol_band_block();
     std::vector<int> fpose(h->N);
     for (int j = 0; j < h->N; ++j) fpose[j] = j;
@@ -356,8 +373,18 @@ static bool use_envelope(const spgo_solver* h, const HostLayout& L, std::vector<
         fb[i] = f;
         depth = std::max(depth, i - f);
     }
-    const bool fits = depth <= chol_band_max_blocks() && chol_band_lds(Pp, depth) <= 160 * 1024 - 1024;
-    return fits && h->opt.factorization != SPGO_FACTOR_DENSE;
+    const int lim = 160 * 1024 - 1024;
+    const bool fits = depth <= chol_band_max_blocks() && chol_band_lds(Pp, depth) <= (size_t)lim;
+    const int nS = std::max(depth, 1);
+    const bool split_fits = fits && nbk >= nS + 2 && chol_split_lds(Pp, nS, 2) <= (size_t)lim;
+    switch (h->opt.factorization) {
+    case SPGO_FACTOR_DENSE: return SPGO_FACTOR_DENSE;
+    case SPGO_FACTOR_ENVELOPE: return fits ? SPGO_FACTOR_ENVELOPE : -1;
+    case SPGO_FACTOR_ENVELOPE_SPLIT: return split_fits ? SPGO_FACTOR_ENVELOPE_SPLIT : -1;
+    default:
+        if (split_fits && nbk >= 2 * depth + 6) return SPGO_FACTOR_ENVELOPE_SPLIT;
+        return fits ? SPGO_FACTOR_ENVELOPE : SPGO_FACTOR_DENSE;
+    }
 }
 
 template <typename T>
@@ -434,7 +461,7 @@ int setup_layout(spgo_solver* h) {
     if ((rc = upload(h, L.ob_t, &lay.ob_t)) || (rc = upload(h, L.ob_meta, &lay.ob_meta)) ||
         (rc = upload(h, L.ob_uv, &lay.ob_uv)) || (rc = upload(h, L.lm_id, &lay.lm_id)) ||
         (rc = upload(h, L.lm_ray, &lay.lm_ray)) || (rc = upload(h, L.lm_pos, &lay.lm_pos)) ||
-        (rc = upload(h, L.chunk, &lay.chunk)) || (rc = upload(h, L.chunk_soff, &lay.chunk_soff)) || (rc = upload(h, L.grp_desc, &lay.grp_desc)) || (rc = upload(h, L.grp_anchor, &lay.grp_anchor)) ||
+        (rc = upload(h, L.chunk, &lay.chunk)) || (rc = upload(h, L.chunk_soff, &lay.chunk_soff)) || (rc = upload(h, L.grp_desc, &lay.grp_desc)) || (rc = upload(h, L.fs_begin, &lay.fs_begin)) || (rc = upload(h, L.fs_list, &lay.fs_list)) || (rc = upload(h, L.grp_anchor, &lay.grp_anchor)) ||
         (rc = upload(h, L.grp_K, &lay.grp_K)) || (rc = upload(h, L.grp_chunks, &lay.grp_chunks)) ||
         (rc = upload(h, L.grp_lm_begin, &lay.grp_lm_begin)) || (rc = upload(h, L.grp_v_off, &lay.grp_v_off)) ||
         (rc = upload(h, L.grp_D_off, &lay.grp_D_off)) || (rc = upload(h, L.grp_d_off, &lay.grp_d_off)) ||
@@ -458,13 +485,38 @@ int setup_layout(spgo_solver* h) {
         }
         std::vector<int> fb;
         int bw = 0;
-        h->band = use_envelope(h, L, fb, bw);
-        if (h->opt.factorization == SPGO_FACTOR_ENVELOPE && !h->band)
-            return h->fail(SPGO_E_LAYOUT, "reduced-system profile too wide for the envelope factorisation");
+        h->path = factor_path(h, L, fb, bw);
+        if (h->path < 0)
+            return h->fail(SPGO_E_LAYOUT, "the reduced-system profile does not fit the requested factorisation");
+        h->band = h->path != SPGO_FACTOR_DENSE;
         if ((rc = upload(h, fb, &h->first))) return rc;
         h->band_depth = bw;
-        if (h->band && prepare_chol_band(h->Pp, bw) != 0)
+        if (h->path == SPGO_FACTOR_ENVELOPE && prepare_chol_band(h->Pp, bw) != 0)
             return h->fail(SPGO_E_DEVICE, "cannot reserve LDS for the envelope Cholesky");
+        if (h->path == SPGO_FACTOR_ENVELOPE_SPLIT) {
+            // separator of D block rows in the middle; the bottom part's profile in
+            // reversed order: local row r is block row nb - 1 - r, its first column
+            // nb - 1 - last(nb - 1 - r), last(i) the last block row j with fb[j] <= i
+            const int nbk = h->Pp / chol_band_block(), nS = std::max(bw, 1);
+            h->s0 = (nbk - nS + 1) / 2;
+            h->s1 = h->s0 + nS;
+            std::vector<int> last(nbk);
+            for (int i = 0; i < nbk; ++i) {
+                int j = i;
+                while (j + 1 < nbk && fb[j + 1] <= i) ++j;
+                last[i] = j;
+            }
+            std::vector<int> fB(nbk - h->s0);
+            for (int r = 0; r < nbk - h->s0; ++r) fB[r] = nbk - 1 - last[nbk - 1 - r];
+            if ((rc = upload(h, fB, &h->firstB))) return rc;
+            const int EBk = chol_band_block();
+            h->Ssep = h->mem.alloc<double>((size_t)2 * nS * nS * EBk * EBk + (size_t)2 * EBk * nS);
+            h->Smrg = h->mem.alloc<double>((size_t)(EBk * nS) * (EBk * nS));
+            h->winvS = h->mem.alloc<double>((size_t)nS * EBk * EBk);
+            if (!h->Ssep || !h->Smrg || !h->winvS) return h->fail(SPGO_E_DEVICE, "device allocation failed");
+            if (prepare_chol_split(h->Pp, bw) != 0)
+                return h->fail(SPGO_E_DEVICE, "cannot reserve LDS for the split envelope Cholesky");
+        }
     }
     h->H = h->mem.alloc<double>((size_t)h->Pp * h->Pp);
     h->gvec = h->mem.alloc<double>(h->Pp);
@@ -519,12 +571,21 @@ AsmArgs make_asm(const spgo_solver* h, int init) {
     return a;
 }
 
+// the envelope factorisation and both solves (one or two kernels)
+static void launch_chol(spgo_solver* h, hipStream_t s) {
+    if (h->path == SPGO_FACTOR_ENVELOPE_SPLIT)
+        launch_chol_split(h->H, h->Pp, h->band_depth, h->s0, h->s1, h->first, h->firstB, h->gvec, h->delta, h->Linv,
+                          h->Ssep, h->Smrg, h->winvS, h->ctl, s);
+    else
+        launch_chol_band(h->H, h->Pp, h->band_depth, h->first, h->gvec, h->delta, h->Linv, h->ctl, s);
+}
+
 template <typename T>
 void launch_iteration(spgo_solver* h, StateBuf<T>* b) {
     hipStream_t s = h->stream;
     launch_form_system<T>(h->lay, b, h->ctl, h->H, h->gvec, h->Pp, s);
     if (h->band) {
-        launch_chol_band(h->H, h->Pp, h->band_depth, h->first, h->gvec, h->delta, h->Linv, h->ctl, s);
+        launch_chol(h, s);
     } else {
         launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
         launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, s);
@@ -697,7 +758,7 @@ bool valid_opts(const spgo_options* o, std::string& why) {
     if (!(o->alpha_min > 0.0) || !(o->alpha_max >= o->alpha_min)) return why = "alpha bounds", false;
     if (o->chunk_landmarks < 0 || o->chunk_landmarks > CHUNK_LM) return why = "chunk_landmarks", false;
     if (o->group_chunks < 0) return why = "group_chunks", false;
-    if (o->factorization < 0 || o->factorization > 2) return why = "factorization", false;
+    if (o->factorization < 0 || o->factorization > 3) return why = "factorization", false;
     return true;
 }
 
@@ -1055,7 +1116,7 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
         else launch_form_system<double>(h->lay, h->bd, h->ctl, h->H, h->gvec, h->Pp, s);
         (void)hipEventRecord(ev[1], s);
         if (h->band) {
-            launch_chol_band(h->H, h->Pp, h->band_depth, h->first, h->gvec, h->delta, h->Linv, h->ctl, s);
+            launch_chol(h, s);
             (void)hipEventRecord(ev[2], s);
         } else {
             launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
@@ -1107,7 +1168,7 @@ int spgo_problem_info(spgo_solver* h, spgo_layout_info* out) {
     out->n_anchors_used = hl.anchors_used;
     std::vector<int> fb;
     int depth = 0;
-    out->factorization = use_envelope(h, hl, fb, depth) ? SPGO_FACTOR_ENVELOPE : SPGO_FACTOR_DENSE;
+    out->factorization = factor_path(h, hl, fb, depth);
     out->profile_depth = depth;
     out->reduced_size = reduced_size(h->N);
     return SPGO_OK;

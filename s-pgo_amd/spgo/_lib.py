@@ -23,6 +23,7 @@ SPGO_PREC_MIXED = 1
 SPGO_FACTOR_AUTO = 0
 SPGO_FACTOR_DENSE = 1
 SPGO_FACTOR_ENVELOPE = 2
+SPGO_FACTOR_ENVELOPE_SPLIT = 3
 
 ERRORS = {-1: "invalid argument", -2: "capacity exceeded", -3: "unsupported layout",
           -4: "device error", -5: "invalid state"}

@@ -47,9 +47,11 @@ def _check_run(p_or_pa, s, iters, oo, X0, a0, tag):
     return X, al, recs
 
 
-@pytest.mark.parametrize("mode,factor", [("lm", 0), ("reference", 0), ("lm", 1), ("reference", 1)])
+@pytest.mark.parametrize("mode,factor", [("lm", 0), ("reference", 0), ("lm", 1), ("reference", 1),
+                                         ("lm", 3), ("reference", 3)])
 def test_c1_parity(mode, factor):
-    # factor 0: automatic (envelope Cholesky here), 1: dense blocked Cholesky
+    # factor 0: automatic (the one-workgroup envelope Cholesky here), 1: dense
+    # blocked Cholesky, 3: envelope Cholesky split at a separator
     p = make_config("c1_small")
     pa = O.from_problem(p)
     go, oo = _opts_pair(mode=mode, factorization=factor)
@@ -84,17 +86,33 @@ def test_medium_parity_huber_and_options(factor):
     _check_run(pa, s, 8, oo, p.X0.astype(np.float64), p.alpha0.astype(np.float64), "medium-huber")
 
 
-@pytest.mark.parametrize("n_poses,n_lm,obs,seed", [(20, 1500, 12, 41), (26, 2000, 16, 42)])
-def test_deep_profile_envelope(n_poses, n_lm, obs, seed):
+@pytest.mark.parametrize("n_poses,n_lm,obs,seed,factor", [(20, 1500, 12, 41, 2), (26, 2000, 16, 42, 2),
+                                                         (20, 1500, 12, 41, 3), (26, 2000, 16, 42, 3)])
+def test_deep_profile_envelope(n_poses, n_lm, obs, seed, factor):
     # landmarks seen over 12 and 16 frames: the reduced system's profile is 4 to
-    # 7 blocks of 16 deep, so the envelope factorisation (forced here) runs the
-    # panel on two and three wavefronts
+    # 6 blocks of 16 deep, so the envelope factorisation (forced here, in one
+    # workgroup or split at a separator) runs the panel on two wavefronts
     p = make_problem(n_poses, n_lm, obs, seed=seed)
     pa = O.from_problem(p)
-    go, oo = _opts_pair(factorization=2)
+    go, oo = _opts_pair(factorization=factor)
     s = solver_from_problem(p, go)
     print(f"[deep-profile] poses={n_poses} span={s.layout_info()['max_span']}", flush=True)
-    _check_run(pa, s, 5, oo, p.X0.astype(np.float64), p.alpha0.astype(np.float64), f"deep-{obs}")
+    assert s.layout_info()["factorization"] == factor
+    _check_run(pa, s, 5, oo, p.X0.astype(np.float64), p.alpha0.astype(np.float64), f"deep-{obs}-{factor}")
+
+
+def test_long_trajectory_split_factorisation():
+    # 48 keyframes: 18 blocks of 16 rows with a profile 3 blocks deep, so the
+    # automatic choice is the split envelope factorisation; the one-workgroup
+    # envelope and the oracle must agree with it
+    p = make_problem(48, 3000, 8, seed=43)
+    pa = O.from_problem(p)
+    X0, a0 = p.X0.astype(np.float64), p.alpha0.astype(np.float64)
+    for factor in (0, 2):
+        go, oo = _opts_pair(factorization=factor)
+        s = solver_from_problem(p, go)
+        assert s.layout_info()["factorization"] == (3 if factor == 0 else 2)
+        _check_run(pa, s, 4, oo, X0, a0, f"long-{factor}")
 
 
 def test_reference_mode_step_size_and_split_steps():
