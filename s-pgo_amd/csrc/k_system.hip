@@ -364,8 +364,16 @@ struct EnvCtx {
     int steps;      // block columns factored (the rest stay in the window)
     int D;          // profile depth of the local system
     int zero_from;  // blocks (i, j) with i, j >= zero_from load as zero
-    int wrev, wn;   // inverse diagonal block of column k goes to slot k, or wn - 1 - k
+    int wrev, wn;   // column k's panel and inverse diagonal block go to slot k, or wn - 1 - k
+    int preloaded;  // the caller has placed every block of the (small) system in the window
 };
+
+// Panel store: for slot s (one per factored block column), the EDMAX blocks
+// below the diagonal in the local system's orientation, row-major 16 x 16,
+// so the copy-out and the backward solve read and write contiguous rows.
+__device__ __forceinline__ double* env_panel(double* Lp, int slot, int i) {
+    return Lp + ((long long)slot * EDMAX + (i - 1)) * EB * EB;
+}
 
 __device__ __forceinline__ int env_wslot(const EnvCtx& c, int k) { return c.wrev ? c.wn - 1 - k : k; }
 
@@ -386,7 +394,7 @@ __device__ __forceinline__ void env_counts(const int* fst, int nloc, int D, int*
 // and L below the diagonal and the inverse diagonal blocks are stored to
 // global memory (not yet waited for).
 template <int DT>
-__device__ void env_forward(const EnvSrc& src, const EnvCtx& c, double* winv, int* sbad) {
+__device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, double* Lp, int* sbad) {
     constexpr int EPRE = 4 * (DT + 1);      // prefetched doubles per lane of one block row
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int D = c.D, W = D + 3, nloc = c.nloc;
@@ -401,7 +409,7 @@ __device__ void env_forward(const EnvSrc& src, const EnvCtx& c, double* winv, in
 
     env_counts(fst, nloc, D, mk);
     // rows 0 .. D of the window; every thread owns one element of each block
-    {
+    if (!c.preloaded) {
         const int r = tid >> 4, cc = tid & 15;
         double v[(DT + 1) * (DT + 2) / 2];
         int n = 0;
@@ -441,23 +449,26 @@ __device__ void env_forward(const EnvSrc& src, const EnvCtx& c, double* winv, in
     // panel kp to global memory for the backward solve: the inverse of its
     // diagonal factor and the blocks below it (one wavefront)
     auto copy_out = [&](int kp, int rkp) {
-        env_invert(ring + (kp & 1) * EBLK, dv + EB * kp, winv + (long long)env_wslot(c, kp) * EB * EB, lane);
+        const int slot = env_wslot(c, kp);
+        env_invert(ring + (kp & 1) * EBLK, dv + EB * kp, winv + (long long)slot * EB * EB, lane);
         const int mp = mk[kp];
         for (int i = 1; i <= mp; ++i) {
             const double* s = env_slot(win, wrap(rkp + i), rkp);
+            double* d = env_panel(Lp, slot, i);
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int e = lane + 64 * t, rr = e >> 4, cc = e & 15;
-                src.H[env_gaddr(src, kp + i, kp, rr, cc)] = s[rr * EST + cc];
+                d[e] = s[rr * EST + cc];
             }
         }
     };
     // rest of column kp's trailing update: tiles (kp+i, kp+j), 2 <= j <= i <= mp,
-    // on wavefronts w0 .. 3
-    auto rest_tiles = [&](int kp, int rkp, int w0) {
+    // on wavefronts w0 .. w0 + nw - 1
+    auto rest_tiles = [&](int kp, int rkp, int w0, int nw) {
         const int mp = mk[kp];
         const int nt = (mp - 1) * mp / 2;
-        for (int t = w - w0; t < nt; t += 4 - w0) {
+        if (w < w0 || w >= w0 + nw) return;
+        for (int t = w - w0; t < nt; t += nw) {
             int i = 2;
             while ((i - 1) * i / 2 <= t) ++i;
             const int j = 2 + t - (i - 2) * (i - 1) / 2;
@@ -529,8 +540,12 @@ __device__ void env_forward(const EnvSrc& src, const EnvCtx& c, double* winv, in
             }
             if (bad && lane == 0) *sbad = 1;
         } else {
-            // wavefront 3: the prefetched block row k + D + 1 into its slot (row
-            // k - 2's, free since the last phase A)
+            // Helper wavefronts nf .. 3.  Wavefront 3 moves the prefetched block
+            // row k + D + 1 into its slot (row k - 2's, free since the last phase
+            // A) and prefetches row k + D + 2; the first helper inverts and copies
+            // out panel k - 1 (its stores first, so they complete while it works
+            // on); the other helpers (all of them if there is only one) apply the
+            // rest of panel k - 1's trailing update.
             const int inew = k + D + 1;
             if (w == 3 && inew < nloc) {
                 const int f0 = fst[inew], nbl = inew - f0 + 1, ri = wrap(rk + D + 1);
@@ -541,15 +556,15 @@ __device__ void env_forward(const EnvSrc& src, const EnvCtx& c, double* winv, in
                         env_slot(win, ri, wrap(rk + f0 + bb - k))[rr * EST + cc] =
                             zero_blk(inew, f0 + bb) ? 0.0 : pre[t];
                 }
+                if (inew + 1 < nloc) prefetch(inew + 1);
             }
             if (k >= 1) {
                 const int rkp = rk == 0 ? W - 1 : rk - 1;
-                rest_tiles(k - 1, rkp, nf);
-                if (w == 3) copy_out(k - 1, rkp);
+                const int nh = 4 - nf;                       // helper wavefronts
+                if (w == nf) copy_out(k - 1, rkp);
+                if (nh >= 2) rest_tiles(k - 1, rkp, nf + 1, nh - 1);
+                else rest_tiles(k - 1, rkp, nf, 1);
             }
-            // wavefront 3: prefetch block row k + D + 2 (global stores and
-            // loads are issued last, so no wavefront waits on them in this phase)
-            if (w == 3 && inew + 1 < nloc) prefetch(inew + 1);
         }
         lds_barrier();
         // ---------------- phase B: tiles (k + i, k + 1), 1 <= i <= m
@@ -559,7 +574,7 @@ __device__ void env_forward(const EnvSrc& src, const EnvCtx& c, double* winv, in
     if (c.steps > 0) {
         // the last factored column: its remaining trailing update and its copy
         const int kp = c.steps - 1, rkp = rk == 0 ? W - 1 : rk - 1;
-        rest_tiles(kp, rkp, 0);
+        rest_tiles(kp, rkp, 0, 4);
         lds_barrier();
         if (w == 3) copy_out(kp, rkp);
     }
@@ -572,21 +587,25 @@ __device__ void env_forward(const EnvSrc& src, const EnvCtx& c, double* winv, in
 // g+8, g+12 of column c over the blocks below the diagonal; z = y_k - that
 // sum goes through LDS so each lane can form its share of x_k = L_kk^-T z
 // from its four entries of column c of L_kk^-1.  The next block's loads are
-// issued before the current block is solved.
+// issued PD blocks ahead (the panels come from other compute units' caches).
 template <int DT>
-__device__ void env_backward(const EnvSrc& src, const EnvCtx& c, const double* winv, int lane) {
+__device__ void env_backward(const EnvCtx c, const double* winv, const double* Lp, int lane) {
     const int cl = lane & 15, g4 = lane >> 4;
     const int* mk = c.mk;
     double* xs = c.xs;
-    double pl[2][4 * DT], wl[2][4];
+    constexpr int PD = DT <= 3 ? 3 : 2;    // blocks whose loads are in flight
+    double pl[PD][4 * DT], wl[PD][4];
     auto load = [&](int k, double* P, double* Wc) {
         const int kc = max(k, 0);
         const int m = mk[kc];
+        const int slot = env_wslot(c, kc);
 #pragma unroll
-        for (int i = 1; i <= DT; ++i)
+        for (int i = 1; i <= DT; ++i) {
+            const double* Pk = env_panel((double*)Lp, slot, max(min(i, m), 1));
 #pragma unroll
-            for (int t = 0; t < 4; ++t) P[4 * (i - 1) + t] = src.H[env_gaddr(src, kc + min(i, m), kc, g4 + 4 * t, cl)];
-        const double* Wk = winv + (long long)env_wslot(c, kc) * EB * EB;
+            for (int t = 0; t < 4; ++t) P[4 * (i - 1) + t] = Pk[(g4 + 4 * t) * EB + cl];
+        }
+        const double* Wk = winv + (long long)slot * EB * EB;
 #pragma unroll
         for (int t = 0; t < 4; ++t) Wc[t] = Wk[(g4 + 4 * t) * EB + cl];
     };
@@ -616,15 +635,21 @@ __device__ void env_backward(const EnvSrc& src, const EnvCtx& c, const double* w
         wave_lds_order();
     };
     if (c.steps <= 0) return;
-    load(c.steps - 1, pl[0], wl[0]);
+    // buffer q holds block k - q; after block k - q is solved it is refilled
+    // with block k - q - PD
+#pragma unroll
+    for (int q = 0; q < PD; ++q) load(c.steps - 1 - q, pl[q], wl[q]);
     int k = c.steps - 1;
-    for (; k >= 1; k -= 2) {
-        load(k - 1, pl[1], wl[1]);
-        solve(k, pl[0], wl[0]);
-        load(k - 2, pl[0], wl[0]);
-        solve(k - 1, pl[1], wl[1]);
+    for (; k >= PD - 1; k -= PD) {
+#pragma unroll
+        for (int q = 0; q < PD; ++q) {
+            solve(k - q, pl[q], wl[q]);
+            load(k - q - PD, pl[q], wl[q]);
+        }
     }
-    if (k == 0) solve(0, pl[0], wl[0]);
+#pragma unroll
+    for (int q = 0; q < PD; ++q)
+        if (k - q >= 0) solve(k - q, pl[q], wl[q]);
 }
 
 // LDS carve-up of one context (window for depth D, nloc block rows)
@@ -643,6 +668,7 @@ __device__ __forceinline__ EnvCtx env_ctx(double* base, int D, int nloc, double*
     c.zero_from = nloc;
     c.wrev = 0;
     c.wn = nloc;
+    c.preloaded = 0;
     return c;
 }
 
@@ -654,7 +680,7 @@ __host__ __device__ inline size_t env_ctx_bytes(int D, int nloc) {
 // ---- one workgroup: the whole system ---------------------------------------
 template <int DT>
 __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, const int* first, const double* g,
-                                                  double* x, double* winv, Control* ctl) {
+                                                  double* x, double* winv, double* Lp, Control* ctl) {
     extern __shared__ double lds[];
     __shared__ int sbad;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -666,9 +692,9 @@ __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, cons
     if (tid == 0) sbad = 0;
     __syncthreads();
     const EnvSrc src{H, Pp, nb, 0};
-    env_forward<DT>(src, c, winv, &sbad);
+    env_forward<DT>(src, c, winv, Lp, &sbad);
     __syncthreads();   // the factor and the inverses in global memory are complete
-    if (w == 0) env_backward<DT>(src, c, winv, lane);
+    if (w == 0) env_backward<DT>(c, winv, Lp, lane);
     __syncthreads();
     for (int i = tid; i < Pp; i += 256) x[i] = c.xs[i];
     if (tid == 0 && sbad) ctl->chol_ok = 0;
@@ -682,7 +708,7 @@ __global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, cons
 template <int DT>
 __global__ __launch_bounds__(256) void k_chol_split1(double* H, int Pp, int D, int s0, int s1, const int* firstT,
                                                      const int* firstB, const double* g, double* x, double* winv,
-                                                     double* S, Control* ctl) {
+                                                     double* Lp, double* S, Control* ctl) {
     extern __shared__ double lds[];
     __shared__ int sbad;
     const int tid = threadIdx.x;
@@ -704,7 +730,7 @@ __global__ __launch_bounds__(256) void k_chol_split1(double* H, int Pp, int D, i
     if (tid == 0) sbad = 0;
     __syncthreads();
     const EnvSrc src{H, Pp, nb, half};
-    env_forward<DT>(src, c, winv, &sbad);
+    env_forward<DT>(src, c, winv, Lp, &sbad);
     // forward solution of the factored rows, and the separator's blocks and right-hand side
     for (int i = tid; i < EB * c.steps; i += 256) x[half ? Pp - 1 - i : i] = c.xs[i];
     const int W = D + 3;
@@ -723,9 +749,10 @@ __global__ __launch_bounds__(256) void k_chol_split1(double* H, int Pp, int D, i
 // ---- split factorisation, kernel 2: the separator (sum of both updates,
 // factored and solved), then the backward solves of both parts ---------------
 template <int DT>
-__global__ __launch_bounds__(256) void k_chol_split2(double* H, int Pp, int D, int s0, int s1, const int* firstT,
-                                                     const int* firstB, double* x, double* winv, const double* S,
-                                                     double* Sm, double* winvS, Control* ctl) {
+__global__ __launch_bounds__(256) void k_chol_split2(int Pp, int D, int s0, int s1, const int* firstT,
+                                                     const int* firstB, double* x, const double* winv,
+                                                     const double* Lp, const double* S, double* Sm, double* winvS,
+                                                     double* LpS, Control* ctl) {
     extern __shared__ double lds[];
     __shared__ int sbad;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -739,31 +766,49 @@ __global__ __launch_bounds__(256) void k_chol_split2(double* H, int Pp, int D, i
     cb.steps = nb - s1;
     cb.wrev = 1;
     cb.wn = nb;
-    // separator = top update + bottom update (reversed and transposed), lower blocks
+    // separator = top update + bottom update (reversed and transposed): its lower
+    // blocks go straight into the window of its factorisation (all nS block rows
+    // fit: the window has nS + 2)
     const double* St = S;
     const double* Sb = S + (long long)nS * nS * EB * EB;
-    for (int e = tid; e < PS * PS; e += 256) {
-        const int row = e / PS, col = e % PS;
-        if (col / EB <= row / EB) {
-            const int I = row / EB, J = col / EB, r = row % EB, cc = col % EB;
-            const int Ib = nS - 1 - J, Jb = nS - 1 - I;   // the bottom system's block (Ib >= Jb)
-            Sm[e] = St[((long long)(I * nS + J) * EB + r) * EB + cc] +
-                    Sb[((long long)(Ib * nS + Jb) * EB + (EB - 1 - cc)) * EB + (EB - 1 - r)];
+    cs.preloaded = 1;
+    for (int e0 = 0; e0 < PS * PS; e0 += 16 * 256) {
+        // sixteen elements per thread with all loads in flight before the stores
+        double va[16], vb[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int e = e0 + q * 256 + tid;
+            const int row = e / PS, col = e % PS;
+            va[q] = vb[q] = 0.0;
+            if (e < PS * PS && col / EB <= row / EB) {
+                const int I = row / EB, J = col / EB, r = row % EB, cc = col % EB;
+                const int Ib = nS - 1 - J, Jb = nS - 1 - I;   // the bottom system's block (Ib >= Jb)
+                va[q] = St[((long long)(I * nS + J) * EB + r) * EB + cc];
+                vb[q] = Sb[((long long)(Ib * nS + Jb) * EB + (EB - 1 - cc)) * EB + (EB - 1 - r)];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int e = e0 + q * 256 + tid;
+            const int row = e / PS, col = e % PS;
+            if (e < PS * PS && col / EB <= row / EB)
+                env_slot(cs.win, row / EB, col / EB)[(row % EB) * EST + col % EB] = va[q] + vb[q];
         }
     }
+    (void)Sm;
     const double* Sr = S + (long long)2 * nS * nS * EB * EB;
     for (int t = tid; t < PS; t += 256) cs.xs[t] = Sr[t] + Sr[PS + PS - 1 - t];
     for (int i = tid; i < nS; i += 256) cs.fst[i] = 0;
     for (int i = tid; i < nT; i += 256) ct.fst[i] = firstT[i];
     for (int i = tid; i < nB; i += 256) cb.fst[i] = firstB[i];
     if (tid == 0) sbad = 0;
-    __syncthreads();   // also makes Sm visible to the workgroup
-    const EnvSrc ss{Sm, PS, nS, 0};
-    env_forward<DT>(ss, cs, winvS, &sbad);
+    __syncthreads();
+    const EnvSrc ss{Sm, PS, nS, 0};   // not read: the window is preloaded
+    env_forward<DT>(ss, cs, winvS, LpS, &sbad);
     env_counts(ct.fst, nT, D, ct.mk);
     env_counts(cb.fst, nB, D, cb.mk);
     __syncthreads();
-    if (w == 0) env_backward<DT>(ss, cs, winvS, lane);
+    if (w == 0) env_backward<DT>(cs, winvS, LpS, lane);
     __syncthreads();
     // the two parts: forward solutions of their rows, the separator's solution after them
     for (int i = tid; i < EB * s0; i += 256) ct.xs[i] = x[i];
@@ -773,8 +818,8 @@ __global__ __launch_bounds__(256) void k_chol_split2(double* H, int Pp, int D, i
         cb.xs[EB * (nb - s1) + t] = cs.xs[PS - 1 - t];
     }
     __syncthreads();
-    if (w == 0) env_backward<DT>(EnvSrc{H, Pp, nb, 0}, ct, winv, lane);
-    if (w == 1) env_backward<DT>(EnvSrc{H, Pp, nb, 1}, cb, winv, lane);
+    if (w == 0) env_backward<DT>(ct, winv, Lp, lane);
+    if (w == 1) env_backward<DT>(cb, winv, Lp, lane);
     __syncthreads();
     for (int i = tid; i < EB * s0; i += 256) x[i] = ct.xs[i];
     for (int t = tid; t < PS; t += 256) x[EB * s0 + t] = cs.xs[t];
@@ -791,28 +836,28 @@ size_t chol_split_lds(int Pp, int D, int part) {
 }
 
 void launch_chol_band(double* H, int Pp, int D, const int* first, const double* g, double* x, double* winv,
-                      Control* ctl, hipStream_t s) {
+                      double* Lp, Control* ctl, hipStream_t s) {
     if (D <= 3)
         hipLaunchKernelGGL(k_chol_env<3>, dim3(1), dim3(256), chol_band_lds(Pp, D), s, H, Pp, D, first, g, x, winv,
-                           ctl);
+                           Lp, ctl);
     else
         hipLaunchKernelGGL(k_chol_env<EDMAX>, dim3(1), dim3(256), chol_band_lds(Pp, D), s, H, Pp, D, first, g, x,
-                           winv, ctl);
+                           winv, Lp, ctl);
 }
 
 void launch_chol_split(double* H, int Pp, int D, int s0, int s1, const int* firstT, const int* firstB,
-                       const double* g, double* x, double* winv, double* S, double* Sm, double* winvS, Control* ctl,
-                       hipStream_t s) {
+                       const double* g, double* x, double* winv, double* Lp, double* S, double* Sm, double* winvS,
+                       double* LpS, Control* ctl, hipStream_t s) {
     if (D <= 3) {
         hipLaunchKernelGGL(k_chol_split1<3>, dim3(2), dim3(256), chol_split_lds(Pp, D, 1), s, H, Pp, D, s0, s1,
-                           firstT, firstB, g, x, winv, S, ctl);
-        hipLaunchKernelGGL(k_chol_split2<3>, dim3(1), dim3(256), chol_split_lds(Pp, D, 2), s, H, Pp, D, s0, s1,
-                           firstT, firstB, x, winv, S, Sm, winvS, ctl);
+                           firstT, firstB, g, x, winv, Lp, S, ctl);
+        hipLaunchKernelGGL(k_chol_split2<3>, dim3(1), dim3(256), chol_split_lds(Pp, D, 2), s, Pp, D, s0, s1,
+                           firstT, firstB, x, winv, Lp, S, Sm, winvS, LpS, ctl);
     } else {
         hipLaunchKernelGGL(k_chol_split1<EDMAX>, dim3(2), dim3(256), chol_split_lds(Pp, D, 1), s, H, Pp, D, s0, s1,
-                           firstT, firstB, g, x, winv, S, ctl);
-        hipLaunchKernelGGL(k_chol_split2<EDMAX>, dim3(1), dim3(256), chol_split_lds(Pp, D, 2), s, H, Pp, D, s0, s1,
-                           firstT, firstB, x, winv, S, Sm, winvS, ctl);
+                           firstT, firstB, g, x, winv, Lp, S, ctl);
+        hipLaunchKernelGGL(k_chol_split2<EDMAX>, dim3(1), dim3(256), chol_split_lds(Pp, D, 2), s, Pp, D, s0, s1,
+                           firstT, firstB, x, winv, Lp, S, Sm, winvS, LpS, ctl);
     }
 }
 
@@ -831,6 +876,7 @@ int prepare_chol_split(int Pp, int D) {
 }
 
 int chol_band_max_blocks() { return EDMAX; }
+size_t chol_panel_doubles(int Pp) { return (size_t)(Pp / EB) * EDMAX * EB * EB; }
 int chol_band_block() { return EB; }
 
 // ---------------------------------------------------------------------------

@@ -25,13 +25,14 @@ void launch_form_system(const Layout& lay, const StateBuf<T>* bufs, Control* ctl
 
 void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t s);
 void launch_chol_band(double* H, int Pp, int D, const int* first, const double* g, double* x, double* winv,
-                      Control* ctl, hipStream_t s);
+                      double* Lp, Control* ctl, hipStream_t s);
 void launch_chol_split(double* H, int Pp, int D, int s0, int s1, const int* firstT, const int* firstB,
-                       const double* g, double* x, double* winv, double* S, double* Sm, double* winvS, Control* ctl,
-                       hipStream_t s);
+                       const double* g, double* x, double* winv, double* Lp, double* S, double* Sm, double* winvS,
+                       double* LpS, Control* ctl, hipStream_t s);
 int prepare_chol_split(int Pp, int D);
 size_t chol_split_lds(int Pp, int D, int part);
 int chol_band_max_blocks();
+size_t chol_panel_doubles(int Pp);
 int prepare_chol_band(int Pp, int D);
 int chol_band_block();
 size_t chol_band_lds(int Pp, int D);

@@ -116,7 +116,8 @@ struct spgo_solver {
     int path = SPGO_FACTOR_DENSE;   // factorisation taken: SPGO_FACTOR_DENSE, _ENVELOPE or _ENVELOPE_SPLIT
     int s0 = 0, s1 = 0;         // split: separator block rows [s0, s1)
     const int* firstB = nullptr;   // split: profile of the reversed bottom part
-    double *Ssep = nullptr, *Smrg = nullptr, *winvS = nullptr;   // split: separator scratch
+    double *Ssep = nullptr, *Smrg = nullptr, *winvS = nullptr, *LpS = nullptr;   // split: separator scratch
+    double* Lpan = nullptr;     // envelope: factored panels for the backward solve
     Control* ctl = nullptr;
     IterRecord* rec = nullptr;
     int rec_cap = 0;
@@ -489,6 +490,8 @@ int setup_layout(spgo_solver* h) {
         if (h->path < 0)
             return h->fail(SPGO_E_LAYOUT, "the reduced-system profile does not fit the requested factorisation");
         h->band = h->path != SPGO_FACTOR_DENSE;
+        if (h->band && !(h->Lpan = h->mem.alloc<double>(chol_panel_doubles(h->Pp))))
+            return h->fail(SPGO_E_DEVICE, "device allocation failed");
         if ((rc = upload(h, fb, &h->first))) return rc;
         h->band_depth = bw;
         if (h->path == SPGO_FACTOR_ENVELOPE && prepare_chol_band(h->Pp, bw) != 0)
@@ -513,7 +516,8 @@ int setup_layout(spgo_solver* h) {
             h->Ssep = h->mem.alloc<double>((size_t)2 * nS * nS * EBk * EBk + (size_t)2 * EBk * nS);
             h->Smrg = h->mem.alloc<double>((size_t)(EBk * nS) * (EBk * nS));
             h->winvS = h->mem.alloc<double>((size_t)nS * EBk * EBk);
-            if (!h->Ssep || !h->Smrg || !h->winvS) return h->fail(SPGO_E_DEVICE, "device allocation failed");
+            h->LpS = h->mem.alloc<double>(chol_panel_doubles(EBk * nS));
+            if (!h->Ssep || !h->Smrg || !h->winvS || !h->LpS) return h->fail(SPGO_E_DEVICE, "device allocation failed");
             if (prepare_chol_split(h->Pp, bw) != 0)
                 return h->fail(SPGO_E_DEVICE, "cannot reserve LDS for the split envelope Cholesky");
         }
@@ -575,9 +579,9 @@ AsmArgs make_asm(const spgo_solver* h, int init) {
 static void launch_chol(spgo_solver* h, hipStream_t s) {
     if (h->path == SPGO_FACTOR_ENVELOPE_SPLIT)
         launch_chol_split(h->H, h->Pp, h->band_depth, h->s0, h->s1, h->first, h->firstB, h->gvec, h->delta, h->Linv,
-                          h->Ssep, h->Smrg, h->winvS, h->ctl, s);
+                          h->Lpan, h->Ssep, h->Smrg, h->winvS, h->LpS, h->ctl, s);
     else
-        launch_chol_band(h->H, h->Pp, h->band_depth, h->first, h->gvec, h->delta, h->Linv, h->ctl, s);
+        launch_chol_band(h->H, h->Pp, h->band_depth, h->first, h->gvec, h->delta, h->Linv, h->Lpan, h->ctl, s);
 }
 
 template <typename T>
