@@ -132,6 +132,15 @@ int spgo_write_incremental_pose(spgo_solver* h, int32_t k, const float T[16]);
 /* Solver::loadInverseDepths / getInverseDepths (Solver.cpp:112-128), N*M values. */
 int spgo_load_inverse_depths(spgo_solver* h, const float* alpha);
 int spgo_get_inverse_depths(spgo_solver* h, float* alpha);
+/* Optimizer.initalize_depth_with_disparity (PythonUtils/Optimizer.py:87-114):
+ * every landmark whose anchor-frame cell holds both pixels gets the inverse of
+ * the least-squares stereo depth along its left ray,
+ *   d = ((v x b_l) . (v x t)) / |v x b_l|^2,  v = R b_r,
+ * with b_c = K_c^-1 p_c and (R, t) = T_r_to_l.  Landmarks without that cell or
+ * with a non-positive depth keep their inverse depth.  n_set (optional)
+ * receives the number of landmarks set. */
+int spgo_init_inverse_depths_from_disparity(spgo_solver* h, int32_t* n_set);
+
 /* CudaSolver.setStepSize (Solver.pyx:106-107): scales the pose update only. */
 int spgo_set_step_size(spgo_solver* h, float step_size);
 

@@ -469,3 +469,21 @@ def pose_errors(X_true: np.ndarray, X_est: np.ndarray) -> np.ndarray:
     rot = np.linalg.norm(log_so3(E[:, :3, :3]), axis=-1)
     tr = np.linalg.norm(E[:, :3, 3], axis=-1)
     return np.stack([rot, tr], -1)
+
+
+def depth_from_disparity(intrinsics, T_rl, pl, pr):
+    """Least-squares stereo depth along the left ray (PythonUtils/Optimizer.py:87-114):
+    b_c = K_c^-1 p_c (homogeneous pixels, n x 3), v = R_rl b_r, N = [v]x,
+    depth = ((N b_l) . (N t_rl)) / |N b_l|^2.  Returns depths (n,)."""
+    intr = np.asarray(intrinsics, dtype=np.float64)
+    T = np.asarray(T_rl, dtype=np.float64).reshape(4, 4)
+    pl = np.asarray(pl, dtype=np.float64)
+    pr = np.asarray(pr, dtype=np.float64)
+    Kl = np.array([[intr[0], 0, intr[2]], [0, intr[1], intr[3]], [0, 0, 1.0]])
+    Kr = np.array([[intr[4], 0, intr[6]], [0, intr[5], intr[7]], [0, 0, 1.0]])
+    bl = pl @ np.linalg.inv(Kl).T
+    br = pr @ np.linalg.inv(Kr).T
+    v = br @ T[:3, :3].T
+    nb = np.cross(v, bl)
+    nt = np.cross(v, T[:3, 3][None, :])
+    return np.sum(nb * nt, axis=1) / np.sum(nb * nb, axis=1)

@@ -989,6 +989,42 @@ int spgo_get_inverse_depths(spgo_solver* h, float* alpha) {
     return SPGO_OK;
 }
 
+int spgo_init_inverse_depths_from_disparity(spgo_solver* h, int32_t* n_set) {
+    if (!h) return SPGO_E_ARG;
+    int rc = begin_state_write(h);
+    if (rc) return rc;
+    const double fl[4] = {h->intr[0], h->intr[1], h->intr[2], h->intr[3]};
+    const double fr[4] = {h->intr[4], h->intr[5], h->intr[6], h->intr[7]};
+    const float* T = h->Trl;
+    std::vector<char> done(h->L, 0);
+    int cnt = 0;
+    for (size_t m = 0; m < h->ma.size(); ++m) {
+        const int l = h->ml[m];
+        if (done[l]) continue;
+        done[l] = 1;   // the anchor is the same for all of a landmark's measurements
+        auto it = h->cell_of.find((long long)h->ma[m] * h->L + l);
+        if (it == h->cell_of.end()) continue;
+        const float* c = &h->cells[6 * it->second];
+        // rays b = K^-1 [u, v, w] of the homogeneous pixels in both cameras
+        const double bl[3] = {(c[0] - fl[2] * c[2]) / fl[0], (c[1] - fl[3] * c[2]) / fl[1], (double)c[2]};
+        const double br[3] = {(c[3] - fr[2] * c[5]) / fr[0], (c[4] - fr[3] * c[5]) / fr[1], (double)c[5]};
+        double v[3], t[3];
+        for (int r = 0; r < 3; ++r) {
+            v[r] = T[4 * r] * br[0] + T[4 * r + 1] * br[1] + T[4 * r + 2] * br[2];
+            t[r] = T[4 * r + 3];
+        }
+        const double nb[3] = {v[1] * bl[2] - v[2] * bl[1], v[2] * bl[0] - v[0] * bl[2], v[0] * bl[1] - v[1] * bl[0]};
+        const double nt[3] = {v[1] * t[2] - v[2] * t[1], v[2] * t[0] - v[0] * t[2], v[0] * t[1] - v[1] * t[0]};
+        const double num = nb[0] * nt[0] + nb[1] * nt[1] + nb[2] * nt[2];
+        const double den = nb[0] * nb[0] + nb[1] * nb[1] + nb[2] * nb[2];
+        if (!(num > 0.0) || !(den > 0.0)) continue;
+        h->alpha[l] = den / num;   // 1 / depth
+        ++cnt;
+    }
+    if (n_set) *n_set = cnt;
+    return SPGO_OK;
+}
+
 int spgo_set_step_size(spgo_solver* h, float s) {
     if (!h) return SPGO_E_ARG;
     if (s != h->step_size) destroy_graph(h);

@@ -72,6 +72,7 @@ SIGNATURES = {
     "spgo_write_incremental_pose": (c_int32, [c_void_p, c_int32, F32P]),
     "spgo_load_inverse_depths": (c_int32, [c_void_p, F32P]),
     "spgo_get_inverse_depths": (c_int32, [c_void_p, F32P]),
+    "spgo_init_inverse_depths_from_disparity": (c_int32, [c_void_p, I32P]),
     "spgo_set_step_size": (c_int32, [c_void_p, c_float]),
     "spgo_get_state_f64": (c_int32, [c_void_p, F64P, F64P]),
     "spgo_set_state_f64": (c_int32, [c_void_p, F64P, F64P]),

@@ -142,6 +142,14 @@ class Solver:
         self._check(self._lib.spgo_get_inverse_depths(self._h, _p32(a)))
         return a
 
+    def init_inverse_depths_from_disparity(self) -> int:
+        """Inverse depths from the anchor frame's stereo pair (least-squares
+        triangulation along the left ray, Optimizer.py:87-114); returns the
+        number of landmarks set."""
+        n = ctypes.c_int32(0)
+        self._check(self._lib.spgo_init_inverse_depths_from_disparity(self._h, ctypes.byref(n)))
+        return int(n.value)
+
     def set_step_size(self, s: float) -> None:
         self._check(self._lib.spgo_set_step_size(self._h, float(s)))
 

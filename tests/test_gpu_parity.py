@@ -274,3 +274,20 @@ def test_c2_full_size_properties_and_first_iterations():
     err0 = O.pose_errors(p.X_true, X0)
     print(f"[c2] cost {c0:.6g} -> {costs[-1]:.6g}; pose err {err0.mean(0)} -> {err.mean(0)}", flush=True)
     assert err[:, 0].mean() < 0.1 * err0[:, 0].mean() and err[:, 1].mean() < 0.1 * err0[:, 1].mean()
+
+
+def test_mixed_precision_reaches_fp64_final_cost():
+    # FP32 linearisation and landmark Schur, FP64 reduced system and Cholesky:
+    # after the same number of LM iterations the cost agrees with the FP64 path
+    # to 1e-6 relative (the FP32 cost's rounding makes accept/reject decisions
+    # near convergence differ, so the iterates are not compared)
+    p = make_problem(40, 4000, 10, seed=77, outlier_frac=0.05)
+    finals = {}
+    for prec in ("fp64", "mixed"):
+        s = solver_from_problem(p, make_options(mode="lm", robust="huber", robust_param=2.0, precision=prec))
+        s.step(30)
+        c0, recs = s.stats()
+        finals[prec] = recs[-1]["cost"]
+        print(f"[mixed] {prec}: c0={c0:.10g} final={finals[prec]:.12g}", flush=True)
+        assert recs[-1]["cost"] < 0.1 * c0
+    assert abs(finals["mixed"] - finals["fp64"]) <= 1e-6 * finals["fp64"]
