@@ -141,13 +141,13 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
     const int lane = tid & 63;
     const int wv = tid >> 6;
 
-    const int cur = A.ctl->cur;
+    const GroupDesc& gd = lay.grp_desc[g];      // one record: no chain of dependent loads
+    const Control* ctl = A.ctl + gd.prob;       // the group's problem (a batch holds several)
+    const int cur = ctl->cur;
     const StateBuf<T>& bin = cur ? B1 : B0;                    // linearisation point
     const StateBuf<T>& bout = A.init ? bin : (cur ? B0 : B1);  // state being linearised
     const bool update = !A.init;
-    const bool chol_ok = update ? (A.ctl->chol_ok != 0) : true;
-
-    const GroupDesc& gd = lay.grp_desc[g];      // one record: no chain of dependent loads
+    const bool chol_ok = update ? (ctl->chol_ok != 0) : true;
     const int a = gd.anchor;
     const int K = gd.K;
     const int K6 = 6 * K;
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
         pose_load(bin.G + 12 * (a + k), Gi);
         pose_mul(Gai, Gi, E);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) x[q] = A.delta[6 * (a + k) + q];
+        for (int q = 0; q < 6; ++q) x[q] = A.delta[gd.delta_off + 6 * (gd.a_loc + k) + q];
         adjoint_apply(E, x, out);
 #pragma unroll
         for (int q = 0; q < 6; ++q) dl[k][q] = (T)out[q];

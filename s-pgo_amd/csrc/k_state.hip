@@ -15,11 +15,15 @@ namespace spgo {
 
 template <typename T>
 __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, StateBuf<T> B0, StateBuf<T> B1,
-                                                    const Control* ctl, const double* delta, double step,
+                                                    const Control* ctls, const double* delta, double step,
                                                     int init) {
+    // one workgroup per problem: its N poses start at pose_base, its frames at frame_base
     __shared__ double scan[2][WG][12];
     const int tid = threadIdx.x;
-    const int N = lay.N;
+    const ProbDesc pd = lay.probs[blockIdx.x];
+    const Control* ctl = ctls + blockIdx.x;
+    const int N = pd.N;
+    const double* dlt = delta + pd.vec_off;
     const int cur = ctl->cur;
     const StateBuf<T>& bin = cur ? B1 : B0;
     const StateBuf<T>& bout = init ? bin : (cur ? B0 : B1);
@@ -32,17 +36,17 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
     double Xkeep[12];   // with one pose per thread it stays in registers for the second pass
     for (int k = k0; k < k1; ++k) {
         double X[12], Xi[12], tmp[12];
-        pose_load(bin.X + 12 * k, X);
+        pose_load(bin.X + 12 * (pd.pose_base + k), X);
         if (!init) {
             double xi[6], E[12];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) xi[q] = ok ? step * delta[6 * k + q] : 0.0;
+            for (int q = 0; q < 6; ++q) xi[q] = ok ? step * dlt[6 * k + q] : 0.0;
             exp_se3(xi, E);
             pose_mul(X, E, tmp);
 #pragma unroll
             for (int q = 0; q < 12; ++q) {
                 X[q] = tmp[q];
-                bout.X[12 * k + q] = tmp[q];
+                bout.X[12 * (pd.pose_base + k) + q] = tmp[q];
             }
         }
 #pragma unroll
@@ -83,11 +87,12 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
         pose_inv(Gf, Gi);
         pose_mul(Gf, Trl, GR);
         pose_inv(GR, GRi);
+        const long long o = 12LL * (pd.frame_base + f);
 #pragma unroll
         for (int q = 0; q < 12; ++q) {
-            bout.G[12 * f + q] = Gf[q];
-            bout.GinvL[12 * f + q] = Gi[q];
-            bout.GinvR[12 * f + q] = GRi[q];
+            bout.G[o + q] = Gf[q];
+            bout.GinvL[o + q] = Gi[q];
+            bout.GinvR[o + q] = GRi[q];
         }
     };
     if (tid == 0) emit(0, G);
@@ -97,7 +102,7 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
 #pragma unroll
             for (int q = 0; q < 12; ++q) X[q] = Xkeep[q];
         } else {
-            pose_load(bout.X + 12 * k, X);   // this thread's own stores: program order
+            pose_load(bout.X + 12 * (pd.pose_base + k), X);   // this thread's own stores: program order
         }
         pose_inv(X, Xi);
         pose_mul(G, Xi, tmp);
@@ -110,24 +115,28 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
 template <typename T>
 void launch_pose_update(const Layout& lay, const Camera& cam, StateBuf<T>* bufs, const Control* ctl,
                         const double* delta, double step_size, int init, hipStream_t s) {
-    hipLaunchKernelGGL(k_pose_update<T>, dim3(1), dim3(WG), 0, s, lay, cam, bufs[0], bufs[1], ctl, delta,
-                       step_size, init);
+    hipLaunchKernelGGL(k_pose_update<T>, dim3(lay.n_probs), dim3(WG), 0, s, lay, cam, bufs[0], bufs[1], ctl,
+                       delta, step_size, init);
 }
 template void launch_pose_update<double>(const Layout&, const Camera&, StateBuf<double>*, const Control*,
                                          const double*, double, int, hipStream_t);
 template void launch_pose_update<float>(const Layout&, const Camera&, StateBuf<float>*, const Control*,
                                         const double*, double, int, hipStream_t);
 
-__global__ __launch_bounds__(WG) void k_accept(const double* c0, const double* c1, int n_groups, Control* ctl,
-                                              IterRecord* rec, int reference_mode, double lam_up,
+__global__ __launch_bounds__(WG) void k_accept(const double* c0, const double* c1, const ProbDesc* probs,
+                                              Control* ctls, IterRecord* recs, int reference_mode, double lam_up,
                                               double lam_down, double lam_min, double lam_max, int init) {
+    // one workgroup per problem: the costs of its groups, its control block and records
     __shared__ double red[WG];
     const int tid = threadIdx.x;
+    const ProbDesc pd = probs[blockIdx.x];
+    Control* ctl = ctls + blockIdx.x;
+    IterRecord* rec = recs + pd.rec_off;
     const int cur = ctl->cur;
     const int out = init ? cur : cur ^ 1;
     const double* cp = out ? c1 : c0;
     double s = 0.0;
-    for (int g = tid; g < n_groups; g += WG) s += cp[g];
+    for (int g = pd.g_begin + tid; g < pd.g_end; g += WG) s += cp[g];
     red[tid] = s;
     __syncthreads();
     for (int w = WG / 2; w > 0; w >>= 1) {
@@ -166,10 +175,10 @@ __global__ __launch_bounds__(WG) void k_accept(const double* c0, const double* c
     ctl->iter = ctl->iter + 1;
 }
 
-void launch_accept(const double* cost_buf0, const double* cost_buf1, int n_groups, Control* ctl, IterRecord* rec,
-                   int reference_mode, double lam_up, double lam_down, double lam_min, double lam_max, int init,
-                   hipStream_t s) {
-    hipLaunchKernelGGL(k_accept, dim3(1), dim3(WG), 0, s, cost_buf0, cost_buf1, n_groups, ctl, rec,
+void launch_accept(const double* cost_buf0, const double* cost_buf1, const Layout& lay, Control* ctl,
+                   IterRecord* rec, int reference_mode, double lam_up, double lam_down, double lam_min, double lam_max,
+                   int init, hipStream_t s) {
+    hipLaunchKernelGGL(k_accept, dim3(lay.n_probs), dim3(WG), 0, s, cost_buf0, cost_buf1, lay.probs, ctl, rec,
                        reference_mode, lam_up, lam_down, lam_min, lam_max, init);
 }
 

@@ -33,30 +33,35 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 constexpr int FS_MAXG = 512;   // groups whose span can reach one pose row
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0, StateBuf<T> B1, Control* ctl,
-                                                     double* H, double* gvec, int Pp) {
+__global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0, StateBuf<T> B1, Control* ctls,
+                                                     double* H, double* gvec) {
     // the covering groups of this row's pose block (precomputed per layout, anchors
     // ascending) with the offset of row `row` inside each group's block
     __shared__ long long rowoff[FS_MAXG];
     __shared__ int canc[FS_MAXG], gdo[FS_MAXG];
-    const int row = blockIdx.x;
+    const int row_g = blockIdx.x;                 // row of all problems' poses
     const int tid = threadIdx.x;
+    const int pg = row_g / 6;
+    const ProbDesc pd = lay.probs[lay.pose_prob[pg]];
+    Control* ctl = ctls + lay.pose_prob[pg];
+    const int row = row_g - 6 * pd.pose_base;    // row inside its problem
+    const int cbase = 6 * pd.pose_base;
     if (row == 0 && tid == 0) ctl->chol_ok = 1;   // reset for this iteration's factorisation
     const StateBuf<T>& bs = ctl->cur ? B1 : B0;
-    const int bi = row / 6;
-    const int e0 = lay.fs_begin[bi];
-    const int nc = min(lay.fs_begin[bi + 1] - e0, FS_MAXG);
+    const int e0 = lay.fs_begin[pg];
+    const int nc = min(lay.fs_begin[pg + 1] - e0, FS_MAXG);
     for (int e = tid; e < nc; e += 256) {
         const FsEntry f = lay.fs_list[e0 + e];
         canc[e] = f.anchor;
-        rowoff[e] = f.D_off + (long long)(row - 6 * f.anchor) * f.K6 - 6 * f.anchor;
-        gdo[e] = f.d_off + row - 6 * f.anchor;
+        rowoff[e] = f.D_off + (long long)(row_g - 6 * f.anchor) * f.K6 - 6 * f.anchor;
+        gdo[e] = f.d_off + row_g - 6 * f.anchor;
     }
     __syncthreads();
+    double* Hr = H + pd.H_off + (long long)row * pd.Pp;
     // every lower entry of the row is written (zeros outside the profile), so a
     // failed factorisation cannot leave values behind for the next one
     for (int col = tid; col <= row; col += 256) {
-        const int bj = col / 6;
+        const int colg = col + cbase, bj = colg / 6;
         // covering groups with anchor <= bj form a prefix (anchors ascending):
         // their blocks reach column col
         int lo = 0, hi = nc;
@@ -72,7 +77,7 @@ __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0,
         for (int eb = 0; eb < ne; eb += 16) {
             double v[16];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) v[i] = (eb + i < ne) ? (double)bs.D[rowoff[eb + i] + col] : 0.0;
+            for (int i = 0; i < 16; ++i) v[i] = (eb + i < ne) ? (double)bs.D[rowoff[eb + i] + colg] : 0.0;
 #pragma unroll
             for (int i = 0; i < 16; i += 2) {
                 h0 += v[i];
@@ -84,22 +89,21 @@ __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0,
             h += ctl->lambda;
             double gs = 0.0;
             for (int q = 0; q < nc; ++q) gs += (double)bs.d[gdo[q]];
-            gvec[row] = gs;
+            gvec[pd.vec_off + row] = gs;
         }
-        H[(long long)row * Pp + col] = h;
+        Hr[col] = h;
     }
 }
 
 template <typename T>
-void launch_form_system(const Layout& lay, const StateBuf<T>* bufs, Control* ctl, double* H, double* g,
-                        int Pp, hipStream_t s) {
-    const int P = 6 * lay.N;
-    hipLaunchKernelGGL(k_form_system<T>, dim3(P), dim3(256), 0, s, lay, bufs[0], bufs[1], ctl, H, g, Pp);
+void launch_form_system(const Layout& lay, const StateBuf<T>* bufs, Control* ctls, double* H, double* g,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_form_system<T>, dim3(6 * lay.N), dim3(256), 0, s, lay, bufs[0], bufs[1], ctls, H, g);
 }
 template void launch_form_system<double>(const Layout&, const StateBuf<double>*, Control*, double*, double*,
-                                         int, hipStream_t);
+                                         hipStream_t);
 template void launch_form_system<float>(const Layout&, const StateBuf<float>*, Control*, double*, double*,
-                                        int, hipStream_t);
+                                        hipStream_t);
 
 // ---------------------------------------------------------------------------
 // blocked Cholesky, block column k
@@ -543,9 +547,8 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
             // Helper wavefronts nf .. 3.  Wavefront 3 moves the prefetched block
             // row k + D + 1 into its slot (row k - 2's, free since the last phase
             // A) and prefetches row k + D + 2; the first helper inverts and copies
-            // out panel k - 1 (its stores first, so they complete while it works
-            // on); the other helpers (all of them if there is only one) apply the
-            // rest of panel k - 1's trailing update.
+            // out panel k - 1; then all helpers share the rest of panel k - 1's
+            // trailing update.
             const int inew = k + D + 1;
             if (w == 3 && inew < nloc) {
                 const int f0 = fst[inew], nbl = inew - f0 + 1, ri = wrap(rk + D + 1);
@@ -560,10 +563,8 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
             }
             if (k >= 1) {
                 const int rkp = rk == 0 ? W - 1 : rk - 1;
-                const int nh = 4 - nf;                       // helper wavefronts
                 if (w == nf) copy_out(k - 1, rkp);
-                if (nh >= 2) rest_tiles(k - 1, rkp, nf + 1, nh - 1);
-                else rest_tiles(k - 1, rkp, nf, 1);
+                rest_tiles(k - 1, rkp, nf, 4 - nf);   // after the copy or the loader
             }
         }
         lds_barrier();
@@ -677,13 +678,23 @@ __host__ __device__ inline size_t env_ctx_bytes(int D, int nloc) {
            (size_t)((nloc + 1) & ~1) * 2 * sizeof(int);
 }
 
-// ---- one workgroup: the whole system ---------------------------------------
+// ---- one workgroup per problem: its whole system ---------------------------
 template <int DT>
-__global__ __launch_bounds__(256) void k_chol_env(double* H, int Pp, int D, const int* first, const double* g,
-                                                  double* x, double* winv, double* Lp, Control* ctl) {
+__global__ __launch_bounds__(256) void k_chol_env(const ProbDesc* probs, double* Hb, const int* firstb,
+                                                  const double* gb, double* xb, double* winvb, double* Lpb,
+                                                  Control* ctls) {
     extern __shared__ double lds[];
     __shared__ int sbad;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const ProbDesc pd = probs[blockIdx.x];
+    const int Pp = pd.Pp, D = pd.D;
+    double* H = Hb + pd.H_off;
+    const int* first = firstb + pd.first_off;
+    const double* g = gb + pd.vec_off;
+    double* x = xb + pd.vec_off;
+    double* winv = winvb + pd.winv_off;
+    double* Lp = Lpb + pd.Lp_off;
+    Control* ctl = ctls + blockIdx.x;
     const int nb = Pp / EB;
     double* end;
     EnvCtx c = env_ctx(lds, D, nb, &end);
@@ -835,14 +846,13 @@ size_t chol_split_lds(int Pp, int D, int part) {
     return env_ctx_bytes(nS - 1, nS) + env_ctx_bytes(0, nb) * 2;
 }
 
-void launch_chol_band(double* H, int Pp, int D, const int* first, const double* g, double* x, double* winv,
-                      double* Lp, Control* ctl, hipStream_t s) {
-    if (D <= 3)
-        hipLaunchKernelGGL(k_chol_env<3>, dim3(1), dim3(256), chol_band_lds(Pp, D), s, H, Pp, D, first, g, x, winv,
-                           Lp, ctl);
+void launch_chol_band(const ProbDesc* probs, int n_probs, int Dmax, size_t lds, double* H, const int* first,
+                      const double* g, double* x, double* winv, double* Lp, Control* ctl, hipStream_t s) {
+    if (Dmax <= 3)
+        hipLaunchKernelGGL(k_chol_env<3>, dim3(n_probs), dim3(256), lds, s, probs, H, first, g, x, winv, Lp, ctl);
     else
-        hipLaunchKernelGGL(k_chol_env<EDMAX>, dim3(1), dim3(256), chol_band_lds(Pp, D), s, H, Pp, D, first, g, x,
-                           winv, Lp, ctl);
+        hipLaunchKernelGGL(k_chol_env<EDMAX>, dim3(n_probs), dim3(256), lds, s, probs, H, first, g, x, winv, Lp,
+                           ctl);
 }
 
 void launch_chol_split(double* H, int Pp, int D, int s0, int s1, const int* firstT, const int* firstB,
@@ -861,10 +871,9 @@ void launch_chol_split(double* H, int Pp, int D, int s0, int s1, const int* firs
     }
 }
 
-int prepare_chol_band(int Pp, int D) {
-    const int bytes = (int)chol_band_lds(Pp, D);
-    const void* f = D <= 3 ? (const void*)k_chol_env<3> : (const void*)k_chol_env<EDMAX>;
-    return (int)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+int prepare_chol_band(int Dmax, size_t lds) {
+    const void* f = Dmax <= 3 ? (const void*)k_chol_env<3> : (const void*)k_chol_env<EDMAX>;
+    return (int)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 int prepare_chol_split(int Pp, int D) {

@@ -1,0 +1,140 @@
+// Host-side structures shared by the single-problem runtime and the batch
+// runtime: device allocations, the host image of a problem's device layout
+// and the solver handle behind include/spgo.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/spgo.h"
+#include "spgo_internal.h"
+
+namespace spgo {
+
+struct DevMem {
+    std::vector<void*> ptrs;
+    template <typename T>
+    T* alloc(size_t n) {
+        void* p = nullptr;
+        if (n == 0) n = 1;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return (T*)p;
+    }
+    void release() {
+        for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+    }
+};
+
+struct HostLayout {
+    std::vector<int> ob_t;
+    std::vector<uint32_t> ob_meta;
+    std::vector<float4> ob_uv;
+    std::vector<int> lm_id;
+    std::vector<float2> lm_ray;
+    std::vector<int2> lm_pos;
+    std::vector<int4> chunk;
+    std::vector<int> chunk_soff;    // per chunk KMAX + 2 offsets: first position of each slot
+    std::vector<int> grp_anchor, grp_K, grp_lm_begin, grp_d_off;
+    std::vector<int2> grp_chunks;
+    std::vector<long long> grp_v_off, grp_D_off;
+    std::vector<int> anchor_grp;
+    std::vector<GroupDesc> grp_desc;
+    std::vector<int> fs_begin;
+    std::vector<FsEntry> fs_list;
+    int kmax = 0;
+    long long v_total = 0, D_total = 0, d_total = 0;
+    int anchors_used = 0;
+};
+
+}  // namespace spgo
+
+struct spgo_solver {
+    using DevMem = spgo::DevMem;
+    using HostLayout = spgo::HostLayout;
+    using Layout = spgo::Layout;
+    using Control = spgo::Control;
+    using IterRecord = spgo::IterRecord;
+    template <typename T>
+    using StateBuf = spgo::StateBuf<T>;
+
+    int N = 0, M = 0, L = 0, F = 0;
+    long long max_meas = 0;
+    spgo_options opt{};
+    float step_size = 1.0f;
+    float intr[8] = {0};
+    float Trl[16] = {0};
+
+    std::unordered_map<long long, int> cell_of;   // frame * L + feat -> cell
+    std::vector<float> cells;                     // 6 floats per cell
+    std::vector<int> ma, mt, ml;
+
+    std::vector<double> X;        // N x 12
+    std::vector<double> alpha;    // L
+    bool host_newer = true;       // host mirror must be uploaded
+    bool dev_newer = false;       // device state newer than host mirror
+    bool layout_dirty = true;
+    bool lin_valid = false;
+    bool lambda_reset = true;
+    bool graph_dirty = true;
+
+    std::string err;
+
+    // device
+    bool dev_ready = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevMem mem;          // problem + buffers (rebuilt with the layout)
+    HostLayout hl;
+    Layout lay{};
+    int prec = SPGO_PREC_FP64;
+    StateBuf<double> bd[2]{};
+    StateBuf<float> bf[2]{};
+    double *H = nullptr, *gvec = nullptr, *delta = nullptr, *Linv = nullptr;
+    int Pp = 0;
+    bool band = false;          // envelope Cholesky in one workgroup (narrow profile)
+    const int* first = nullptr; // first nonzero block column of every block row
+    int band_depth = 0;         // max over block rows of (row - first)
+    int path = SPGO_FACTOR_DENSE;   // factorisation taken: SPGO_FACTOR_DENSE, _ENVELOPE or _ENVELOPE_SPLIT
+    int s0 = 0, s1 = 0;         // split: separator block rows [s0, s1)
+    const int* firstB = nullptr;   // split: profile of the reversed bottom part
+    double *Ssep = nullptr, *Smrg = nullptr, *winvS = nullptr, *LpS = nullptr;   // split: separator scratch
+    double* Lpan = nullptr;     // envelope: factored panels for the backward solve
+    Control* ctl = nullptr;
+    IterRecord* rec = nullptr;
+    int rec_cap = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+
+    std::vector<IterRecord> last_rec;
+    double last_cost0 = 0.0;
+    double last_ms = 0.0;
+
+    int fail(int code, const std::string& m) {
+        err = m;
+        return code;
+    }
+};
+
+namespace spgo {
+
+int check_hip(spgo_solver* h, hipError_t e, const char* what);
+// rotation part of a 3x4 pose replaced by its polar factor
+void project_rotation(double* P);
+// device layout of one problem; tile shape from the solver's options, or the
+// given chunk_landmarks / group_chunks (0: default)
+int build_layout(spgo_solver* h, HostLayout& out);
+int build_layout(spgo_solver* h, int chunk_landmarks, int group_chunks, HostLayout& out);
+// rows of the padded reduced pose system of n_poses poses
+int reduced_size(int n_poses);
+// block profile of the reduced system: fb[i] first nonzero block column of
+// block row i; returns the depth max(i - fb[i])
+int profile_blocks(int n_poses, const HostLayout& L, std::vector<int>& fb);
+// device state into the host mirror if the device is newer
+int pull_state(spgo_solver* h);
+Camera make_camera(const spgo_solver* h);
+
+}  // namespace spgo

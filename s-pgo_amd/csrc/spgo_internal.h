@@ -16,9 +16,10 @@ constexpr int WG = 256;             // threads per workgroup of the wide kernels
 // Everything the linearisation needs to start a landmark group, in one record
 // (one memory round trip): the group's fields and those of its first chunk.
 struct alignas(16) GroupDesc {
-    int anchor, K, chunk_begin, chunk_end;
-    int lm_begin, d_off, pad0, pad1;
+    int anchor, K, chunk_begin, chunk_end;   // anchor: frame index in the state arrays
+    int lm_begin, d_off, prob, a_loc;        // prob: problem of a batch (0 alone); a_loc: anchor inside it
     long long v_off, D_off;
+    long long delta_off, pad2;               // first entry of the problem's reduced-system solution
     int4 chunk0;                  // (first active landmark, n landmarks, first obs, n obs)
     int soff0[KMAX + 2];          // slot offsets of the first chunk
 };
@@ -28,6 +29,17 @@ struct alignas(16) GroupDesc {
 struct FsEntry {
     int anchor, K6, d_off, pad;
     long long D_off;
+};
+
+// One problem of a batch (a lone solver is a batch of one).  The per-problem
+// kernels run one workgroup per problem.
+struct ProbDesc {
+    int N, pose_base, frame_base, lm_base;   // poses; first pose, frame and landmark slot in the state arrays
+    int g_begin, g_end;                      // its landmark groups
+    int Pp, D;                               // padded reduced-system size, profile depth in blocks of 16
+    long long H_off, vec_off;                // its reduced system (Pp x Pp, row-major) and vectors (Pp)
+    long long winv_off, Lp_off;              // inverse diagonal blocks and factored panels
+    int first_off, rec_off;                  // block profile (Pp / 16 entries); first iteration record
 };
 
 // Device control block: which state buffer is current, damping, cost.
@@ -75,17 +87,12 @@ struct Layout {
     const int4* chunk;            // (first active landmark, n landmarks, first obs, n obs)
     const int* chunk_soff;        // per chunk KMAX + 2 entries: first position of slot s (slot-major order)
     // groups
-    const int* grp_anchor;
-    const int* grp_K;             // pose span of the group
-    const int2* grp_chunks;       // [begin, end) chunk range
-    const int* grp_lm_begin;      // first active landmark
-    const long long* grp_v_off;   // offset of the group's v block (6K per landmark)
-    const long long* grp_D_off;   // offset of the group's (6K)^2 block
-    const int* grp_d_off;         // offset of the group's 6K vector
-    const int* anchor_grp;        // CSR over anchor frames, F+1 entries
     const GroupDesc* grp_desc;    // per group, for k_assemble
     const int* fs_begin;          // per pose row block (N + 1): range in fs_list
     const FsEntry* fs_list;       // covering groups per pose row block, anchors ascending
+    int n_probs;                  // problems (1 for a lone solver)
+    const ProbDesc* probs;
+    const int* pose_prob;         // problem of every pose (N entries)
 };
 
 // One of the two state/linearisation buffer sets (current and candidate).

@@ -11,7 +11,7 @@ struct AsmArgs {
     Camera cam;
     Robust rb;
     double eps, alpha_min, alpha_max;
-    const Control* ctl;
+    const Control* ctl;       // per problem (one alone), indexed by GroupDesc::prob
     int init;                 // 1: linearise the current state in place
     const double* delta;      // reduced-system solution (candidate mode)
 };
@@ -20,12 +20,14 @@ template <typename T>
 void launch_assemble(const AsmArgs& a, const StateBuf<T>* bufs, hipStream_t s);
 
 template <typename T>
-void launch_form_system(const Layout& lay, const StateBuf<T>* bufs, Control* ctl,
-                        double* H, double* g, int Pp, hipStream_t s);
+void launch_form_system(const Layout& lay, const StateBuf<T>* bufs, Control* ctls, double* H, double* g,
+                        hipStream_t s);
 
 void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t s);
-void launch_chol_band(double* H, int Pp, int D, const int* first, const double* g, double* x, double* winv,
-                      double* Lp, Control* ctl, hipStream_t s);
+// envelope Cholesky and both solves, one workgroup per problem of the table;
+// Dmax the largest profile depth, lds the largest chol_band_lds of the problems
+void launch_chol_band(const ProbDesc* probs, int n_probs, int Dmax, size_t lds, double* H, const int* first,
+                      const double* g, double* x, double* winv, double* Lp, Control* ctl, hipStream_t s);
 void launch_chol_split(double* H, int Pp, int D, int s0, int s1, const int* firstT, const int* firstB,
                        const double* g, double* x, double* winv, double* Lp, double* S, double* Sm, double* winvS,
                        double* LpS, Control* ctl, hipStream_t s);
@@ -33,7 +35,7 @@ int prepare_chol_split(int Pp, int D);
 size_t chol_split_lds(int Pp, int D, int part);
 int chol_band_max_blocks();
 size_t chol_panel_doubles(int Pp);
-int prepare_chol_band(int Pp, int D);
+int prepare_chol_band(int Dmax, size_t lds);
 int chol_band_block();
 size_t chol_band_lds(int Pp, int D);
 void launch_chol_solve(const double* H, const double* Linv, const double* g, double* delta, int Pp,
@@ -43,9 +45,9 @@ template <typename T>
 void launch_pose_update(const Layout& lay, const Camera& cam, StateBuf<T>* bufs, const Control* ctl,
                         const double* delta, double step_size, int init, hipStream_t s);
 
-void launch_accept(const double* cost_buf0, const double* cost_buf1, int n_groups, Control* ctl,
-                   IterRecord* rec, int reference_mode, double lam_up, double lam_down, double lam_min,
-                   double lam_max, int init, hipStream_t s);
+void launch_accept(const double* cost_buf0, const double* cost_buf1, const Layout& lay, Control* ctl,
+                   IterRecord* rec, int reference_mode, double lam_up, double lam_down, double lam_min, double lam_max,
+                   int init, hipStream_t s);
 
 void launch_fill(double* p, double v, long long n, hipStream_t s);
 void launch_set_diag(double* H, int Pp, int from, double v, hipStream_t s);

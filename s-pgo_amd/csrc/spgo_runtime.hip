@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/spgo.h"
+#include "spgo_host.h"
 #include "spgo_internal.h"
 #include "spgo_kernels.h"
 
@@ -32,109 +33,9 @@ namespace {
 
 thread_local std::string g_create_error;
 
-struct DevMem {
-    std::vector<void*> ptrs;
-    template <typename T>
-    T* alloc(size_t n) {
-        void* p = nullptr;
-        if (n == 0) n = 1;
-        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
-        ptrs.push_back(p);
-        return (T*)p;
-    }
-    void release() {
-        for (void* p : ptrs) (void)hipFree(p);
-        ptrs.clear();
-    }
-};
-
-struct HostLayout {
-    std::vector<int> ob_t;
-    std::vector<uint32_t> ob_meta;
-    std::vector<float4> ob_uv;
-    std::vector<int> lm_id;
-    std::vector<float2> lm_ray;
-    std::vector<int2> lm_pos;
-    std::vector<int4> chunk;
-    std::vector<int> chunk_soff;    // per chunk KMAX + 2 offsets: first position of each slot
-    std::vector<int> grp_anchor, grp_K, grp_lm_begin, grp_d_off;
-    std::vector<int2> grp_chunks;
-    std::vector<long long> grp_v_off, grp_D_off;
-    std::vector<int> anchor_grp;
-    std::vector<GroupDesc> grp_desc;
-    std::vector<int> fs_begin;
-    std::vector<FsEntry> fs_list;
-    int kmax = 0;
-    long long v_total = 0, D_total = 0, d_total = 0;
-    int anchors_used = 0;
-};
-
-struct EngineBase {
-    virtual ~EngineBase() {}
-};
-
 }  // namespace
 
-struct spgo_solver {
-    int N = 0, M = 0, L = 0, F = 0;
-    long long max_meas = 0;
-    spgo_options opt{};
-    float step_size = 1.0f;
-    float intr[8] = {0};
-    float Trl[16] = {0};
-
-    std::unordered_map<long long, int> cell_of;   // frame * L + feat -> cell
-    std::vector<float> cells;                     // 6 floats per cell
-    std::vector<int> ma, mt, ml;
-
-    std::vector<double> X;        // N x 12
-    std::vector<double> alpha;    // L
-    bool host_newer = true;       // host mirror must be uploaded
-    bool dev_newer = false;       // device state newer than host mirror
-    bool layout_dirty = true;
-    bool lin_valid = false;
-    bool lambda_reset = true;
-    bool graph_dirty = true;
-
-    std::string err;
-
-    // device
-    bool dev_ready = false;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevMem mem;          // problem + buffers (rebuilt with the layout)
-    HostLayout hl;
-    Layout lay{};
-    int prec = SPGO_PREC_FP64;
-    StateBuf<double> bd[2]{};
-    StateBuf<float> bf[2]{};
-    double *H = nullptr, *gvec = nullptr, *delta = nullptr, *Linv = nullptr;
-    int Pp = 0;
-    bool band = false;          // envelope Cholesky in one workgroup (narrow profile)
-    const int* first = nullptr; // first nonzero block column of every block row
-    int band_depth = 0;         // max over block rows of (row - first)
-    int path = SPGO_FACTOR_DENSE;   // factorisation taken: SPGO_FACTOR_DENSE, _ENVELOPE or _ENVELOPE_SPLIT
-    int s0 = 0, s1 = 0;         // split: separator block rows [s0, s1)
-    const int* firstB = nullptr;   // split: profile of the reversed bottom part
-    double *Ssep = nullptr, *Smrg = nullptr, *winvS = nullptr, *LpS = nullptr;   // split: separator scratch
-    double* Lpan = nullptr;     // envelope: factored panels for the backward solve
-    Control* ctl = nullptr;
-    IterRecord* rec = nullptr;
-    int rec_cap = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-
-    std::vector<IterRecord> last_rec;
-    double last_cost0 = 0.0;
-    double last_ms = 0.0;
-
-    int fail(int code, const std::string& m) {
-        err = m;
-        return code;
-    }
-};
-
-namespace {
+namespace spgo {
 
 int check_hip(spgo_solver* h, hipError_t e, const char* what) {
     if (e == hipSuccess) return SPGO_OK;
@@ -179,11 +80,15 @@ void destroy_graph(spgo_solver* h) {
     h->graph_dirty = true;
 }
 
-// Build the device layout from the measurement list.
 int build_layout(spgo_solver* h, HostLayout& out) {
+    return build_layout(h, h->opt.chunk_landmarks, h->opt.group_chunks, out);
+}
+
+// Build the device layout from the measurement list.
+int build_layout(spgo_solver* h, int chunk_landmarks, int group_chunks_opt, HostLayout& out) {
     const int O = (int)h->ma.size();
-    const int chunk_lm = std::max(1, std::min(h->opt.chunk_landmarks > 0 ? h->opt.chunk_landmarks : CHUNK_LM, CHUNK_LM));
-    const int group_chunks = std::max(1, h->opt.group_chunks > 0 ? h->opt.group_chunks : 1);
+    const int chunk_lm = std::max(1, std::min(chunk_landmarks > 0 ? chunk_landmarks : CHUNK_LM, CHUNK_LM));
+    const int group_chunks = std::max(1, group_chunks_opt > 0 ? group_chunks_opt : 1);
     HostLayout hl;
     // measurements of each landmark, targets ascending (stable)
     std::vector<int> order(O);
@@ -323,6 +228,9 @@ int build_layout(spgo_solver* h, HostLayout& out) {
         GroupDesc& d = hl.grp_desc[g];
         d = GroupDesc{};
         d.anchor = hl.grp_anchor[g];
+        d.a_loc = hl.grp_anchor[g];
+        d.prob = 0;
+        d.delta_off = 0;
         d.K = hl.grp_K[g];
         d.chunk_begin = hl.grp_chunks[g].x;
         d.chunk_end = hl.grp_chunks[g].y;
@@ -349,31 +257,37 @@ int build_layout(spgo_solver* h, HostLayout& out) {
 
 // Size of the reduced system with padding: a whole number of blocks for
 // either factorisation.
-static int reduced_size(int n_poses) { return ((6 * n_poses + NB - 1) / NB) * NB; }
+int reduced_size(int n_poses) { return ((6 * n_poses + NB - 1) / NB) * NB; }
 
 // Block profile of the reduced system for the envelope factorisation: pose j
 // couples back to the smallest anchor among the landmark groups covering it,
 // fb[i] is the first nonzero block column of block row i and depth the
 // largest i - fb[i].  Returns the factorisation path (SPGO_FACTOR_DENSE,
 // _ENVELOPE or _ENVELOPE_SPLIT), or -1 if the requested one cannot be used.
-static int factor_path(const spgo_solver* h, const HostLayout& L, std::vector<int>& fb, int& depth) {
-    const int P = 6 * h->N, Pp = reduced_size(h->N), BBk = chol_band_block();
-    std::vector<int> fpose(h->N);
-    for (int j = 0; j < h->N; ++j) fpose[j] = j;
+int profile_blocks(int n_poses, const HostLayout& L, std::vector<int>& fb) {
+    const int P = 6 * n_poses, Pp = reduced_size(n_poses), BBk = chol_band_block();
+    std::vector<int> fpose(n_poses);
+    for (int j = 0; j < n_poses; ++j) fpose[j] = j;
     for (size_t g = 0; g < L.grp_anchor.size(); ++g)
         for (int k = 0; k < L.grp_K[g]; ++k) {
             const int j = L.grp_anchor[g] + k;
-            if (j < h->N) fpose[j] = std::min(fpose[j], L.grp_anchor[g]);
+            if (j < n_poses) fpose[j] = std::min(fpose[j], L.grp_anchor[g]);
         }
     const int nbk = Pp / BBk;
     fb.assign(nbk, 0);
-    depth = 0;
+    int depth = 0;
     for (int i = 0; i < nbk; ++i) {
         int f = i;
         for (int r = i * BBk; r < (i + 1) * BBk && r < P; ++r) f = std::min(f, (6 * fpose[r / 6]) / BBk);
         fb[i] = f;
         depth = std::max(depth, i - f);
     }
+    return depth;
+}
+
+static int factor_path(const spgo_solver* h, const HostLayout& L, std::vector<int>& fb, int& depth) {
+    const int Pp = reduced_size(h->N), nbk = Pp / chol_band_block();
+    depth = profile_blocks(h->N, L, fb);
     const int lim = 160 * 1024 - 1024;
     const bool fits = depth <= chol_band_max_blocks() && chol_band_lds(Pp, depth) <= (size_t)lim;
     const int nS = std::max(depth, 1);
@@ -462,11 +376,7 @@ int setup_layout(spgo_solver* h) {
     if ((rc = upload(h, L.ob_t, &lay.ob_t)) || (rc = upload(h, L.ob_meta, &lay.ob_meta)) ||
         (rc = upload(h, L.ob_uv, &lay.ob_uv)) || (rc = upload(h, L.lm_id, &lay.lm_id)) ||
         (rc = upload(h, L.lm_ray, &lay.lm_ray)) || (rc = upload(h, L.lm_pos, &lay.lm_pos)) ||
-        (rc = upload(h, L.chunk, &lay.chunk)) || (rc = upload(h, L.chunk_soff, &lay.chunk_soff)) || (rc = upload(h, L.grp_desc, &lay.grp_desc)) || (rc = upload(h, L.fs_begin, &lay.fs_begin)) || (rc = upload(h, L.fs_list, &lay.fs_list)) || (rc = upload(h, L.grp_anchor, &lay.grp_anchor)) ||
-        (rc = upload(h, L.grp_K, &lay.grp_K)) || (rc = upload(h, L.grp_chunks, &lay.grp_chunks)) ||
-        (rc = upload(h, L.grp_lm_begin, &lay.grp_lm_begin)) || (rc = upload(h, L.grp_v_off, &lay.grp_v_off)) ||
-        (rc = upload(h, L.grp_D_off, &lay.grp_D_off)) || (rc = upload(h, L.grp_d_off, &lay.grp_d_off)) ||
-        (rc = upload(h, L.anchor_grp, &lay.anchor_grp)))
+        (rc = upload(h, L.chunk, &lay.chunk)) || (rc = upload(h, L.chunk_soff, &lay.chunk_soff)) || (rc = upload(h, L.grp_desc, &lay.grp_desc)) || (rc = upload(h, L.fs_begin, &lay.fs_begin)) || (rc = upload(h, L.fs_list, &lay.fs_list)))
         return rc;
     h->prec = h->opt.precision;
     if (h->prec == SPGO_PREC_MIXED)
@@ -494,7 +404,7 @@ int setup_layout(spgo_solver* h) {
             return h->fail(SPGO_E_DEVICE, "device allocation failed");
         if ((rc = upload(h, fb, &h->first))) return rc;
         h->band_depth = bw;
-        if (h->path == SPGO_FACTOR_ENVELOPE && prepare_chol_band(h->Pp, bw) != 0)
+        if (h->path == SPGO_FACTOR_ENVELOPE && prepare_chol_band(bw, chol_band_lds(h->Pp, bw)) != 0)
             return h->fail(SPGO_E_DEVICE, "cannot reserve LDS for the envelope Cholesky");
         if (h->path == SPGO_FACTOR_ENVELOPE_SPLIT) {
             // separator of D block rows in the middle; the bottom part's profile in
@@ -529,6 +439,19 @@ int setup_layout(spgo_solver* h) {
     h->ctl = h->mem.alloc<Control>(1);
     if (!h->H || !h->gvec || !h->delta || !h->Linv || !h->ctl)
         return h->fail(SPGO_E_DEVICE, "device allocation failed");
+    {
+        // a problem table of one entry: the batch kernels' view of this solver
+        ProbDesc pd{};
+        pd.N = h->N;
+        pd.g_begin = 0;
+        pd.g_end = lay.n_groups;
+        pd.Pp = h->Pp;
+        pd.D = h->band_depth;
+        std::vector<ProbDesc> probs(1, pd);
+        std::vector<int> pose_prob(h->N, 0);
+        if ((rc = upload(h, probs, &lay.probs)) || (rc = upload(h, pose_prob, &lay.pose_prob))) return rc;
+        lay.n_probs = 1;
+    }
     launch_fill(h->H, 0.0, (long long)h->Pp * h->Pp, h->stream);
     launch_set_diag(h->H, h->Pp, P, 1.0, h->stream);
     launch_fill(h->gvec, 0.0, h->Pp, h->stream);
@@ -581,13 +504,14 @@ static void launch_chol(spgo_solver* h, hipStream_t s) {
         launch_chol_split(h->H, h->Pp, h->band_depth, h->s0, h->s1, h->first, h->firstB, h->gvec, h->delta, h->Linv,
                           h->Lpan, h->Ssep, h->Smrg, h->winvS, h->LpS, h->ctl, s);
     else
-        launch_chol_band(h->H, h->Pp, h->band_depth, h->first, h->gvec, h->delta, h->Linv, h->Lpan, h->ctl, s);
+        launch_chol_band(h->lay.probs, 1, h->band_depth, chol_band_lds(h->Pp, h->band_depth), h->H, h->first, h->gvec,
+                         h->delta, h->Linv, h->Lpan, h->ctl, s);
 }
 
 template <typename T>
 void launch_iteration(spgo_solver* h, StateBuf<T>* b) {
     hipStream_t s = h->stream;
-    launch_form_system<T>(h->lay, b, h->ctl, h->H, h->gvec, h->Pp, s);
+    launch_form_system<T>(h->lay, b, h->ctl, h->H, h->gvec, s);
     if (h->band) {
         launch_chol(h, s);
     } else {
@@ -596,8 +520,7 @@ void launch_iteration(spgo_solver* h, StateBuf<T>* b) {
     }
     launch_pose_update<T>(h->lay, make_camera(h), b, h->ctl, h->delta, (double)h->step_size, 0, s);
     launch_assemble<T>(make_asm(h, 0), b, s);
-    const int n_groups = h->lay.n_groups;
-    launch_accept(b[0].cost, b[1].cost, n_groups, h->ctl, h->rec, h->opt.mode == SPGO_MODE_REFERENCE,
+    launch_accept(b[0].cost, b[1].cost, h->lay, h->ctl, h->rec, h->opt.mode == SPGO_MODE_REFERENCE,
                   h->opt.lam_up, h->opt.lam_down, h->opt.lam_min, h->opt.lam_max, 0, s);
 }
 
@@ -606,7 +529,7 @@ void launch_linearise(spgo_solver* h, StateBuf<T>* b) {
     hipStream_t s = h->stream;
     launch_pose_update<T>(h->lay, make_camera(h), b, h->ctl, h->delta, 0.0, 1, s);
     launch_assemble<T>(make_asm(h, 1), b, s);
-    launch_accept(b[0].cost, b[1].cost, h->lay.n_groups, h->ctl, h->rec, 0, 0, 0, 0, 0, 1, s);
+    launch_accept(b[0].cost, b[1].cost, h->lay, h->ctl, h->rec, 0, 0, 0, 0, 0, 1, s);
 }
 
 // Copy the device state into the host mirror if the device is newer.
@@ -1111,9 +1034,9 @@ int spgo_linearize(spgo_solver* h, double lambda, double* H, double* g, double* 
     rc = check_hip(h, hipMemcpy(h->ctl, &c, sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");
     if (rc) return rc;
     if (h->prec == SPGO_PREC_MIXED)
-        launch_form_system<float>(h->lay, h->bf, h->ctl, h->H, h->gvec, h->Pp, h->stream);
+        launch_form_system<float>(h->lay, h->bf, h->ctl, h->H, h->gvec, h->stream);
     else
-        launch_form_system<double>(h->lay, h->bd, h->ctl, h->H, h->gvec, h->Pp, h->stream);
+        launch_form_system<double>(h->lay, h->bd, h->ctl, h->H, h->gvec, h->stream);
     rc = check_hip(h, hipStreamSynchronize(h->stream), "form system");
     if (rc) return rc;
     const int P = 6 * h->N;
@@ -1152,8 +1075,8 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
     const bool mixed = h->prec == SPGO_PREC_MIXED;
     for (int it = 0; it < iterations && rc == SPGO_OK; ++it) {
         (void)hipEventRecord(ev[0], s);
-        if (mixed) launch_form_system<float>(h->lay, h->bf, h->ctl, h->H, h->gvec, h->Pp, s);
-        else launch_form_system<double>(h->lay, h->bd, h->ctl, h->H, h->gvec, h->Pp, s);
+        if (mixed) launch_form_system<float>(h->lay, h->bf, h->ctl, h->H, h->gvec, s);
+        else launch_form_system<double>(h->lay, h->bd, h->ctl, h->H, h->gvec, s);
         (void)hipEventRecord(ev[1], s);
         if (h->band) {
             launch_chol(h, s);
@@ -1171,11 +1094,11 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
         else launch_assemble<double>(make_asm(h, 0), h->bd, s);
         (void)hipEventRecord(ev[5], s);
         if (mixed)
-            launch_accept(h->bf[0].cost, h->bf[1].cost, h->lay.n_groups, h->ctl, h->rec,
+            launch_accept(h->bf[0].cost, h->bf[1].cost, h->lay, h->ctl, h->rec,
                           h->opt.mode == SPGO_MODE_REFERENCE, h->opt.lam_up, h->opt.lam_down, h->opt.lam_min,
                           h->opt.lam_max, 0, s);
         else
-            launch_accept(h->bd[0].cost, h->bd[1].cost, h->lay.n_groups, h->ctl, h->rec,
+            launch_accept(h->bd[0].cost, h->bd[1].cost, h->lay, h->ctl, h->rec,
                           h->opt.mode == SPGO_MODE_REFERENCE, h->opt.lam_up, h->opt.lam_down, h->opt.lam_min,
                           h->opt.lam_max, 0, s);
         (void)hipEventRecord(ev[6], s);
