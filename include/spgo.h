@@ -187,6 +187,63 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
 /* Message of the last failed call on h (h == NULL: last failed spgo_create). */
 const char* spgo_last_error(spgo_solver* h);
 
+/* ---- batches of independent problems ----------------------------------------
+ * No reference counterpart: the reference steps one Solver per problem
+ * (include/Solver.hpp:18-85, one launch sequence per step in
+ * Solver_cuda.cu:557-795).  A batch steps many solver handles together: their
+ * measurements, calibration and state are written through the calls above,
+ * then each LM phase runs as one launch over all of them (the reduced systems
+ * factored by the envelope Cholesky, one workgroup per problem).  Every
+ * problem keeps its own damping and accept/reject, so it takes the steps it
+ * would take alone; the options (precision, robust kernel, tile shape) are the
+ * batch's and apply to all problems.  The problems must share one stereo
+ * calibration; the pose update is applied with step size 1. */
+typedef struct spgo_batch spgo_batch;
+
+typedef struct spgo_batch_info {
+    int32_t n_problems;
+    int32_t n_poses, n_landmarks, n_measurements, n_groups;   /* totals; 0 before the first step */
+    int32_t max_depth;        /* deepest reduced-system profile, blocks of 16 */
+    int32_t max_span;         /* longest landmark pose span                   */
+    int32_t chunk_landmarks, group_chunks;   /* tile shape of the options    */
+    int32_t reserved[7];
+} spgo_batch_info;
+
+/* The handles must outlive the batch.  NULL opts = spgo_default_options. */
+int spgo_batch_create(int32_t n_problems, spgo_solver* const* problems, const spgo_options* opts, spgo_batch** out);
+int spgo_batch_destroy(spgo_batch* b);
+int spgo_batch_set_options(spgo_batch* b, const spgo_options* opts);
+int spgo_batch_get_options(spgo_batch* b, spgo_options* out);
+/* The first step (and the first after spgo_batch_reload, or after a change of
+ * precision or tile shape) reads every problem's measurements and current
+ * state; later steps continue from the batch's own device state. */
+int spgo_batch_reload(spgo_batch* b);
+/* Re-read only the problems' states (not their measurements) and reset every
+ * problem's damping to lambda0; the next step linearises them first. */
+int spgo_batch_reset_states(spgo_batch* b);
+int spgo_batch_step(spgo_batch* b, int32_t iterations);
+/* State of one problem, as spgo_get_state_f64 (the batch's state once stepped). */
+int spgo_batch_get_state_f64(spgo_batch* b, int32_t problem, double* poses, double* alpha);
+/* Write every problem's batch state into its solver, as spgo_set_state_f64. */
+int spgo_batch_store(spgo_batch* b);
+/* Per-iteration records of one problem from the last spgo_batch_step. */
+int spgo_batch_get_stats(spgo_batch* b, int32_t problem, spgo_iter_stats* out, int32_t max_records,
+                         int32_t* n_out, double* initial_cost);
+/* Device time (ms) of the iterations of the last spgo_batch_step. */
+int spgo_batch_last_step_ms(spgo_batch* b, double* ms);
+int spgo_batch_get_info(spgo_batch* b, spgo_batch_info* out);
+/* As spgo_profile_phases, over the whole batch (phase 2, the solves, is part of
+ * phase 1: the envelope kernel factors and solves). */
+int spgo_batch_profile_phases(spgo_batch* b, int32_t iterations, double ms[SPGO_N_PHASES]);
+/* Tile-shape search over chunk_landmarks in {16, 32, 48, 64} and group_chunks
+ * in {1, 2, 4}: each shape is timed over `iterations` LM iterations from the
+ * problems' current states; the fastest is kept in the batch's options and the
+ * batch is left to reload the problems on its next step. */
+int spgo_batch_autotune(spgo_batch* b, int32_t iterations, int32_t* chunk_landmarks, int32_t* group_chunks,
+                        double* best_ms);
+/* Message of the last failed call on b (b == NULL: last failed spgo_batch_create). */
+const char* spgo_batch_last_error(spgo_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -136,5 +136,7 @@ int profile_blocks(int n_poses, const HostLayout& L, std::vector<int>& fb);
 // device state into the host mirror if the device is newer
 int pull_state(spgo_solver* h);
 Camera make_camera(const spgo_solver* h);
+// options within the ranges the kernels support; why names the first bad field
+bool valid_opts(const spgo_options* o, std::string& why);
 
 }  // namespace spgo

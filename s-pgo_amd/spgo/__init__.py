@@ -4,6 +4,6 @@
 ABI) through ctypes; ``build_ip.Solver.CudaSolver`` is the drop-in for the
 reference's Cython class.
 """
-from .solver import Solver, make_options, solver_from_problem  # noqa: F401
+from .solver import Batch, Solver, make_options, solver_from_problem  # noqa: F401
 
-__all__ = ["Solver", "make_options", "solver_from_problem"]
+__all__ = ["Batch", "Solver", "make_options", "solver_from_problem"]

@@ -51,6 +51,13 @@ class LayoutInfo(ctypes.Structure):
                 ("profile_depth", c_int32), ("reduced_size", c_int32), ("reserved", c_int32)]
 
 
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("n_problems", c_int32), ("n_poses", c_int32), ("n_landmarks", c_int32),
+                ("n_measurements", c_int32), ("n_groups", c_int32), ("max_depth", c_int32),
+                ("max_span", c_int32), ("chunk_landmarks", c_int32), ("group_chunks", c_int32),
+                ("reserved", c_int32 * 7)]
+
+
 F32P = POINTER(c_float)
 F64P = POINTER(c_double)
 I32P = POINTER(c_int32)
@@ -84,6 +91,21 @@ SIGNATURES = {
     "spgo_last_step_ms": (c_int32, [c_void_p, F64P]),
     "spgo_profile_phases": (c_int32, [c_void_p, c_int32, F64P]),
     "spgo_last_error": (c_char_p, [c_void_p]),
+    "spgo_batch_create": (c_int32, [c_int32, POINTER(c_void_p), POINTER(Options), POINTER(c_void_p)]),
+    "spgo_batch_destroy": (c_int32, [c_void_p]),
+    "spgo_batch_set_options": (c_int32, [c_void_p, POINTER(Options)]),
+    "spgo_batch_get_options": (c_int32, [c_void_p, POINTER(Options)]),
+    "spgo_batch_reload": (c_int32, [c_void_p]),
+    "spgo_batch_reset_states": (c_int32, [c_void_p]),
+    "spgo_batch_step": (c_int32, [c_void_p, c_int32]),
+    "spgo_batch_get_state_f64": (c_int32, [c_void_p, c_int32, F64P, F64P]),
+    "spgo_batch_store": (c_int32, [c_void_p]),
+    "spgo_batch_get_stats": (c_int32, [c_void_p, c_int32, POINTER(IterStats), c_int32, I32P, F64P]),
+    "spgo_batch_last_step_ms": (c_int32, [c_void_p, F64P]),
+    "spgo_batch_get_info": (c_int32, [c_void_p, POINTER(BatchInfo)]),
+    "spgo_batch_profile_phases": (c_int32, [c_void_p, c_int32, F64P]),
+    "spgo_batch_autotune": (c_int32, [c_void_p, c_int32, I32P, I32P, F64P]),
+    "spgo_batch_last_error": (c_char_p, [c_void_p]),
 }
 
 _lib = None
@@ -105,10 +127,10 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     return lib
 
 
-def check(rc: int, handle=None) -> None:
+def check(rc: int, handle=None, batch: bool = False) -> None:
     if rc != 0:
         lib = load()
-        msg = lib.spgo_last_error(handle)
+        msg = (lib.spgo_batch_last_error if batch else lib.spgo_last_error)(handle)
         msg = msg.decode() if msg else ""
         raise RuntimeError(f"spgo error {rc} ({ERRORS.get(rc, '?')}): {msg}")
 

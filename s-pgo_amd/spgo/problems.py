@@ -131,7 +131,7 @@ def look_at_origin(position: np.ndarray) -> np.ndarray:
 # --------------------------------------------------------------------------
 # generator
 # --------------------------------------------------------------------------
-def make_problem(n_poses: int, n_landmarks: int, obs_per_landmark: int, seed: int,
+def make_problem(n_poses: int, n_landmarks: int, obs_per_landmark, seed: int,
                  fx: float = 500.0, fy: float = 500.0, cx: float = 320.0, cy: float = 240.0,
                  width: int = 640, height: int = 480, baseline: float = 0.12,
                  radius: float = 2.0, box: float = 4.0, noise_px: float = 1.0,
@@ -147,7 +147,8 @@ def make_problem(n_poses: int, n_landmarks: int, obs_per_landmark: int, seed: in
       is hosted by frame ``a`` and observed (both cameras valid: depth above
       ``min_depth``, pixel inside the image) in the ``obs_per_landmark``
       consecutive frames ``a .. a+obs_per_landmark-1``.  Candidates without such
-      a run are redrawn.
+      a run are redrawn.  ``obs_per_landmark`` may be a pair ``(lo, hi)``: each
+      candidate then draws its own count uniformly in ``lo..hi``.
     * Pixel noise N(0, noise_px^2) on every coordinate; a fraction
       ``outlier_frac`` of the non-host stereo observations get an extra
       U(-outlier_px, outlier_px) error per coordinate.
@@ -159,8 +160,13 @@ def make_problem(n_poses: int, n_landmarks: int, obs_per_landmark: int, seed: in
     rng = np.random.default_rng(seed)
     N = int(n_poses)
     F = N + 1
-    K = int(obs_per_landmark)
-    if K < 1 or K > F:
+    if np.ndim(obs_per_landmark) == 0:
+        K = int(obs_per_landmark)
+        k_lo = k_hi = K
+    else:
+        k_lo, k_hi = (int(v) for v in obs_per_landmark)
+        K = None
+    if k_lo < 1 or k_hi > F or k_lo > k_hi:
         raise ValueError("obs_per_landmark must be in [1, n_poses + 1]")
 
     theta = 2.0 * math.pi * np.arange(F) / F
@@ -183,40 +189,59 @@ def make_problem(n_poses: int, n_landmarks: int, obs_per_landmark: int, seed: in
         v = fy * Pc[..., 1] / zs + cy
         return np.stack([u, v], -1), z
 
-    pts = []
-    hosts = []
-    need = int(n_landmarks)
-    n_start = F - K + 1
-    while need > 0:
-        m = max(4 * need, 1024)
-        cand = rng.uniform(-box / 2, box / 2, size=(m, 3))
-        pl, zl = project(Ginv, cand)
-        pr, zr = project(GRinv, cand)
-        ok = ((zl > min_depth) & (zr > min_depth)
-              & (pl[..., 0] >= 0) & (pl[..., 0] < width) & (pl[..., 1] >= 0) & (pl[..., 1] < height)
-              & (pr[..., 0] >= 0) & (pr[..., 0] < width) & (pr[..., 1] >= 0) & (pr[..., 1] < height))
-        # run of K consecutive visible frames starting at s, s in [0, F-K]
-        c = np.concatenate([np.zeros((m, 1), np.int64), np.cumsum(ok, axis=1)], axis=1)
-        run = (c[:, K:K + n_start] - c[:, 0:n_start]) == K          # (m, n_start)
-        has = run.any(axis=1)
-        # choose a start uniformly among valid ones
-        score = np.where(run, rng.random(run.shape), -1.0)
-        start = np.argmax(score, axis=1)
-        idx = np.nonzero(has)[0][:need]
-        pts.append(cand[idx])
-        hosts.append(start[idx])
-        need -= idx.size
+    def draw(k: int, need: int):
+        # `need` landmarks with a run of k consecutive visible frames
+        pts, hosts = [], []
+        n_start = F - k + 1
+        while need > 0:
+            m = max(4 * need, 1024)
+            cand = rng.uniform(-box / 2, box / 2, size=(m, 3))
+            pl, zl = project(Ginv, cand)
+            pr, zr = project(GRinv, cand)
+            ok = ((zl > min_depth) & (zr > min_depth)
+                  & (pl[..., 0] >= 0) & (pl[..., 0] < width) & (pl[..., 1] >= 0) & (pl[..., 1] < height)
+                  & (pr[..., 0] >= 0) & (pr[..., 0] < width) & (pr[..., 1] >= 0) & (pr[..., 1] < height))
+            # run of k consecutive visible frames starting at s, s in [0, F-k]
+            c = np.concatenate([np.zeros((m, 1), np.int64), np.cumsum(ok, axis=1)], axis=1)
+            run = (c[:, k:k + n_start] - c[:, 0:n_start]) == k          # (m, n_start)
+            has = run.any(axis=1)
+            # choose a start uniformly among valid ones
+            score = np.where(run, rng.random(run.shape), -1.0)
+            start = np.argmax(score, axis=1)
+            idx = np.nonzero(has)[0][:need]
+            pts.append(cand[idx])
+            hosts.append(start[idx])
+            need -= idx.size
+        return np.concatenate(pts), np.concatenate(hosts)
+
+    if K is not None:
+        pts, hosts = draw(K, int(n_landmarks))
+        pts, hosts = [pts], [hosts]
+        counts = [np.full(hosts[0].size, K, np.int64)]
+    else:
+        # every landmark's count drawn first, so the counts are exactly uniform
+        kd = rng.integers(k_lo, k_hi + 1, size=int(n_landmarks))
+        pts, hosts, counts = [], [], []
+        for k in range(k_lo, k_hi + 1):
+            nk = int((kd == k).sum())
+            if nk:
+                pk, hk = draw(k, nk)
+                pts.append(pk)
+                hosts.append(hk)
+                counts.append(np.full(nk, k, np.int64))
 
     P = np.concatenate(pts)[:n_landmarks]
     host = np.concatenate(hosts)[:n_landmarks].astype(np.int64)
+    kl = np.concatenate(counts)[:n_landmarks].astype(np.int64)
     order = np.argsort(host, kind="stable")       # landmark ids grouped by host frame
-    P, host = P[order], host[order]
+    P, host, kl = P[order], host[order], kl[order]
     L = P.shape[0]
 
-    # measurements: landmark-major, frames host..host+K-1 (CUDAmanager.py:69-79)
-    meas_l = np.repeat(np.arange(L), K)
-    meas_a = np.repeat(host, K)
-    meas_t = meas_a + np.tile(np.arange(K), L)
+    # measurements: landmark-major, frames host..host+k-1 (CUDAmanager.py:69-79)
+    meas_l = np.repeat(np.arange(L), kl)
+    meas_a = np.repeat(host, kl)
+    first = np.concatenate([[0], np.cumsum(kl)[:-1]])
+    meas_t = meas_a + (np.arange(meas_l.size) - np.repeat(first, kl))
     Pm = P[meas_l]
     Pcl = np.einsum('nij,nj->ni', Ginv[meas_t, :3, :3], Pm) + Ginv[meas_t, :3, 3]
     Pcr = np.einsum('nij,nj->ni', GRinv[meas_t, :3, :3], Pm) + GRinv[meas_t, :3, 3]
@@ -267,6 +292,25 @@ CONFIGS = {
     "c3_mixed": dict(n_poses=256, n_landmarks=100000, obs_per_landmark=10, seed=3003,
                      outlier_frac=0.05, outlier_px=50.0),
 }
+
+
+# Batch of independent sliding windows: keyframes per window uniform in 8..32,
+# landmarks uniform in 200..2000, observations per landmark uniform in 3..8.
+WINDOW_BATCH = dict(n_windows=512, n_range=(8, 32), l_range=(200, 2000), k_range=(3, 8), seed=4004)
+
+
+def make_window_batch(n_windows: int, n_range=(8, 32), l_range=(200, 2000), k_range=(3, 8),
+                      seed: int = 4004, **kw) -> list:
+    """``n_windows`` independent problems from the same generator, each with its
+    own keyframe count, landmark count and seed (drawn from one stream)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for w in range(int(n_windows)):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        L = int(rng.integers(l_range[0], l_range[1] + 1))
+        s = int(rng.integers(0, 2**31 - 1))
+        out.append(make_problem(n, L, (k_range[0], k_range[1]), s, name=f"window{w}", **kw))
+    return out
 
 
 def make_config(name: str, **overrides) -> Problem:

@@ -13,7 +13,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import F32P, F64P, I32P, IterStats, LayoutInfo, Options
+from ._lib import BatchInfo, F32P, F64P, I32P, IterStats, LayoutInfo, Options
 
 
 def _f32(a, n: Optional[int] = None) -> np.ndarray:
@@ -232,3 +232,113 @@ def solver_from_problem(p, options: Optional[Options] = None) -> Solver:
         s.write_incremental_pose(k, p.X0[k].reshape(16))
     s.load_inverse_depths(p.alpha0)
     return s
+
+
+class Batch:
+    """Independent problems stepped together: one launch per LM phase for all.
+
+    The problems are ``Solver`` objects loaded through their own API (they must
+    share one stereo calibration); the batch holds references to them.  The
+    first ``step`` reads their measurements and states; later steps continue
+    from the batch's device state until ``reload``.  ``store`` writes the
+    batch's states back into the solvers.
+    """
+
+    def __init__(self, solvers, options: Optional[Options] = None):
+        self._lib = _lib.load()
+        self.solvers = list(solvers)
+        hs = (ctypes.c_void_p * len(self.solvers))(*[s._h.value for s in self.solvers])
+        h = ctypes.c_void_p()
+        rc = self._lib.spgo_batch_create(len(self.solvers), hs, ctypes.byref(options) if options is not None
+                                         else None, ctypes.byref(h))
+        _lib.check(rc, None, batch=True)
+        self._h = h
+
+    def _check(self, rc: int) -> None:
+        _lib.check(rc, self._h, batch=True)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.spgo_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self) -> int:
+        return len(self.solvers)
+
+    def set_options(self, o: Options) -> None:
+        self._check(self._lib.spgo_batch_set_options(self._h, ctypes.byref(o)))
+
+    def get_options(self) -> Options:
+        o = Options()
+        self._check(self._lib.spgo_batch_get_options(self._h, ctypes.byref(o)))
+        return o
+
+    def reload(self) -> None:
+        self._check(self._lib.spgo_batch_reload(self._h))
+
+    def reset_states(self) -> None:
+        """Restart from the solvers' states (measurements kept); damping reset."""
+        self._check(self._lib.spgo_batch_reset_states(self._h))
+
+    def step(self, iterations: int) -> None:
+        self._check(self._lib.spgo_batch_step(self._h, int(iterations)))
+
+    def get_state(self, p: int):
+        """(poses (N,4,4) float64, inverse depths (N*M,) float64) of problem p."""
+        s = self.solvers[p]
+        X = np.empty((s.N, 4, 4), np.float64)
+        al = np.empty(s.N * s.M, np.float64)
+        self._check(self._lib.spgo_batch_get_state_f64(self._h, int(p), X.ctypes.data_as(F64P),
+                                                       al.ctypes.data_as(F64P)))
+        return X, al
+
+    def store(self) -> None:
+        self._check(self._lib.spgo_batch_store(self._h))
+
+    def stats(self, p: int):
+        """(initial cost, per-iteration dicts) of problem p in the last step."""
+        n = ctypes.c_int32()
+        c0 = ctypes.c_double()
+        self._check(self._lib.spgo_batch_get_stats(self._h, int(p), None, 0, ctypes.byref(n), ctypes.byref(c0)))
+        buf = (IterStats * max(n.value, 1))()
+        self._check(self._lib.spgo_batch_get_stats(self._h, int(p), buf, n.value, ctypes.byref(n),
+                                                   ctypes.byref(c0)))
+        recs = [dict(cost_candidate=buf[i].cost_candidate, cost=buf[i].cost, lam=buf[i].lambda_,
+                     accepted=bool(buf[i].accepted), chol_ok=bool(buf[i].chol_ok)) for i in range(n.value)]
+        return c0.value, recs
+
+    def last_step_ms(self) -> float:
+        v = ctypes.c_double()
+        self._check(self._lib.spgo_batch_last_step_ms(self._h, ctypes.byref(v)))
+        return v.value
+
+    def info(self) -> dict:
+        i = BatchInfo()
+        self._check(self._lib.spgo_batch_get_info(self._h, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in BatchInfo._fields_ if k != "reserved"}
+
+    def profile_phases(self, iterations: int) -> dict:
+        """Mean device ms per LM iteration of each phase over the batch (the
+        envelope kernel's solves are inside "cholesky")."""
+        ms = np.zeros(len(Solver.PHASES), np.float64)
+        self._check(self._lib.spgo_batch_profile_phases(self._h, int(iterations), ms.ctypes.data_as(F64P)))
+        d = dict(zip(Solver.PHASES, ms.tolist()))
+        d.pop("solve")
+        return d
+
+    def autotune(self, iterations: int = 3) -> dict:
+        """Time every candidate tile shape and keep the fastest (see
+        spgo_batch_autotune); returns the chosen shape and its device time."""
+        cl = ctypes.c_int32()
+        gc = ctypes.c_int32()
+        ms = ctypes.c_double()
+        self._check(self._lib.spgo_batch_autotune(self._h, int(iterations), ctypes.byref(cl), ctypes.byref(gc),
+                                                  ctypes.byref(ms)))
+        return {"chunk_landmarks": cl.value, "group_chunks": gc.value, "ms": ms.value}
+
