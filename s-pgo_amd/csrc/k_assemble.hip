@@ -141,15 +141,27 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
     const int lane = tid & 63;
     const int wv = tid >> 6;
 
-    const GroupDesc& gd = lay.grp_desc[g];      // one record: no chain of dependent loads
-    const Control* ctl = A.ctl + gd.prob;       // the group's problem (a batch holds several)
+    // The group record is copied into registers before the kernel's first global
+    // store.  A load issued after a store returns only after that store has
+    // completed (the vector memory counter retires in order), and the compiler
+    // must reload any field it cannot prove unchanged by the stores.  So every
+    // global load of this kernel sits before its stores.
+    const GroupDesc* gdp = lay.grp_desc + g;
+    const int a = gdp->anchor;
+    const int K = gdp->K;
+    const int prob = gdp->prob, a_loc = gdp->a_loc;
+    const int chunk_begin = gdp->chunk_begin, chunk_end = gdp->chunk_end;
+    const int glm0 = gdp->lm_begin, d_off = gdp->d_off;
+    const long long voff = gdp->v_off, D_off = gdp->D_off, delta_off = gdp->delta_off;
+    const int4 chunk0 = gdp->chunk0;
+    const Control* ctl = A.ctl + prob;          // the group's problem (a batch holds several)
     const int cur = ctl->cur;
-    const StateBuf<T>& bin = cur ? B1 : B0;                    // linearisation point
-    const StateBuf<T>& bout = A.init ? bin : (cur ? B0 : B1);  // state being linearised
+    // by value: a selected reference to the kernel arguments would put both
+    // structures in scratch memory and read every pointer back from there
+    const StateBuf<T> bin = cur ? B1 : B0;                    // linearisation point
+    const StateBuf<T> bout = A.init ? bin : (cur ? B0 : B1);  // state being linearised
     const bool update = !A.init;
     const bool chol_ok = update ? (ctl->chol_ok != 0) : true;
-    const int a = gd.anchor;
-    const int K = gd.K;
     const int K6 = 6 * K;
     const int nT = (K6 + 16) >> 4;             // tile rows: B columns and the gradient row K6
     const int nTiles = nT * (nT + 1) / 2;
@@ -168,38 +180,43 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
     __shared__ T dly[6 * KT];
     __shared__ double red[NWAVE];
     __shared__ double Adk[KT][36];              // Ad(G_a^-1 G_{a+k}) of the state being linearised
+    __shared__ int soffL[KMAX + 2];             // slot offsets of the current chunk
 
-    // ---- group set-up ------------------------------------------------------
-    for (int i = tid; i < (KT + 1) * 27; i += WG) (&S[0][0])[i] = (T)0;
-    if (tid < 2 * (K + 1)) {
-        const int c = tid / (K + 1), s = tid % (K + 1);
-        double Ga[12], Gti[12], Tr[12];
-        pose_load(bout.G + 12 * a, Ga);
-        pose_load((c ? bout.GinvR : bout.GinvL) + 12 * (a + s), Gti);
-        pose_mul(Gti, Ga, Tr);
+    // ---- group set-up: relative poses, adjoints, the update mapped to the anchor
+    // frame (run in the first chunk, once its measurement loads are in flight)
+    auto group_setup = [&]() __attribute__((always_inline)) {
+        for (int i = tid; i < (KT + 1) * 27; i += WG) (&S[0][0])[i] = (T)0;
+        if (tid < 2 * (K + 1)) {
+            const int c = tid / (K + 1), s = tid % (K + 1);
+            double Ga[12], Gti[12], Tr[12];
+            pose_load(bout.G + 12 * a, Ga);
+            pose_load((c ? bout.GinvR : bout.GinvL) + 12 * (a + s), Gti);
+            pose_mul(Gti, Ga, Tr);
 #pragma unroll
-        for (int i = 0; i < 12; ++i) Trel[c][s][i] = (T)Tr[i];
-    }
-    if (tid >= 64 && tid < 64 + K) {
-        const int k = tid - 64;
-        double Gai[12], Gi[12], E[12];
-        pose_load(bout.GinvL + 12 * a, Gai);
-        pose_load(bout.G + 12 * (a + k), Gi);
-        pose_mul(Gai, Gi, E);
-        for (int e = 0; e < 36; ++e) Adk[k][e] = adjoint_entry(E, e / 6, e % 6);
-    }
-    if (update && tid >= 128 && tid < 128 + K) {
-        const int k = tid - 128;
-        double Gai[12], Gi[12], E[12], x[6], out[6];
-        pose_load(bin.GinvL + 12 * a, Gai);
-        pose_load(bin.G + 12 * (a + k), Gi);
-        pose_mul(Gai, Gi, E);
+            for (int i = 0; i < 12; ++i) Trel[c][s][i] = (T)Tr[i];
+        }
+        if (tid >= 64 && tid < 64 + K) {
+            const int k = tid - 64;
+            double Gai[12], Gi[12], E[12];
+            pose_load(bout.GinvL + 12 * a, Gai);
+            pose_load(bout.G + 12 * (a + k), Gi);
+            pose_mul(Gai, Gi, E);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) x[q] = A.delta[gd.delta_off + 6 * (gd.a_loc + k) + q];
-        adjoint_apply(E, x, out);
+            for (int e = 0; e < 36; ++e) Adk[k][e] = adjoint_entry(E, e / 6, e % 6);
+        }
+        if (update && tid >= 128 && tid < 128 + K) {
+            const int k = tid - 128;
+            double Gai[12], Gi[12], E[12], x[6], out[6];
+            pose_load(bin.GinvL + 12 * a, Gai);
+            pose_load(bin.G + 12 * (a + k), Gi);
+            pose_mul(Gai, Gi, E);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) dl[k][q] = (T)out[q];
-    }
+            for (int q = 0; q < 6; ++q) x[q] = A.delta[delta_off + 6 * (a_loc + k) + q];
+            adjoint_apply(E, x, out);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) dl[k][q] = (T)out[q];
+        }
+    };
 
     typename Vec4<T>::type acc[SH::MAXTILE];
 #pragma unroll
@@ -209,14 +226,13 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
 
     const T fx0 = (T)A.cam.fx[0], fy0 = (T)A.cam.fy[0], cx0 = (T)A.cam.cx[0], cy0 = (T)A.cam.cy[0];
     const T inv_param = A.rb.param != 0.0 ? (T)(1.0 / A.rb.param) : (T)0;
-    const int glm0 = gd.lm_begin;
-    const long long voff = gd.v_off;
 
-    for (int ci = gd.chunk_begin; ci < gd.chunk_end; ++ci) {
-        const bool first = ci == gd.chunk_begin;
-        const int4 ch = first ? gd.chunk0 : lay.chunk[ci];
+    for (int ci = chunk_begin; ci < chunk_end; ++ci) {
+        const bool first = ci == chunk_begin;
+        const int4 ch = first ? chunk0 : lay.chunk[ci];
         const int lm0 = ch.x, nl = ch.y, ob0 = ch.z;
-        const int* soff = first ? gd.soff0 : lay.chunk_soff + (long long)ci * (KMAX + 2);
+        const int* soff = first ? gdp->soff0 : lay.chunk_soff + (long long)ci * (KMAX + 2);
+        if (tid < KMAX + 2) soffL[tid] = soff[tid];
         // everything phases 0 and 1 read from global memory is requested now:
         // the measurements of this wavefront's first slot segments and the
         // landmark records
@@ -235,22 +251,23 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
             lray = lay.lm_ray[lm0 + tid];
             lps = lay.lm_pos[lm0 + tid];
         }
+        if (first) group_setup();
         // ---- phase 0: back-substitution and anchor-frame point per landmark ----
         // the chunk's B columns (nl x 6K, contiguous) into LDS with coalesced loads
         const T* vin = bin.v + voff + (long long)(lm0 - glm0) * K6;
-        if (update && chol_ok) {
+        const int nv = nl * K6;
+        if (update && chol_ok && nv > 0) {
             // loads in flight in batches of NB before their stores (two round trips
-            // for spans up to 8)
+            // for spans up to 8); indices clamped so every load is unconditional
             constexpr int NV = (CHUNK_LM * 6 * KT + WG - 1) / WG;
             constexpr int NBV = 6;
-            const int nv = nl * K6;
 #pragma unroll
             for (int r0 = 0; r0 < NV; r0 += NBV) {
                 T vv[NBV];
 #pragma unroll
                 for (int r = 0; r < NBV; ++r) {
                     const int i = tid + (r0 + r) * WG;
-                    vv[r] = (r0 + r < NV && i < nv) ? vin[i] : (T)0;
+                    if (r0 + r < NV) vv[r] = vin[min(i, nv - 1)];
                 }
 #pragma unroll
                 for (int r = 0; r < NBV; ++r) {
@@ -261,11 +278,12 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
         }
         for (int i = tid; i < CHUNK_LM * (6 * KT + 1); i += WG) U[i] = (T)0;
         lds_barrier();   // also orders the set-up above
+        double al = 0.0;   // the landmark's new inverse depth, stored with C and g_alpha
         if (tid < nl) {
             const int act = lm0 + tid;
             const int id = lid;
             lpos[tid] = lps;
-            double al = bin.alpha[id];
+            al = bin.alpha[id];
             if (update) {
                 double dal = 0.0;
                 if (chol_ok) {
@@ -277,7 +295,6 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
                     dal = s / (double)bin.Ct[act];
                 }
                 al = fmin(fmax(al + dal, A.alpha_min), A.alpha_max);
-                bout.alpha[id] = al;
             }
             const float2 ray = lray;
             const double x = ((double)ray.x - A.cam.cx[0]) / A.cam.fx[0];
@@ -297,7 +314,7 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
         // is the contiguous segment soff[s] .. soff[s+1]; its pair sums are one
         // reduction over the wavefront (no atomics, fixed order).
         for (int seg = wv; seg <= K; seg += NWAVE) {
-            const int sb = soff[seg], se = soff[seg + 1];
+            const int sb = soffL[seg], se = soffL[seg + 1];
             T pr[27];
 #pragma unroll
             for (int q = 0; q < 27; ++q) pr[q] = (T)0;
@@ -435,6 +452,7 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
             const int act = lm0 + tid;
             const T Ct = C + (T)A.eps;
             const T isc = (T)1 / sqrt(Ct);
+            if (update) bout.alpha[lid] = al;
             bout.Ct[act] = Ct;
             bout.g[act] = G;
             iscl[tid] = isc;
@@ -541,7 +559,7 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
     }
     lds_barrier();
     // lower blocks out (the reduced system reads block rows i >= j)
-    T* Dg = bout.D + gd.D_off;
+    T* Dg = bout.D + D_off;
     for (int it = tid; it < K6 * K6; it += WG) {
         const int row = it / K6, col = it - row * K6;
         if (row / 6 >= col / 6) Dg[it] = Dl[it];
@@ -551,7 +569,7 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
         double s = 0.0;
 #pragma unroll
         for (int q = 0; q < 6; ++q) s += Adk[k][6 * q + r] * (double)dly[6 * k + q];
-        bout.d[gd.d_off + tid] = (T)s;
+        bout.d[d_off + tid] = (T)s;
     }
     cost = wave_sum(cost);
     if (lane == 0) red[wv] = cost;

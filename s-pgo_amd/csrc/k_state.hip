@@ -25,8 +25,8 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
     const int N = pd.N;
     const double* dlt = delta + pd.vec_off;
     const int cur = ctl->cur;
-    const StateBuf<T>& bin = cur ? B1 : B0;
-    const StateBuf<T>& bout = init ? bin : (cur ? B0 : B1);
+    const StateBuf<T> bin = cur ? B1 : B0;
+    const StateBuf<T> bout = init ? bin : (cur ? B0 : B1);
     const bool ok = init ? false : (ctl->chol_ok != 0);
 
     const int chunk = (N + WG - 1) / WG;

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0,
     const int row = row_g - 6 * pd.pose_base;    // row inside its problem
     const int cbase = 6 * pd.pose_base;
     if (row == 0 && tid == 0) ctl->chol_ok = 1;   // reset for this iteration's factorisation
-    const StateBuf<T>& bs = ctl->cur ? B1 : B0;
+    const StateBuf<T> bs = ctl->cur ? B1 : B0;
     const int e0 = lay.fs_begin[pg];
     const int nc = min(lay.fs_begin[pg + 1] - e0, FS_MAXG);
     for (int e = tid; e < nc; e += 256) {
