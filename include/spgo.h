@@ -244,6 +244,46 @@ int spgo_batch_autotune(spgo_batch* b, int32_t iterations, int32_t* chunk_landma
 /* Message of the last failed call on b (b == NULL: last failed spgo_batch_create). */
 const char* spgo_batch_last_error(spgo_batch* b);
 
+/* ---- learned preconditioner for CG on the reduced system ---------------------
+ * No reference counterpart: the reference factors the reduced system with a
+ * dense Cholesky (linear_solvers.cu:195-218).  A message-passing network over
+ * the pose covisibility band (three layers, hidden width 64, FP32) predicts a
+ * factored approximate inverse P = G^T G of a block-banded reduced system H,
+ *   G_ii = (I + tril(A_i)) C_i,  G_{i,i-1} = B_i C_{i-1},
+ * C_i the inverse Cholesky factor of H_ii; an untrained network (head at zero)
+ * gives block Jacobi.  Systems are block bands: Hband[i][d] = H_{i,i-d}
+ * (6x6 row-major) for d = 0..band_blocks (1..16), nodes of all systems
+ * concatenated; g is T x 6.  Training minimises the mean over systems of
+ * Kaporin's condition measure of P^1/2 H P^1/2 (relative to block Jacobi),
+ * log(tr(G H G^T) / 6n) - log det(G^T G) / 6n, with Adam; forward, backward,
+ * loss and PCG run in this library's kernels. */
+typedef struct spgo_lpcg spgo_lpcg;
+int spgo_lpcg_create(int32_t band_blocks, uint64_t seed, spgo_lpcg** out);
+int spgo_lpcg_destroy(spgo_lpcg* h);
+int spgo_lpcg_n_params(spgo_lpcg* h, int32_t* n);
+/* Parameters, FP32 flat: W_in (64 x 40), b_in, per layer W_s, W_m (64 x 64), b,
+ * then W_o (57 x 64), W_k (57 x 40), b_o.  Setting them resets Adam. */
+int spgo_lpcg_get_params(spgo_lpcg* h, float* out);
+int spgo_lpcg_set_params(spgo_lpcg* h, const float* in);
+int spgo_lpcg_load_systems(spgo_lpcg* h, int32_t n_systems, const int32_t* n_nodes, const double* Hband,
+                           const double* g);
+/* Mean loss and its gradient (FP32, flat like the parameters) over the loaded
+ * systems; block Jacobi scores 0. */
+int spgo_lpcg_loss_grad(spgo_lpcg* h, double* loss, float* grad);
+/* `steps` Adam steps (beta 0.9 / 0.999) over all loaded systems; losses
+ * (optional, steps entries) receives the loss before each step. */
+int spgo_lpcg_train(spgo_lpcg* h, int32_t steps, float lr, float* losses);
+/* PCG on every loaded system from x = 0 to ||r|| <= tol ||g||: learned != 0
+ * uses the network's preconditioner, 0 block Jacobi.  iters, relres (per
+ * system) and x (T x 6) are optional outputs. */
+int spgo_lpcg_solve(spgo_lpcg* h, int32_t learned, double tol, int32_t max_iter, int32_t* iters, double* relres,
+                    double* x);
+/* Device ms of the last solve: [0] network + preconditioner blocks, [1] PCG. */
+int spgo_lpcg_last_ms(spgo_lpcg* h, double ms[2]);
+/* Node inputs (T x 40, FP32) and inverse Cholesky factors C_i (T x 36). */
+int spgo_lpcg_get_features(spgo_lpcg* h, float* X, double* C);
+const char* spgo_lpcg_last_error(spgo_lpcg* h);
+
 #ifdef __cplusplus
 }
 #endif

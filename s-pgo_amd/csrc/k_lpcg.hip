@@ -1,0 +1,589 @@
+// Learned preconditioner for CG on the reduced pose system.
+//
+// No reference counterpart: the reference factors the reduced system with
+// torch's Cholesky (linear_solvers.cu:195-218).  Here a small message-passing
+// network over the pose covisibility band predicts a factored approximate
+// inverse P = G^T G of the block-banded reduced system H:
+//   G_ii = (I + tril(out_i[0:21])) C_i,   G_{i,i-1} = out_i[21:57] C_{i-1},
+// C_i the inverse of the lower Cholesky factor of H_ii.  With the network's
+// head at zero P is block Jacobi.  Systems are stored as block bands,
+// Hb[i][d] = H_{i,i-d} (6x6 row-major, d = 0..W), nodes of all systems
+// concatenated.
+//
+// Kernels: features (C_i, normalised neighbour blocks and their norms), the
+// network (embedding, three message-passing layers, head) forward and
+// backward in FP32, the training loss in FP64 -- Kaporin's condition measure
+// of P^1/2 H P^1/2 without its constant,
+//   L = log(sum_i t_i / 6n) - (2 / 6n) sum_i sum_k log |(A_i)_kk|,
+//   t_i = tr(A_i A_i^T) + 2 tr(A_i N_i B_i^T) + tr(B_i B_i^T),
+// with A_i = I + tril(out_i[0:21]), B_i = out_i[21:57], N_i = C_i H_{i,i-1}
+// C_{i-1}^T -- Adam, and PCG (one workgroup per system, FP64).
+#include "spgo_device.h"
+#include "spgo_lpcg.h"
+
+namespace spgo {
+
+namespace {
+
+__device__ __forceinline__ bool chol6(const double* A, double* L) {
+    // lower Cholesky factor of the SPD 6x6 A (row-major); false if not positive
+    for (int i = 0; i < 36; ++i) L[i] = 0.0;
+    for (int j = 0; j < 6; ++j) {
+        double d = A[j * 6 + j];
+        for (int k = 0; k < j; ++k) d -= L[j * 6 + k] * L[j * 6 + k];
+        if (!(d > 0.0)) return false;
+        const double ljj = sqrt(d);
+        L[j * 6 + j] = ljj;
+        for (int i = j + 1; i < 6; ++i) {
+            double s = A[i * 6 + j];
+            for (int k = 0; k < j; ++k) s -= L[i * 6 + k] * L[j * 6 + k];
+            L[i * 6 + j] = s / ljj;
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ void inv_lower6(const double* L, double* C) {
+    // C = L^-1 (lower triangular)
+    for (int i = 0; i < 36; ++i) C[i] = 0.0;
+    for (int j = 0; j < 6; ++j) {
+        C[j * 6 + j] = 1.0 / L[j * 6 + j];
+        for (int i = j + 1; i < 6; ++i) {
+            double s = 0.0;
+            for (int k = j; k < i; ++k) s += L[i * 6 + k] * C[k * 6 + j];
+            C[i * 6 + j] = -s / L[i * 6 + i];
+        }
+    }
+}
+
+__device__ __forceinline__ int sys_lo(const int* sys_begin, const int* node_sys, int i) {
+    return sys_begin[node_sys[i]];
+}
+__device__ __forceinline__ int sys_hi(const int* sys_begin, const int* node_sys, int i) {
+    return sys_begin[node_sys[i] + 1];
+}
+
+}  // namespace
+
+// ---- features -------------------------------------------------------------
+__global__ void k_lp_chol(int T, int W, const double* Hb, double* C, int* bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T) return;
+    double A[36], L[36], Ci[36];
+    const double* h = Hb + (long long)i * (W + 1) * 36;
+    for (int k = 0; k < 36; ++k) A[k] = h[k];
+    if (!chol6(A, L)) {
+        atomicOr(bad, 1);
+        for (int k = 0; k < 36; ++k) C[(long long)i * 36 + k] = (k % 7 == 0) ? 1.0 : 0.0;
+        return;
+    }
+    inv_lower6(L, Ci);
+    for (int k = 0; k < 36; ++k) C[(long long)i * 36 + k] = Ci[k];
+}
+
+// thread per (node, d), d = 1..W: s[i][d] = ||C_i H_{i,i-d} C_{i-d}^T||_F and,
+// for d = 1, the normalised block itself into the node input
+__global__ void k_lp_feat1(int T, int W, const double* Hb, const double* C, const int* node_sys,
+                           const int* sys_begin, double* s, float* X, double* N1) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T * W) return;
+    const int i = t / W, d = t - i * W + 1;
+    const int lo = sys_lo(sys_begin, node_sys, i);
+    double nrm = 0.0;
+    double Nb[36];
+    if (i - d >= lo) {
+        const double* Hd = Hb + ((long long)i * (W + 1) + d) * 36;
+        const double* Ci = C + (long long)i * 36;
+        const double* Cj = C + (long long)(i - d) * 36;
+        double Tm[36];
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) {
+                double a = 0.0;
+                for (int k = 0; k <= r; ++k) a += Ci[r * 6 + k] * Hd[k * 6 + c];
+                Tm[r * 6 + c] = a;
+            }
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) {
+                double a = 0.0;
+                for (int k = 0; k <= c; ++k) a += Tm[r * 6 + k] * Cj[c * 6 + k];
+                Nb[r * 6 + c] = a;
+                nrm += a * a;
+            }
+    } else {
+        for (int k = 0; k < 36; ++k) Nb[k] = 0.0;
+    }
+    s[(long long)i * (W + 1) + d] = sqrt(nrm);
+    if (d == 1)
+        for (int k = 0; k < 36; ++k) {
+            X[(long long)i * LP_NIN + k] = (float)Nb[k];
+            N1[(long long)i * 36 + k] = Nb[k];
+        }
+}
+
+__global__ void k_lp_feat2(int T, int W, const double* s, const int* node_sys, const int* sys_begin, float* X,
+                           double* wsum) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T) return;
+    const int lo = sys_lo(sys_begin, node_sys, i), hi = sys_hi(sys_begin, node_sys, i);
+    double tot = 0.0;
+    for (int d = 1; d <= W; ++d) {
+        if (i - d >= lo) tot += s[(long long)i * (W + 1) + d];
+        if (i + d < hi) tot += s[(long long)(i + d) * (W + 1) + d];
+    }
+    float* x = X + (long long)i * LP_NIN;
+    x[36] = W >= 1 ? (float)s[(long long)i * (W + 1) + 1] : 0.f;
+    x[37] = (W >= 1 && i + 1 < hi) ? (float)s[(long long)(i + 1) * (W + 1) + 1] : 0.f;
+    x[38] = (float)(tot / (W > 0 ? W : 1));
+    x[39] = 1.f;
+    wsum[i] = 1.0 + tot;
+}
+
+// ---- network --------------------------------------------------------------
+// z[n][o] = b[o] + W1[o][:] in1[n][:] + W2[o][:] in2[n][:] for o < O (O <= 64);
+// h = relu(z) when h != nullptr.  A block stages W1 and W2 in LDS (padded rows)
+// and handles LP_LIN_NODES nodes, a thread one output of four nodes at a time.
+constexpr int LP_LIN_NODES = 32;
+__global__ __launch_bounds__(256) void k_lp_linear(int T, int O, int I1, const float* in1, const float* W1, int I2,
+                                                   const float* in2, const float* W2, const float* b, float* z,
+                                                   float* h) {
+    extern __shared__ float wl[];   // O x (I1 + 1), then O x (I2 + 1)
+    float* w2l = wl + O * (I1 + 1);
+    const int tid = threadIdx.x;
+    for (int k = tid; k < O * I1; k += 256) wl[(k / I1) * (I1 + 1) + k % I1] = W1[k];
+    for (int k = tid; k < O * I2; k += 256) w2l[(k / I2) * (I2 + 1) + k % I2] = W2[k];
+    __syncthreads();
+    const int o = tid & 63, q = tid >> 6;
+    if (o >= O) return;
+    const int n0 = blockIdx.x * LP_LIN_NODES;
+    for (int nn = q; nn < LP_LIN_NODES; nn += 4) {
+        const int n = n0 + nn;
+        if (n >= T) break;
+        float a = b[o];
+        const float* x1 = in1 + (long long)n * I1;
+        for (int k = 0; k < I1; ++k) a += wl[o * (I1 + 1) + k] * x1[k];
+        if (I2 > 0) {
+            const float* x2 = in2 + (long long)n * I2;
+            for (int k = 0; k < I2; ++k) a += w2l[o * (I2 + 1) + k] * x2[k];
+        }
+        z[(long long)n * O + o] = a;
+        if (h) h[(long long)n * O + o] = a > 0.f ? a : 0.f;
+    }
+}
+
+// dx[n][k] (=|+=) sum_o W[o][k] dy[n][o], k < I (I <= 64); W is O x I
+__global__ __launch_bounds__(256) void k_lp_linearT(int T, int O, int I, const float* dy, const float* Wt, float* dx,
+                                                    int accumulate) {
+    extern __shared__ float wl[];   // I x (O + 1): transposed
+    const int tid = threadIdx.x;
+    for (int k = tid; k < O * I; k += 256) {
+        const int o = k / I, c = k % I;
+        wl[c * (O + 1) + o] = Wt[k];
+    }
+    __syncthreads();
+    const int c = tid & 63, q = tid >> 6;
+    if (c >= I) return;
+    const int n0 = blockIdx.x * LP_LIN_NODES;
+    for (int nn = q; nn < LP_LIN_NODES; nn += 4) {
+        const int n = n0 + nn;
+        if (n >= T) break;
+        float a = 0.f;
+        const float* g = dy + (long long)n * O;
+        for (int o = 0; o < O; ++o) a += wl[c * (O + 1) + o] * g[o];
+        if (accumulate) dx[(long long)n * I + c] += a;
+        else dx[(long long)n * I + c] = a;
+    }
+}
+
+// m_i = sum_j s_ij h_j / wsum_i over the band neighbours j of i (same system)
+__global__ void k_lp_agg(int T, int W, const double* s, const double* wsum, const int* node_sys,
+                         const int* sys_begin, const float* hin, float* m) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T * LP_HID) return;
+    const int i = t / LP_HID, k = t - i * LP_HID;
+    const int lo = sys_lo(sys_begin, node_sys, i), hi = sys_hi(sys_begin, node_sys, i);
+    double a = 0.0;
+    for (int d = 1; d <= W; ++d) {
+        if (i - d >= lo) a += s[(long long)i * (W + 1) + d] * hin[(long long)(i - d) * LP_HID + k];
+        if (i + d < hi) a += s[(long long)(i + d) * (W + 1) + d] * hin[(long long)(i + d) * LP_HID + k];
+    }
+    m[t] = (float)(a / wsum[i]);
+}
+
+// transpose of k_lp_agg, added: dh_j += sum_i s_ij / wsum_i dm_i
+__global__ void k_lp_aggT(int T, int W, const double* s, const double* wsum, const int* node_sys,
+                          const int* sys_begin, const float* dm, float* dh) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T * LP_HID) return;
+    const int j = t / LP_HID, k = t - j * LP_HID;
+    const int lo = sys_lo(sys_begin, node_sys, j), hi = sys_hi(sys_begin, node_sys, j);
+    double a = 0.0;
+    for (int d = 1; d <= W; ++d) {
+        if (j + d < hi) a += s[(long long)(j + d) * (W + 1) + d] * dm[(long long)(j + d) * LP_HID + k] / wsum[j + d];
+        if (j - d >= lo) a += s[(long long)j * (W + 1) + d] * dm[(long long)(j - d) * LP_HID + k] / wsum[j - d];
+    }
+    dh[t] += (float)a;
+}
+
+__global__ void k_lp_reluback(int n, const float* dh, const float* z, float* dz) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dz[t] = z[t] > 0.f ? dh[t] : 0.f;
+}
+
+// gW[o][k] = sum_n dy[n][o] x[n][k]; gb[o] = sum_n dy[n][o] (thread per weight)
+__global__ void k_lp_wgrad(int T, int O, int I, const float* dy, const float* x, float* gW, float* gb) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < O * I) {
+        const int o = t / I, k = t - o * I;
+        float a0 = 0.f, a1 = 0.f;
+        int n = 0;
+        for (; n + 1 < T; n += 2) {
+            a0 += dy[(long long)n * O + o] * x[(long long)n * I + k];
+            a1 += dy[(long long)(n + 1) * O + o] * x[(long long)(n + 1) * I + k];
+        }
+        if (n < T) a0 += dy[(long long)n * O + o] * x[(long long)n * I + k];
+        gW[t] = a0 + a1;
+    } else if (gb && t < O * I + O) {
+        const int o = t - O * I;
+        float a = 0.f;
+        for (int n = 0; n < T; ++n) a += dy[(long long)n * O + o];
+        gb[o] = a;
+    }
+}
+
+// ---- preconditioner blocks and the loss -------------------------------------
+__device__ __forceinline__ void tril_index(int k, int& r, int& c) {
+    r = 0;
+    while ((r + 1) * (r + 2) / 2 <= k) ++r;
+    c = k - r * (r + 1) / 2;
+}
+
+// G_ii = (I + tril(out[0:21])) C_i, G_{i,i-1} = out[21:57] C_{i-1}; out == nullptr: block Jacobi
+__global__ void k_lp_buildG(int T, const float* out, const double* C, const int* node_sys, const int* sys_begin,
+                            double* Gd, double* Go) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T) return;
+    const int lo = sys_lo(sys_begin, node_sys, i);
+    double A[36];
+    for (int k = 0; k < 36; ++k) A[k] = (k % 7 == 0) ? 1.0 : 0.0;
+    if (out) {
+        for (int k = 0; k < 21; ++k) {
+            int r, c;
+            tril_index(k, r, c);
+            A[r * 6 + c] += (double)out[(long long)i * LP_NOUT + k];
+        }
+    }
+    const double* Ci = C + (long long)i * 36;
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) {
+            double a = 0.0;
+            for (int k = c; k <= r; ++k) a += A[r * 6 + k] * Ci[k * 6 + c];
+            Gd[(long long)i * 36 + r * 6 + c] = a;
+        }
+    if (out && i > lo) {
+        const double* Cj = C + (long long)(i - 1) * 36;
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) {
+                double a = 0.0;
+                for (int k = c; k < 6; ++k) a += (double)out[(long long)i * LP_NOUT + 21 + r * 6 + k] * Cj[k * 6 + c];
+                Go[(long long)i * 36 + r * 6 + c] = a;
+            }
+    } else {
+        for (int k = 0; k < 36; ++k) Go[(long long)i * 36 + k] = 0.0;
+    }
+}
+
+__device__ __forceinline__ void band_row(int i, int lo, int hi, int W, const double* Hb, const double* x,
+                                         double* y) {
+    for (int a = 0; a < 6; ++a) y[a] = 0.0;
+    for (int d = 0; d <= W; ++d) {
+        if (i - d >= lo) {
+            const double* B = Hb + ((long long)i * (W + 1) + d) * 36;
+            const double* xv = x + (long long)(i - d) * 6;
+            for (int a = 0; a < 6; ++a)
+                for (int b = 0; b < 6; ++b) y[a] += B[a * 6 + b] * xv[b];
+        }
+        if (d > 0 && i + d < hi) {
+            const double* B = Hb + ((long long)(i + d) * (W + 1) + d) * 36;
+            const double* xv = x + (long long)(i + d) * 6;
+            for (int a = 0; a < 6; ++a)
+                for (int b = 0; b < 6; ++b) y[a] += B[b * 6 + a] * xv[b];
+        }
+    }
+}
+
+// per node: t_i and sum_k log |(A_i)_kk|
+__device__ __forceinline__ void lp_AB(const float* out, int i, bool has_b, double* A, double* B) {
+    for (int k = 0; k < 36; ++k) A[k] = (k % 7 == 0) ? 1.0 : 0.0;
+    for (int k = 0; k < 21; ++k) {
+        int r, c;
+        tril_index(k, r, c);
+        A[r * 6 + c] += (double)out[(long long)i * LP_NOUT + k];
+    }
+    for (int k = 0; k < 36; ++k) B[k] = has_b ? (double)out[(long long)i * LP_NOUT + 21 + k] : 0.0;
+}
+
+__global__ void k_lp_kap_node(int T, const float* out, const double* N1, const int* node_sys, const int* sys_begin,
+                              double* tnode, double* ldnode) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T) return;
+    double A[36], B[36];
+    lp_AB(out, i, i > sys_lo(sys_begin, node_sys, i), A, B);
+    const double* N = N1 + (long long)i * 36;
+    double t = 0.0, ld = 0.0;
+    for (int k = 0; k < 36; ++k) t += A[k] * A[k] + B[k] * B[k];
+    // 2 tr(A N B^T) = 2 sum_rc (A N)_rc B_rc
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) {
+            double an = 0.0;
+            for (int k = 0; k <= r; ++k) an += A[r * 6 + k] * N[k * 6 + c];
+            t += 2.0 * an * B[r * 6 + c];
+        }
+    for (int k = 0; k < 6; ++k) ld += log(fabs(A[k * 7]));
+    tnode[i] = t;
+    ldnode[i] = ld;
+}
+
+// per system (one workgroup): Tsys = sum t_i, loss = log(Tsys / 6n) - 2 sum ld_i / 6n
+__global__ __launch_bounds__(256) void k_lp_kap_sys(const int* sys_begin, const double* tnode, const double* ldnode,
+                                                    double* Tsys, double* loss) {
+    __shared__ double r0[256], r1[256];
+    const int sidx = blockIdx.x, tid = threadIdx.x;
+    const int lo = sys_begin[sidx], hi = sys_begin[sidx + 1];
+    double a = 0.0, b = 0.0;
+    for (int i = lo + tid; i < hi; i += 256) {
+        a += tnode[i];
+        b += ldnode[i];
+    }
+    r0[tid] = a;
+    r1[tid] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            r0[tid] += r0[tid + w];
+            r1[tid] += r1[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double n6 = 6.0 * (hi - lo);
+        Tsys[sidx] = r0[0];
+        loss[sidx] = log(r0[0] / n6) - 2.0 * r1[0] / n6;
+    }
+}
+
+// dL/dout, mean over the S systems: dA = (2 A + 2 B N^T) / T_s - (2 / 6n) diag(1 / a_kk)
+// (lower triangle), dB = (2 A N + 2 B) / T_s
+__global__ void k_lp_kap_dout(int T, int S, const float* out, const double* N1, const double* Tsys,
+                              const int* node_sys, const int* sys_begin, float* dout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T) return;
+    const int sidx = node_sys[i];
+    const int lo = sys_begin[sidx], hi = sys_begin[sidx + 1];
+    const bool has_b = i > lo;
+    double A[36], B[36];
+    lp_AB(out, i, has_b, A, B);
+    const double* N = N1 + (long long)i * 36;
+    const double iT = 1.0 / Tsys[sidx], n6 = 6.0 * (hi - lo), sc = 1.0 / S;
+    for (int k = 0; k < 21; ++k) {
+        int r, c;
+        tril_index(k, r, c);
+        double bn = 0.0;   // (B N^T)_rc
+        for (int q = 0; q < 6; ++q) bn += B[r * 6 + q] * N[c * 6 + q];
+        double d = (2.0 * A[r * 6 + c] + 2.0 * bn) * iT;
+        if (r == c) d -= 2.0 / n6 / A[r * 7];
+        dout[(long long)i * LP_NOUT + k] = (float)(d * sc);
+    }
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) {
+            double d = 0.0;
+            if (has_b) {
+                double an = 0.0;
+                for (int k = 0; k <= r; ++k) an += A[r * 6 + k] * N[k * 6 + c];
+                d = (2.0 * an + 2.0 * B[r * 6 + c]) * iT;
+            }
+            dout[(long long)i * LP_NOUT + 21 + r * 6 + c] = (float)(d * sc);
+        }
+}
+
+__global__ void k_lp_adam(int n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2,
+                          float eps, float c1, float c2) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float gg = g[t];
+    const float mm = b1 * m[t] + (1.f - b1) * gg;
+    const float vv = b2 * v[t] + (1.f - b2) * gg * gg;
+    m[t] = mm;
+    v[t] = vv;
+    p[t] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+// ---- PCG: one workgroup per system ------------------------------------------
+// z = G^T G r with the blocks Gd (Go == nullptr: block diagonal); x starts at 0.
+__global__ __launch_bounds__(256) void k_lp_pcg(const int* sys_begin, int W, const double* Hb, const double* Gd,
+                                                const double* Go, const double* g, double* x, double* r, double* z,
+                                                double* p, double* q, double* tmp, double tol, int maxit,
+                                                int* iters, double* relres) {
+    __shared__ double red[256];
+    const int sidx = blockIdx.x, tid = threadIdx.x;
+    const int lo = sys_begin[sidx], hi = sys_begin[sidx + 1];
+    auto sum = [&](double v) {
+        red[tid] = v;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (tid < w) red[tid] += red[tid + w];
+            __syncthreads();
+        }
+        const double s = red[0];
+        __syncthreads();
+        return s;
+    };
+    auto prec = [&]() {
+        // tmp = G r, then z = G^T tmp
+        for (int i = lo + tid; i < hi; i += 256)
+            for (int a = 0; a < 6; ++a) {
+                double s = 0.0;
+                for (int b = 0; b < 6; ++b) s += Gd[(long long)i * 36 + a * 6 + b] * r[(long long)i * 6 + b];
+                if (Go && i > lo)
+                    for (int b = 0; b < 6; ++b) s += Go[(long long)i * 36 + a * 6 + b] * r[(long long)(i - 1) * 6 + b];
+                tmp[(long long)i * 6 + a] = s;
+            }
+        __syncthreads();
+        for (int i = lo + tid; i < hi; i += 256)
+            for (int a = 0; a < 6; ++a) {
+                double s = 0.0;
+                for (int b = 0; b < 6; ++b) s += Gd[(long long)i * 36 + b * 6 + a] * tmp[(long long)i * 6 + b];
+                if (Go && i + 1 < hi)
+                    for (int b = 0; b < 6; ++b)
+                        s += Go[(long long)(i + 1) * 36 + b * 6 + a] * tmp[(long long)(i + 1) * 6 + b];
+                z[(long long)i * 6 + a] = s;
+            }
+        __syncthreads();
+    };
+    double gg = 0.0;
+    for (long long e = 6LL * lo + tid; e < 6LL * hi; e += 256) {
+        x[e] = 0.0;
+        r[e] = g[e];
+        gg += g[e] * g[e];
+    }
+    __syncthreads();
+    const double g0 = sqrt(sum(gg));
+    prec();
+    double rz = 0.0;
+    for (long long e = 6LL * lo + tid; e < 6LL * hi; e += 256) {
+        p[e] = z[e];
+        rz += r[e] * z[e];
+    }
+    rz = sum(rz);
+    int it = 0;
+    double rn = g0;
+    while (it < maxit && rn > tol * g0 && g0 > 0.0) {
+        double pq = 0.0;
+        for (int i = lo + tid; i < hi; i += 256) {
+            double yy[6];
+            band_row(i, lo, hi, W, Hb, p, yy);
+            for (int a = 0; a < 6; ++a) {
+                q[(long long)i * 6 + a] = yy[a];
+                pq += p[(long long)i * 6 + a] * yy[a];
+            }
+        }
+        pq = sum(pq);
+        const double al = rz / pq;
+        double rr = 0.0;
+        for (long long e = 6LL * lo + tid; e < 6LL * hi; e += 256) {
+            x[e] += al * p[e];
+            r[e] -= al * q[e];
+            rr += r[e] * r[e];
+        }
+        rn = sqrt(sum(rr));
+        ++it;
+        if (!(rn > tol * g0)) break;
+        prec();
+        double rz2 = 0.0;
+        for (long long e = 6LL * lo + tid; e < 6LL * hi; e += 256) rz2 += r[e] * z[e];
+        rz2 = sum(rz2);
+        const double be = rz2 / rz;
+        rz = rz2;
+        for (long long e = 6LL * lo + tid; e < 6LL * hi; e += 256) p[e] = z[e] + be * p[e];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        iters[sidx] = it;
+        relres[sidx] = g0 > 0.0 ? rn / g0 : 0.0;
+    }
+}
+
+// ---- launchers ------------------------------------------------------------------
+namespace {
+inline unsigned nblk(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+}
+
+void lp_launch_features(const LpSet& s, double* C, double* sfeat, float* X, double* N1, double* wsum, int* bad,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_lp_chol, dim3(nblk(s.T)), dim3(256), 0, st, s.T, s.W, s.Hb, C, bad);
+    if (s.W > 0)
+        hipLaunchKernelGGL(k_lp_feat1, dim3(nblk((long long)s.T * s.W)), dim3(256), 0, st, s.T, s.W, s.Hb, C,
+                           s.node_sys, s.sys_begin, sfeat, X, N1);
+    hipLaunchKernelGGL(k_lp_feat2, dim3(nblk(s.T)), dim3(256), 0, st, s.T, s.W, sfeat, s.node_sys, s.sys_begin, X,
+                       wsum);
+}
+
+void lp_launch_linear(int T, int O, int I1, const float* in1, const float* W1, int I2, const float* in2,
+                      const float* W2, const float* b, float* z, float* h, hipStream_t st) {
+    const size_t lds = sizeof(float) * ((size_t)O * (I1 + 1) + (size_t)O * (I2 + 1));
+    hipLaunchKernelGGL(k_lp_linear, dim3(nblk(T, LP_LIN_NODES)), dim3(256), lds, st, T, O, I1, in1, W1, I2, in2, W2,
+                       b, z, h);
+}
+
+void lp_launch_linearT(int T, int O, int I, const float* dy, const float* Wt, float* dx, int acc, hipStream_t st) {
+    const size_t lds = sizeof(float) * (size_t)I * (O + 1);
+    hipLaunchKernelGGL(k_lp_linearT, dim3(nblk(T, LP_LIN_NODES)), dim3(256), lds, st, T, O, I, dy, Wt, dx, acc);
+}
+
+void lp_launch_agg(const LpSet& s, const double* sfeat, const double* wsum, const float* hin, float* m, int transpose,
+                   hipStream_t st) {
+    if (transpose)
+        hipLaunchKernelGGL(k_lp_aggT, dim3(nblk((long long)s.T * LP_HID)), dim3(256), 0, st, s.T, s.W, sfeat, wsum,
+                           s.node_sys, s.sys_begin, hin, m);
+    else
+        hipLaunchKernelGGL(k_lp_agg, dim3(nblk((long long)s.T * LP_HID)), dim3(256), 0, st, s.T, s.W, sfeat, wsum,
+                           s.node_sys, s.sys_begin, hin, m);
+}
+
+void lp_launch_reluback(int n, const float* dh, const float* z, float* dz, hipStream_t st) {
+    hipLaunchKernelGGL(k_lp_reluback, dim3(nblk(n)), dim3(256), 0, st, n, dh, z, dz);
+}
+
+void lp_launch_wgrad(int T, int O, int I, const float* dy, const float* x, float* gW, float* gb, hipStream_t st) {
+    hipLaunchKernelGGL(k_lp_wgrad, dim3(nblk((long long)O * I + O)), dim3(256), 0, st, T, O, I, dy, x, gW, gb);
+}
+
+void lp_launch_buildG(const LpSet& s, const float* out, const double* C, double* Gd, double* Go, hipStream_t st) {
+    hipLaunchKernelGGL(k_lp_buildG, dim3(nblk(s.T)), dim3(256), 0, st, s.T, out, C, s.node_sys, s.sys_begin, Gd, Go);
+}
+
+void lp_launch_kap(const LpSet& s, const float* out, const double* N1, double* tnode, double* ldnode, double* Tsys,
+                   double* loss, hipStream_t st) {
+    hipLaunchKernelGGL(k_lp_kap_node, dim3(nblk(s.T)), dim3(256), 0, st, s.T, out, N1, s.node_sys, s.sys_begin, tnode,
+                       ldnode);
+    hipLaunchKernelGGL(k_lp_kap_sys, dim3(s.S), dim3(256), 0, st, s.sys_begin, tnode, ldnode, Tsys, loss);
+}
+
+void lp_launch_kap_dout(const LpSet& s, const float* out, const double* N1, const double* Tsys, float* dout,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_lp_kap_dout, dim3(nblk(s.T)), dim3(256), 0, st, s.T, s.S, out, N1, Tsys, s.node_sys,
+                       s.sys_begin, dout);
+}
+
+void lp_launch_adam(int n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+                    int step, hipStream_t st) {
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_lp_adam, dim3(nblk(n)), dim3(256), 0, st, n, p, g, m, v, lr, b1, b2, eps, c1, c2);
+}
+
+void lp_launch_pcg(const LpSet& s, const double* Gd, const double* Go, double* x, double* r, double* z, double* p,
+                   double* q, double* tmp, double tol, int maxit, int* iters, double* relres, hipStream_t st) {
+    hipLaunchKernelGGL(k_lp_pcg, dim3(s.S), dim3(256), 0, st, s.sys_begin, s.W, s.Hb, Gd, Go, s.g, x, r, z, p, q, tmp,
+                       tol, maxit, iters, relres);
+}
+
+}  // namespace spgo

@@ -1,0 +1,54 @@
+// Learned-preconditioner PCG: sizes, the device view of a set of block-banded
+// systems, and the kernel launchers (k_lpcg.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace spgo {
+
+constexpr int LP_HID = 64;     // hidden width
+constexpr int LP_NIN = 40;     // node input: normalised (i, i-1) block, 2 norms, degree, 1
+constexpr int LP_NOUT = 57;    // 21 lower-triangle corrections of G_ii, 36 entries of G_{i,i-1}
+constexpr int LP_NLAYER = 3;   // message-passing layers
+constexpr int LP_WMAX = 16;    // widest band (blocks below the diagonal)
+
+// Parameter layout (FP32, flat): W_in (HID x NIN), b_in, then per layer W_s,
+// W_m (HID x HID) and b, then W_o (NOUT x HID), W_k (NOUT x NIN), b_o.
+constexpr int LP_OFF_WIN = 0;
+constexpr int LP_OFF_BIN = LP_OFF_WIN + LP_HID * LP_NIN;
+constexpr int LP_OFF_L0 = LP_OFF_BIN + LP_HID;
+constexpr int LP_LAYER_SIZE = 2 * LP_HID * LP_HID + LP_HID;
+constexpr int LP_OFF_WO = LP_OFF_L0 + LP_NLAYER * LP_LAYER_SIZE;
+constexpr int LP_OFF_WK = LP_OFF_WO + LP_NOUT * LP_HID;
+constexpr int LP_OFF_BO = LP_OFF_WK + LP_NOUT * LP_NIN;
+constexpr int LP_NPARAM = LP_OFF_BO + LP_NOUT;
+
+// Systems on the device: T nodes (6x6 blocks) of S systems, band W.
+struct LpSet {
+    int T, S, W;
+    const double* Hb;       // T x (W + 1) x 36: Hb[i][d] = H_{i,i-d}
+    const double* g;        // T x 6
+    const int* node_sys;    // T
+    const int* sys_begin;   // S + 1
+};
+
+void lp_launch_features(const LpSet& s, double* C, double* sfeat, float* X, double* N1, double* wsum, int* bad,
+                        hipStream_t st);
+void lp_launch_linear(int T, int O, int I1, const float* in1, const float* W1, int I2, const float* in2,
+                      const float* W2, const float* b, float* z, float* h, hipStream_t st);
+void lp_launch_linearT(int T, int O, int I, const float* dy, const float* Wt, float* dx, int acc, hipStream_t st);
+void lp_launch_agg(const LpSet& s, const double* sfeat, const double* wsum, const float* hin, float* m, int transpose,
+                   hipStream_t st);
+void lp_launch_reluback(int n, const float* dh, const float* z, float* dz, hipStream_t st);
+void lp_launch_wgrad(int T, int O, int I, const float* dy, const float* x, float* gW, float* gb, hipStream_t st);
+void lp_launch_buildG(const LpSet& s, const float* out, const double* C, double* Gd, double* Go, hipStream_t st);
+// Kaporin loss: per-node terms, per-system T and loss; gradient wrt the head output
+void lp_launch_kap(const LpSet& s, const float* out, const double* N1, double* tnode, double* ldnode, double* Tsys,
+                   double* loss, hipStream_t st);
+void lp_launch_kap_dout(const LpSet& s, const float* out, const double* N1, const double* Tsys, float* dout,
+                        hipStream_t st);
+void lp_launch_adam(int n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+                    int step, hipStream_t st);
+void lp_launch_pcg(const LpSet& s, const double* Gd, const double* Go, double* x, double* r, double* z, double* p,
+                   double* q, double* tmp, double tol, int maxit, int* iters, double* relres, hipStream_t st);
+
+}  // namespace spgo

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_uint64, c_void_p
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libspgo.so")
@@ -106,6 +106,18 @@ SIGNATURES = {
     "spgo_batch_profile_phases": (c_int32, [c_void_p, c_int32, F64P]),
     "spgo_batch_autotune": (c_int32, [c_void_p, c_int32, I32P, I32P, F64P]),
     "spgo_batch_last_error": (c_char_p, [c_void_p]),
+    "spgo_lpcg_create": (c_int32, [c_int32, c_uint64, POINTER(c_void_p)]),
+    "spgo_lpcg_destroy": (c_int32, [c_void_p]),
+    "spgo_lpcg_n_params": (c_int32, [c_void_p, I32P]),
+    "spgo_lpcg_get_params": (c_int32, [c_void_p, F32P]),
+    "spgo_lpcg_set_params": (c_int32, [c_void_p, F32P]),
+    "spgo_lpcg_load_systems": (c_int32, [c_void_p, c_int32, I32P, F64P, F64P]),
+    "spgo_lpcg_loss_grad": (c_int32, [c_void_p, F64P, F32P]),
+    "spgo_lpcg_train": (c_int32, [c_void_p, c_int32, c_float, F32P]),
+    "spgo_lpcg_solve": (c_int32, [c_void_p, c_int32, c_double, c_int32, I32P, F64P, F64P]),
+    "spgo_lpcg_last_ms": (c_int32, [c_void_p, F64P]),
+    "spgo_lpcg_get_features": (c_int32, [c_void_p, F32P, F64P]),
+    "spgo_lpcg_last_error": (c_char_p, [c_void_p]),
 }
 
 _lib = None
@@ -127,10 +139,15 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     return lib
 
 
-def check(rc: int, handle=None, batch: bool = False) -> None:
+def check(rc: int, handle=None, batch: bool = False, kind: str = "") -> None:
     if rc != 0:
         lib = load()
-        msg = (lib.spgo_batch_last_error if batch else lib.spgo_last_error)(handle)
+        fn = lib.spgo_last_error
+        if batch or kind == "batch":
+            fn = lib.spgo_batch_last_error
+        elif kind == "lpcg":
+            fn = lib.spgo_lpcg_last_error
+        msg = fn(handle)
         msg = msg.decode() if msg else ""
         raise RuntimeError(f"spgo error {rc} ({ERRORS.get(rc, '?')}): {msg}")
 
