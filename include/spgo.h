@@ -51,11 +51,16 @@ extern "C" {
  * the profile into a top and a bottom part factored concurrently; DENSE: the
  * multi-kernel blocked Cholesky.  AUTO picks ENVELOPE_SPLIT for long narrow
  * profiles (at least 2 D + 6 blocks of 16 rows, D the profile depth),
- * ENVELOPE for short narrow ones and DENSE for wide ones. */
+ * ENVELOPE for short narrow ones and DENSE for wide ones.  PCG: conjugate
+ * gradients preconditioned by the network set with spgo_set_preconditioner
+ * (block Jacobi without one) to pcg_tol; when CG does not converge in
+ * pcg_max_iter iterations the envelope Cholesky (as AUTO would pick it)
+ * computes the step instead. */
 #define SPGO_FACTOR_AUTO 0
 #define SPGO_FACTOR_DENSE 1
 #define SPGO_FACTOR_ENVELOPE 2
 #define SPGO_FACTOR_ENVELOPE_SPLIT 3
+#define SPGO_FACTOR_PCG 4
 
 typedef struct spgo_options {
     int32_t mode;            /* SPGO_MODE_*                                    */
@@ -72,7 +77,9 @@ typedef struct spgo_options {
     int32_t chunk_landmarks; /* tile shape: landmarks per LDS chunk (<= 64)    */
     int32_t group_chunks;    /* tile shape: chunks per workgroup               */
     int32_t factorization;   /* SPGO_FACTOR_*                                   */
-    int32_t reserved[7];
+    int32_t pcg_max_iter;    /* CG iterations before the Cholesky fallback (PCG) */
+    int32_t reserved[6];
+    double pcg_tol;          /* CG stops at ||r|| <= pcg_tol ||g|| (PCG)         */
 } spgo_options;
 
 typedef struct spgo_iter_stats {
@@ -81,12 +88,14 @@ typedef struct spgo_iter_stats {
     double lambda;           /* damping used by this iteration's solve         */
     int32_t accepted;
     int32_t chol_ok;         /* 0 if the reduced system was not positive definite */
+    int32_t pcg_iters;       /* CG iterations of the step (SPGO_FACTOR_PCG)      */
+    int32_t pcg_fallback;    /* 1: the Cholesky computed the step               */
 } spgo_iter_stats;
 
 typedef struct spgo_layout_info {
     int32_t n_poses, n_slots, n_measurements, n_landmarks;
     int32_t n_groups, n_chunks, max_span, n_anchors_used;
-    int32_t factorization;   /* SPGO_FACTOR_DENSE, _ENVELOPE or _ENVELOPE_SPLIT: the path spgo_step takes */
+    int32_t factorization;   /* SPGO_FACTOR_DENSE, _ENVELOPE, _ENVELOPE_SPLIT or _PCG: the path spgo_step takes */
     int32_t profile_depth;   /* blocks of 16 below the diagonal in the reduced system's profile */
     int32_t reduced_size;    /* rows of the padded reduced pose system                          */
     int32_t reserved;
@@ -187,6 +196,14 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
 /* Message of the last failed call on h (h == NULL: last failed spgo_create). */
 const char* spgo_last_error(spgo_solver* h);
 
+struct spgo_lpcg;
+/* The learned preconditioner of SPGO_FACTOR_PCG (NULL: block Jacobi).  The
+ * solver reads the network's parameters at every step, so training it later
+ * takes effect; it must outlive its use.  Its band (spgo_lpcg_create) must
+ * cover the problem's: landmark spans of K poses need K - 1 blocks.  No
+ * reference counterpart. */
+int spgo_set_preconditioner(spgo_solver* h, struct spgo_lpcg* net);
+
 /* ---- batches of independent problems ----------------------------------------
  * No reference counterpart: the reference steps one Solver per problem
  * (include/Solver.hpp:18-85, one launch sequence per step in
@@ -257,7 +274,7 @@ const char* spgo_batch_last_error(spgo_batch* b);
  * Kaporin's condition measure of P^1/2 H P^1/2 (relative to block Jacobi),
  * log(tr(G H G^T) / 6n) - log det(G^T G) / 6n, with Adam; forward, backward,
  * loss and PCG run in this library's kernels. */
-typedef struct spgo_lpcg spgo_lpcg;
+typedef struct spgo_lpcg spgo_lpcg;   /* declared above with spgo_set_preconditioner */
 int spgo_lpcg_create(int32_t band_blocks, uint64_t seed, spgo_lpcg** out);
 int spgo_lpcg_destroy(spgo_lpcg* h);
 int spgo_lpcg_n_params(spgo_lpcg* h, int32_t* n);

@@ -19,6 +19,7 @@
 // with A_i = I + tril(out_i[0:21]), B_i = out_i[21:57], N_i = C_i H_{i,i-1}
 // C_{i-1}^T -- Adam, and PCG (one workgroup per system, FP64).
 #include "spgo_device.h"
+#include "spgo_internal.h"
 #include "spgo_lpcg.h"
 
 namespace spgo {
@@ -422,7 +423,7 @@ __global__ void k_lp_adam(int n, float* p, const float* g, float* m, float* v, f
 __global__ __launch_bounds__(256) void k_lp_pcg(const int* sys_begin, int W, const double* Hb, const double* Gd,
                                                 const double* Go, const double* g, double* x, double* r, double* z,
                                                 double* p, double* q, double* tmp, double tol, int maxit,
-                                                int* iters, double* relres) {
+                                                int* iters, double* relres, Control* ctl, const int* bad) {
     __shared__ double red[256];
     const int sidx = blockIdx.x, tid = threadIdx.x;
     const int lo = sys_begin[sidx], hi = sys_begin[sidx + 1];
@@ -476,7 +477,8 @@ __global__ __launch_bounds__(256) void k_lp_pcg(const int* sys_begin, int W, con
     rz = sum(rz);
     int it = 0;
     double rn = g0;
-    while (it < maxit && rn > tol * g0 && g0 > 0.0) {
+    bool broke = bad && *bad;   // a diagonal block that is not positive definite: no CG
+    while (!broke && it < maxit && rn > tol * g0 && g0 > 0.0) {
         double pq = 0.0;
         for (int i = lo + tid; i < hi; i += 256) {
             double yy[6];
@@ -487,6 +489,10 @@ __global__ __launch_bounds__(256) void k_lp_pcg(const int* sys_begin, int W, con
             }
         }
         pq = sum(pq);
+        if (!(pq > 0.0) || !(rz > 0.0)) {   // not positive definite (or not finite)
+            broke = true;
+            break;
+        }
         const double al = rz / pq;
         double rr = 0.0;
         for (long long e = 6LL * lo + tid; e < 6LL * hi; e += 256) {
@@ -507,10 +513,39 @@ __global__ __launch_bounds__(256) void k_lp_pcg(const int* sys_begin, int W, con
         __syncthreads();
     }
     if (tid == 0) {
-        iters[sidx] = it;
-        relres[sidx] = g0 > 0.0 ? rn / g0 : 0.0;
+        const bool ok = !broke && (rn <= tol * g0 || g0 == 0.0);
+        if (iters) iters[sidx] = it;
+        if (relres) relres[sidx] = broke ? -1.0 : (g0 > 0.0 ? rn / g0 : 0.0);
+        if (ctl) {
+            ctl[sidx].pcg_iters = it;
+            ctl[sidx].pcg_fail = ok ? 0 : 1;
+        }
     }
 }
+
+// Hb[i][d] = H_{i,i-d} from the lower triangle of a dense row-major Pp x Pp H
+// (the diagonal block symmetrised); zero where i - d < 0
+__global__ void k_lp_band_from_dense(int N, int W, const double* H, int Pp, double* Hb) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)N * (W + 1) * 36) return;
+    const int e = (int)(t % 36);
+    const long long bd = t / 36;
+    const int d = (int)(bd % (W + 1)), i = (int)(bd / (W + 1));
+    const int r = e / 6, c = e - r * 6;
+    double v = 0.0;
+    if (i - d >= 0) {
+        int row = 6 * i + r, col = 6 * (i - d) + c;
+        if (col > row) {
+            const int tmp = row;
+            row = col;
+            col = tmp;
+        }
+        v = H[(long long)row * Pp + col];
+    }
+    Hb[t] = v;
+}
+
+__global__ void k_lp_zero_flag(int* p) { *p = 0; }
 
 // ---- launchers ------------------------------------------------------------------
 namespace {
@@ -581,9 +616,17 @@ void lp_launch_adam(int n, float* p, const float* g, float* m, float* v, float l
 }
 
 void lp_launch_pcg(const LpSet& s, const double* Gd, const double* Go, double* x, double* r, double* z, double* p,
-                   double* q, double* tmp, double tol, int maxit, int* iters, double* relres, hipStream_t st) {
+                   double* q, double* tmp, double tol, int maxit, int* iters, double* relres, hipStream_t st,
+                   Control* ctl, const int* bad) {
     hipLaunchKernelGGL(k_lp_pcg, dim3(s.S), dim3(256), 0, st, s.sys_begin, s.W, s.Hb, Gd, Go, s.g, x, r, z, p, q, tmp,
-                       tol, maxit, iters, relres);
+                       tol, maxit, iters, relres, ctl, bad);
+}
+
+void lp_launch_zero_flag(int* p, hipStream_t st) { hipLaunchKernelGGL(k_lp_zero_flag, dim3(1), dim3(1), 0, st, p); }
+
+void lp_launch_band_from_dense(int N, int W, const double* H, int Pp, double* Hb, hipStream_t st) {
+    hipLaunchKernelGGL(k_lp_band_from_dense, dim3(nblk((long long)N * (W + 1) * 36)), dim3(256), 0, st, N, W, H, Pp,
+                       Hb);
 }
 
 }  // namespace spgo

@@ -171,6 +171,8 @@ __global__ __launch_bounds__(WG) void k_accept(const double* c0, const double* c
     r.lambda = lam_used;
     r.accepted = accept;
     r.chol_ok = ok;
+    r.pcg_iters = ctl->pcg_iters;
+    r.pcg_fail = ctl->pcg_fail;
     rec[ctl->iter] = r;
     ctl->iter = ctl->iter + 1;
 }

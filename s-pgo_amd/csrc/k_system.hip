@@ -46,7 +46,13 @@ __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0,
     Control* ctl = ctls + lay.pose_prob[pg];
     const int row = row_g - 6 * pd.pose_base;    // row inside its problem
     const int cbase = 6 * pd.pose_base;
-    if (row == 0 && tid == 0) ctl->chol_ok = 1;   // reset for this iteration's factorisation
+    if (row == 0 && tid == 0) {
+        // reset for this iteration's factorisation: without a converged CG solve
+        // the Cholesky computes the step
+        ctl->chol_ok = 1;
+        ctl->pcg_fail = 1;
+        ctl->pcg_iters = 0;
+    }
     const StateBuf<T> bs = ctl->cur ? B1 : B0;
     const int e0 = lay.fs_begin[pg];
     const int nc = min(lay.fs_begin[pg + 1] - e0, FS_MAXG);
@@ -695,6 +701,7 @@ __global__ __launch_bounds__(256) void k_chol_env(const ProbDesc* probs, double*
     double* winv = winvb + pd.winv_off;
     double* Lp = Lpb + pd.Lp_off;
     Control* ctl = ctls + blockIdx.x;
+    if (ctl->pcg_fail == 0) return;   // the CG solve of this iteration converged
     const int nb = Pp / EB;
     double* end;
     EnvCtx c = env_ctx(lds, D, nb, &end);
@@ -724,6 +731,7 @@ __global__ __launch_bounds__(256) void k_chol_split1(double* H, int Pp, int D, i
     __shared__ int sbad;
     const int tid = threadIdx.x;
     const int half = blockIdx.x;
+    if (ctl->pcg_fail == 0) return;   // the CG solve of this iteration converged
     const int nb = Pp / EB, nS = s1 - s0;
     const int nloc = half ? nb - s0 : s1;
     double* end;
@@ -767,6 +775,7 @@ __global__ __launch_bounds__(256) void k_chol_split2(int Pp, int D, int s0, int 
     extern __shared__ double lds[];
     __shared__ int sbad;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    if (ctl->pcg_fail == 0) return;   // the CG solve of this iteration converged
     const int nb = Pp / EB, nS = s1 - s0, PS = EB * nS;
     const int nT = s1, nB = nb - s0;   // local systems of the two parts, separator included
     double* end;

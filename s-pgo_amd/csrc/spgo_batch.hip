@@ -344,6 +344,7 @@ int setup(spgo_batch* b) {
     for (int p = 0; p < np; ++p) {
         c[p] = Control{};
         c[p].chol_ok = 1;
+        c[p].pcg_fail = 1;
         c[p].lambda = b->opt.mode == SPGO_MODE_REFERENCE ? 0.0 : b->opt.lambda0;
     }
     rc = bcheck(b, hipMemcpy(b->ctl, c.data(), np * sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");

@@ -10,6 +10,7 @@
 
 #include "../../include/spgo.h"
 #include "spgo_internal.h"
+#include "spgo_lpcg.h"
 
 namespace spgo {
 
@@ -103,6 +104,10 @@ struct spgo_solver {
     const int* firstB = nullptr;   // split: profile of the reversed bottom part
     double *Ssep = nullptr, *Smrg = nullptr, *winvS = nullptr, *LpS = nullptr;   // split: separator scratch
     double* Lpan = nullptr;     // envelope: factored panels for the backward solve
+    bool pcg = false;           // SPGO_FACTOR_PCG: CG first, the envelope path above as fallback
+    spgo_lpcg* net = nullptr;   // its preconditioner network (nullptr: block Jacobi)
+    const float* net_params = nullptr;   // the network's device parameters for this step
+    spgo::LpSolve lps;          // CG buffers
     Control* ctl = nullptr;
     IterRecord* rec = nullptr;
     int rec_cap = 0;

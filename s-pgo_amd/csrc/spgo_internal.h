@@ -47,15 +47,18 @@ struct Control {
     int cur;                // index (0/1) of the accepted state buffers
     int iter;               // records written in this step
     int chol_ok;            // set by the factorisation of this iteration
-    int pad;
+    int pcg_fail;           // 0: this iteration's PCG converged and the Cholesky is skipped
     double lambda;
     double cost;            // cost of the current state
     double cost0;           // cost at the start of the step
+    int pcg_iters;          // CG iterations of this iteration's step (0 without CG)
+    int pad;
 };
 
 struct IterRecord {
     double cost_candidate, cost, lambda;
     int accepted, chol_ok;
+    int pcg_iters, pcg_fail;   // CG iterations of the step; 1: the Cholesky computed it
 };
 
 struct Camera {

@@ -3,6 +3,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
+#include "spgo_internal.h"
+
+struct spgo_lpcg;
+
 namespace spgo {
 
 constexpr int LP_HID = 64;     // hidden width
@@ -48,7 +54,47 @@ void lp_launch_kap_dout(const LpSet& s, const float* out, const double* N1, cons
                         hipStream_t st);
 void lp_launch_adam(int n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
                     int step, hipStream_t st);
+// PCG from x = 0 on every system; with ctl, ctl[s].pcg_iters / pcg_fail are set;
+// with bad (device flag of a failed diagonal factor), nothing is solved when set
 void lp_launch_pcg(const LpSet& s, const double* Gd, const double* Go, double* x, double* r, double* z, double* p,
-                   double* q, double* tmp, double tol, int maxit, int* iters, double* relres, hipStream_t st);
+                   double* q, double* tmp, double tol, int maxit, int* iters, double* relres, hipStream_t st,
+                   Control* ctl = nullptr, const int* bad = nullptr);
+void lp_launch_band_from_dense(int N, int W, const double* H, int Pp, double* Hb, hipStream_t st);
+void lp_launch_zero_flag(int* p, hipStream_t st);   // a kernel, so a captured graph clears it too
+
+// per-node buffers of one network forward pass
+struct LpNet {
+    float* X;
+    float* Z[LP_NLAYER + 1];
+    float* Hh[LP_NLAYER + 1];
+    float* M[LP_NLAYER];
+    float* OUT;
+    const double* sfeat;
+    const double* wsum;
+};
+void lp_forward(const LpSet& s, const float* P, const LpNet& b, hipStream_t st);
+// the network's parameters on the device, pushed if the host copy is newer
+// (call outside graph capture); nullptr on failure
+const float* lpcg_device_params(spgo_lpcg* h);
+int lpcg_band_blocks(const spgo_lpcg* h);
+
+// One CG solve of a solver's reduced system inside an LM iteration: the block
+// band of the dense H, features, the network (or block Jacobi), PCG into x.
+struct LpSolve {
+    int N = 0, W = 0;
+    LpSet set{};
+    LpNet net{};
+    double *Hb = nullptr, *C = nullptr, *N1 = nullptr, *sfeat = nullptr, *wsum = nullptr, *Gd = nullptr, *Go = nullptr;
+    double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *tmp = nullptr;
+    int *bad = nullptr, *node_sys = nullptr, *sys_begin = nullptr;
+    std::vector<void*> ptrs;
+    int setup(int n_poses, int band_blocks);   // 0 or a HIP error
+    void release();
+};
+// x = CG solution of the leading 6N x 6N block of the dense H (lower triangle,
+// row-major Pp x Pp) with right-hand side g; P: network parameters or nullptr
+// (block Jacobi).  Writes ctl->pcg_iters and ctl->pcg_fail.
+void lp_solve_launch(const LpSolve& b, const double* H, int Pp, const double* g, double* x, const float* P,
+                     double tol, int maxit, Control* ctl, hipStream_t st);
 
 }  // namespace spgo

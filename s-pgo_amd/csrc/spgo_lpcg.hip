@@ -118,17 +118,17 @@ const float* layer_b(const float* P, int l) { return layer_wm(P, l) + LP_HID * L
 
 // network forward over the loaded set: OUT = head(h_3, x)
 void forward(spgo_lpcg* h) {
-    const int T = h->set.T;
-    hipStream_t st = h->stream;
-    lp_launch_linear(T, LP_HID, LP_NIN, h->X, h->P + LP_OFF_WIN, 0, nullptr, nullptr, h->P + LP_OFF_BIN, h->Z[0],
-                     h->Hh[0], st);
-    for (int l = 0; l < LP_NLAYER; ++l) {
-        lp_launch_agg(h->set, h->sfeat, h->wsum, h->Hh[l], h->M[l], 0, st);
-        lp_launch_linear(T, LP_HID, LP_HID, h->Hh[l], layer_ws(h->P, l), LP_HID, h->M[l], layer_wm(h->P, l),
-                         layer_b(h->P, l), h->Z[l + 1], h->Hh[l + 1], st);
+    LpNet b;
+    b.X = h->X;
+    for (int l = 0; l <= LP_NLAYER; ++l) {
+        b.Z[l] = h->Z[l];
+        b.Hh[l] = h->Hh[l];
+        if (l < LP_NLAYER) b.M[l] = h->M[l];
     }
-    lp_launch_linear(T, LP_NOUT, LP_HID, h->Hh[LP_NLAYER], h->P + LP_OFF_WO, LP_NIN, h->X, h->P + LP_OFF_WK,
-                     h->P + LP_OFF_BO, h->OUT, nullptr, st);
+    b.OUT = h->OUT;
+    b.sfeat = h->sfeat;
+    b.wsum = h->wsum;
+    lp_forward(h->set, h->P, b, h->stream);
 }
 
 // loss per system into lsys
@@ -169,6 +169,96 @@ int mean_loss(spgo_lpcg* h, double* loss) {
 }
 
 }  // namespace
+
+namespace spgo {
+
+void lp_forward(const LpSet& s, const float* P, const LpNet& b, hipStream_t st) {
+    const int T = s.T;
+    lp_launch_linear(T, LP_HID, LP_NIN, b.X, P + LP_OFF_WIN, 0, nullptr, nullptr, P + LP_OFF_BIN, b.Z[0], b.Hh[0], st);
+    for (int l = 0; l < LP_NLAYER; ++l) {
+        lp_launch_agg(s, b.sfeat, b.wsum, b.Hh[l], b.M[l], 0, st);
+        lp_launch_linear(T, LP_HID, LP_HID, b.Hh[l], layer_ws(P, l), LP_HID, b.M[l], layer_wm(P, l), layer_b(P, l),
+                         b.Z[l + 1], b.Hh[l + 1], st);
+    }
+    lp_launch_linear(T, LP_NOUT, LP_HID, b.Hh[LP_NLAYER], P + LP_OFF_WO, LP_NIN, b.X, P + LP_OFF_WK, P + LP_OFF_BO,
+                     b.OUT, nullptr, st);
+}
+
+int LpSolve::setup(int n_poses, int band_blocks) {
+    release();
+    N = n_poses;
+    W = band_blocks;
+    auto take = [&](size_t bytes) -> void* {
+        void* ptr = nullptr;
+        if (hipMalloc(&ptr, bytes ? bytes : 8) != hipSuccess) return nullptr;
+        ptrs.push_back(ptr);
+        return ptr;
+    };
+    const size_t T = (size_t)N;
+    Hb = (double*)take(T * (W + 1) * 36 * 8);
+    C = (double*)take(T * 36 * 8);
+    N1 = (double*)take(T * 36 * 8);
+    sfeat = (double*)take(T * (W + 1) * 8);
+    wsum = (double*)take(T * 8);
+    Gd = (double*)take(T * 36 * 8);
+    Go = (double*)take(T * 36 * 8);
+    r = (double*)take(T * 6 * 8);
+    z = (double*)take(T * 6 * 8);
+    p = (double*)take(T * 6 * 8);
+    q = (double*)take(T * 6 * 8);
+    tmp = (double*)take(T * 6 * 8);
+    bad = (int*)take(4);
+    node_sys = (int*)take(T * 4);
+    sys_begin = (int*)take(8);
+    net.X = (float*)take(T * LP_NIN * 4);
+    for (int l = 0; l <= LP_NLAYER; ++l) {
+        net.Z[l] = (float*)take(T * LP_HID * 4);
+        net.Hh[l] = (float*)take(T * LP_HID * 4);
+        if (l < LP_NLAYER) net.M[l] = (float*)take(T * LP_HID * 4);
+    }
+    net.OUT = (float*)take(T * LP_NOUT * 4);
+    net.sfeat = sfeat;
+    net.wsum = wsum;
+    if (ptrs.size() != 16 + 2 * (LP_NLAYER + 1) + LP_NLAYER + 1) return (int)hipErrorOutOfMemory;   // one failed
+    std::vector<int> ns(T, 0);
+    const int sb[2] = {0, N};
+    hipError_t e = hipMemcpy(node_sys, ns.data(), T * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(sys_begin, sb, 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return (int)e;
+    set = LpSet{N, 1, W, Hb, nullptr, node_sys, sys_begin};
+    return 0;
+}
+
+void LpSolve::release() {
+    for (void* q2 : ptrs)
+        if (q2) (void)hipFree(q2);
+    ptrs.clear();
+}
+
+void lp_solve_launch(const LpSolve& b, const double* H, int Pp, const double* g, double* x, const float* P,
+                     double tol, int maxit, Control* ctl, hipStream_t st) {
+    LpSet s = b.set;
+    s.g = g;
+    lp_launch_band_from_dense(b.N, b.W, H, Pp, b.Hb, st);
+    lp_launch_zero_flag(b.bad, st);
+    lp_launch_features(s, b.C, b.sfeat, b.net.X, b.N1, b.wsum, b.bad, st);
+    if (P) {
+        lp_forward(s, P, b.net, st);
+        lp_launch_buildG(s, b.net.OUT, b.C, b.Gd, b.Go, st);
+        lp_launch_pcg(s, b.Gd, b.Go, x, b.r, b.z, b.p, b.q, b.tmp, tol, maxit, nullptr, nullptr, st, ctl, b.bad);
+    } else {
+        lp_launch_pcg(s, b.C, nullptr, x, b.r, b.z, b.p, b.q, b.tmp, tol, maxit, nullptr, nullptr, st, ctl, b.bad);
+    }
+}
+
+const float* lpcg_device_params(spgo_lpcg* h) {
+    if (!h || ensure_device(h) || push_params(h) || hipStreamSynchronize(h->stream) != hipSuccess) return nullptr;
+    return h->P;
+}
+
+int lpcg_band_blocks(const spgo_lpcg* h) { return h ? h->W : 0; }
+
+}  // namespace spgo
 
 extern "C" {
 

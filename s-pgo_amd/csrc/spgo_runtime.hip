@@ -296,6 +296,9 @@ static int factor_path(const spgo_solver* h, const HostLayout& L, std::vector<in
     case SPGO_FACTOR_DENSE: return SPGO_FACTOR_DENSE;
     case SPGO_FACTOR_ENVELOPE: return fits ? SPGO_FACTOR_ENVELOPE : -1;
     case SPGO_FACTOR_ENVELOPE_SPLIT: return split_fits ? SPGO_FACTOR_ENVELOPE_SPLIT : -1;
+    case SPGO_FACTOR_PCG:   // the envelope Cholesky behind CG, as AUTO picks it
+        if (split_fits && nbk >= 2 * depth + 6) return SPGO_FACTOR_ENVELOPE_SPLIT;
+        return fits ? SPGO_FACTOR_ENVELOPE : -1;
     default:
         if (split_fits && nbk >= 2 * depth + 6) return SPGO_FACTOR_ENVELOPE_SPLIT;
         return fits ? SPGO_FACTOR_ENVELOPE : SPGO_FACTOR_DENSE;
@@ -400,6 +403,17 @@ int setup_layout(spgo_solver* h) {
         if (h->path < 0)
             return h->fail(SPGO_E_LAYOUT, "the reduced-system profile does not fit the requested factorisation");
         h->band = h->path != SPGO_FACTOR_DENSE;
+        h->pcg = h->opt.factorization == SPGO_FACTOR_PCG;
+        h->lps.release();
+        if (h->pcg) {
+            // CG over the pose blocks: spans of K poses couple blocks up to K - 1 apart
+            const int need = std::max(1, L.kmax - 1);
+            const int W = h->net ? lpcg_band_blocks(h->net) : std::min(need, LP_WMAX);
+            if (need > W)
+                return h->fail(SPGO_E_LAYOUT, "landmark spans need a band of " + std::to_string(need) +
+                                                  " blocks; the preconditioner network covers " + std::to_string(W));
+            if (h->lps.setup(h->N, W) != 0) return h->fail(SPGO_E_DEVICE, "device allocation failed (CG buffers)");
+        }
         if (h->band && !(h->Lpan = h->mem.alloc<double>(chol_panel_doubles(h->Pp))))
             return h->fail(SPGO_E_DEVICE, "device allocation failed");
         if ((rc = upload(h, fb, &h->first))) return rc;
@@ -459,6 +473,7 @@ int setup_layout(spgo_solver* h) {
     Control c{};
     c.cur = 0;
     c.chol_ok = 1;
+    c.pcg_fail = 1;
     c.lambda = 0.0;
     rc = check_hip(h, hipMemcpy(h->ctl, &c, sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");
     if (rc) return rc;
@@ -512,6 +527,9 @@ template <typename T>
 void launch_iteration(spgo_solver* h, StateBuf<T>* b) {
     hipStream_t s = h->stream;
     launch_form_system<T>(h->lay, b, h->ctl, h->H, h->gvec, s);
+    if (h->pcg)
+        lp_solve_launch(h->lps, h->H, h->Pp, h->gvec, h->delta, h->net_params, h->opt.pcg_tol, h->opt.pcg_max_iter,
+                        h->ctl, s);
     if (h->band) {
         launch_chol(h, s);
     } else {
@@ -610,6 +628,9 @@ int run_step(spgo_solver* h, int iterations) {
     rc = check_hip(h, hipMemcpy(h->ctl, &c, sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");
     if (rc) return rc;
     const bool mixed = h->prec == SPGO_PREC_MIXED;
+    h->net_params = nullptr;
+    if (h->pcg && h->net && !(h->net_params = lpcg_device_params(h->net)))
+        return h->fail(SPGO_E_DEVICE, "cannot read the preconditioner network's parameters");
     if (!h->lin_valid) {
         if (mixed)
             launch_linearise<float>(h, h->bf);
@@ -685,7 +706,9 @@ bool valid_opts(const spgo_options* o, std::string& why) {
     if (!(o->alpha_min > 0.0) || !(o->alpha_max >= o->alpha_min)) return why = "alpha bounds", false;
     if (o->chunk_landmarks < 0 || o->chunk_landmarks > CHUNK_LM) return why = "chunk_landmarks", false;
     if (o->group_chunks < 0) return why = "group_chunks", false;
-    if (o->factorization < 0 || o->factorization > 3) return why = "factorization", false;
+    if (o->factorization < 0 || o->factorization > 4) return why = "factorization", false;
+    if (o->pcg_max_iter < 1) return why = "pcg_max_iter", false;
+    if (!(o->pcg_tol > 0.0)) return why = "pcg_tol", false;
     return true;
 }
 
@@ -716,6 +739,8 @@ int spgo_default_options(spgo_options* o) {
     o->use_graph = 1;
     o->chunk_landmarks = CHUNK_LM;
     o->group_chunks = 1;
+    o->pcg_max_iter = 200;
+    o->pcg_tol = 1e-8;
     return SPGO_OK;
 }
 
@@ -758,6 +783,7 @@ int spgo_destroy(spgo_solver* h) {
     if (!h) return SPGO_E_ARG;
     destroy_graph(h);
     h->mem.release();
+    h->lps.release();
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -993,6 +1019,8 @@ int spgo_get_stats(spgo_solver* h, spgo_iter_stats* out, int32_t max_records, in
             out[i].lambda = h->last_rec[i].lambda;
             out[i].accepted = h->last_rec[i].accepted;
             out[i].chol_ok = h->last_rec[i].chol_ok;
+            out[i].pcg_iters = h->last_rec[i].pcg_iters;
+            out[i].pcg_fallback = h->pcg ? h->last_rec[i].pcg_fail : 0;
         }
     }
     return SPGO_OK;
@@ -1078,6 +1106,9 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
         if (mixed) launch_form_system<float>(h->lay, h->bf, h->ctl, h->H, h->gvec, s);
         else launch_form_system<double>(h->lay, h->bd, h->ctl, h->H, h->gvec, s);
         (void)hipEventRecord(ev[1], s);
+        if (h->pcg)
+            lp_solve_launch(h->lps, h->H, h->Pp, h->gvec, h->delta, h->net_params, h->opt.pcg_tol,
+                            h->opt.pcg_max_iter, h->ctl, s);
         if (h->band) {
             launch_chol(h, s);
             (void)hipEventRecord(ev[2], s);
@@ -1116,6 +1147,14 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
     return SPGO_OK;
 }
 
+int spgo_set_preconditioner(spgo_solver* h, spgo_lpcg* net) {
+    if (!h) return SPGO_E_ARG;
+    h->net = net;
+    h->layout_dirty = true;   // the CG band follows the network's
+    destroy_graph(h);
+    return SPGO_OK;
+}
+
 int spgo_problem_info(spgo_solver* h, spgo_layout_info* out) {
     if (!h || !out) return SPGO_E_ARG;
     HostLayout hl;
@@ -1132,6 +1171,7 @@ int spgo_problem_info(spgo_solver* h, spgo_layout_info* out) {
     std::vector<int> fb;
     int depth = 0;
     out->factorization = factor_path(h, hl, fb, depth);
+    if (h->opt.factorization == SPGO_FACTOR_PCG && out->factorization >= 0) out->factorization = SPGO_FACTOR_PCG;
     out->profile_depth = depth;
     out->reduced_size = reduced_size(h->N);
     return SPGO_OK;

@@ -24,6 +24,7 @@ SPGO_FACTOR_AUTO = 0
 SPGO_FACTOR_DENSE = 1
 SPGO_FACTOR_ENVELOPE = 2
 SPGO_FACTOR_ENVELOPE_SPLIT = 3
+SPGO_FACTOR_PCG = 4
 
 ERRORS = {-1: "invalid argument", -2: "capacity exceeded", -3: "unsupported layout",
           -4: "device error", -5: "invalid state"}
@@ -36,12 +37,13 @@ class Options(ctypes.Structure):
                 ("alpha_min", c_double), ("alpha_max", c_double),
                 ("refine_steps", c_int32), ("use_graph", c_int32),
                 ("chunk_landmarks", c_int32), ("group_chunks", c_int32),
-                ("factorization", c_int32), ("reserved", c_int32 * 7)]
+                ("factorization", c_int32), ("pcg_max_iter", c_int32), ("reserved", c_int32 * 6),
+                ("pcg_tol", c_double)]
 
 
 class IterStats(ctypes.Structure):
     _fields_ = [("cost_candidate", c_double), ("cost", c_double), ("lambda_", c_double),
-                ("accepted", c_int32), ("chol_ok", c_int32)]
+                ("accepted", c_int32), ("chol_ok", c_int32), ("pcg_iters", c_int32), ("pcg_fallback", c_int32)]
 
 
 class LayoutInfo(ctypes.Structure):
@@ -91,6 +93,7 @@ SIGNATURES = {
     "spgo_last_step_ms": (c_int32, [c_void_p, F64P]),
     "spgo_profile_phases": (c_int32, [c_void_p, c_int32, F64P]),
     "spgo_last_error": (c_char_p, [c_void_p]),
+    "spgo_set_preconditioner": (c_int32, [c_void_p, c_void_p]),
     "spgo_batch_create": (c_int32, [c_int32, POINTER(c_void_p), POINTER(Options), POINTER(c_void_p)]),
     "spgo_batch_destroy": (c_int32, [c_void_p]),
     "spgo_batch_set_options": (c_int32, [c_void_p, POINTER(Options)]),

@@ -181,7 +181,8 @@ class Solver:
         buf = (IterStats * max(n.value, 1))()
         self._check(self._lib.spgo_get_stats(self._h, buf, n.value, ctypes.byref(n), ctypes.byref(c0)))
         recs = [dict(cost_candidate=buf[i].cost_candidate, cost=buf[i].cost, lam=buf[i].lambda_,
-                     accepted=bool(buf[i].accepted), chol_ok=bool(buf[i].chol_ok)) for i in range(n.value)]
+                     accepted=bool(buf[i].accepted), chol_ok=bool(buf[i].chol_ok),
+                     pcg_iters=int(buf[i].pcg_iters), pcg_fallback=bool(buf[i].pcg_fallback)) for i in range(n.value)]
         return c0.value, recs
 
     def get_lambda(self) -> float:
@@ -220,6 +221,12 @@ class Solver:
         v = ctypes.c_double()
         self._check(self._lib.spgo_last_step_ms(self._h, ctypes.byref(v)))
         return v.value
+
+    def set_preconditioner(self, net) -> None:
+        """The learned preconditioner (an ``lpcg.LearnedPCG``) of the PCG
+        factorisation, or None for block Jacobi.  The solver keeps a reference."""
+        self._check(self._lib.spgo_set_preconditioner(self._h, net._h if net is not None else None))
+        self._net = net
 
 
 def solver_from_problem(p, options: Optional[Options] = None) -> Solver:
@@ -310,7 +317,8 @@ class Batch:
         self._check(self._lib.spgo_batch_get_stats(self._h, int(p), buf, n.value, ctypes.byref(n),
                                                    ctypes.byref(c0)))
         recs = [dict(cost_candidate=buf[i].cost_candidate, cost=buf[i].cost, lam=buf[i].lambda_,
-                     accepted=bool(buf[i].accepted), chol_ok=bool(buf[i].chol_ok)) for i in range(n.value)]
+                     accepted=bool(buf[i].accepted), chol_ok=bool(buf[i].chol_ok),
+                     pcg_iters=int(buf[i].pcg_iters), pcg_fallback=bool(buf[i].pcg_fallback)) for i in range(n.value)]
         return c0.value, recs
 
     def last_step_ms(self) -> float:

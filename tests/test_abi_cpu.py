@@ -36,8 +36,11 @@ def test_default_options_and_struct_layout():
     assert o.mode == _lib.SPGO_MODE_LM and o.robust == _lib.SPGO_ROBUST_CAUCHY
     assert o.robust_param == 9.0 and o.eps == 1e-4 and o.lambda0 == 1e-4
     assert o.alpha_min == pytest.approx(1 / 30) and o.alpha_max == pytest.approx(5.0)
-    assert ctypes.sizeof(_lib.Options) == 4 * 4 + 9 * 8 + 4 * 4 + 8 * 4
+    assert ctypes.sizeof(_lib.Options) == 4 * 4 + 9 * 8 + 5 * 4 + 7 * 4 + 8
+    assert _lib.Options.pcg_tol.offset == 136
     assert o.factorization == _lib.SPGO_FACTOR_AUTO
+    assert o.pcg_max_iter == 200 and o.pcg_tol == 1e-8
+    assert ctypes.sizeof(_lib.IterStats) == 3 * 8 + 4 * 4
 
 
 def test_reference_facade_semantics_on_host():

@@ -85,3 +85,50 @@ def test_training_beats_block_jacobi_on_held_out_systems():
         H = LP.dense_from_band(Hb)
         assert np.linalg.norm(H @ x[o:o + n].reshape(-1) - g.reshape(-1)) <= 1.5e-8 * np.linalg.norm(g)
         o += n
+
+
+def _lm_run(p, iters, **kw):
+    from spgo.solver import make_options, solver_from_problem
+    net = kw.pop("net", None)
+    s = solver_from_problem(p, make_options(mode="lm", **kw))
+    if net is not None:
+        s.set_preconditioner(net)
+    s.step(iters)
+    return s.stats(), s
+
+
+def test_lm_step_with_cg_matches_the_cholesky_path():
+    p = make_problem(40, 3000, 6, seed=61)
+    (c0, ref), _ = _lm_run(p, 10)
+    # block-Jacobi CG to 1e-11: the same LM iterates as the exact solve
+    (c1, cg), s = _lm_run(p, 10, factorization=4, pcg_tol=1e-11, pcg_max_iter=2000)
+    assert s.layout_info()["factorization"] == 4
+    print(f"[lm-cg] CG iterations per step {[r['pcg_iters'] for r in cg]}", flush=True)
+    assert c1 == pytest.approx(c0, rel=1e-12)
+    for r, q in zip(ref, cg):
+        assert q["accepted"] == r["accepted"] and not q["pcg_fallback"] and q["pcg_iters"] > 0
+        assert q["cost"] == pytest.approx(r["cost"], rel=1e-8)
+    # CG that cannot converge in 2 iterations: the Cholesky computes every step
+    (c2, fb), _ = _lm_run(p, 10, factorization=4, pcg_max_iter=2)
+    for r, q in zip(ref, fb):
+        assert q["pcg_fallback"] and q["pcg_iters"] == 2
+        assert q["cost"] == pytest.approx(r["cost"], rel=1e-12) and q["accepted"] == r["accepted"]
+
+
+def test_lm_step_with_the_trained_network():
+    train = solver_systems(_problems(range(300, 306), N=32, L=1500), W, states_per_problem=2)
+    lp = LearnedPCG(W, seed=9)
+    lp.load(train)
+    lp.train(200, lr=1e-3)
+    p = make_problem(48, 4000, 6, seed=77)
+    (c0, ref), _ = _lm_run(p, 15)
+    (cj, bj), _ = _lm_run(p, 15, factorization=4)
+    (cl, le), s = _lm_run(p, 15, factorization=4, net=lp)
+    it_j = [r["pcg_iters"] for r in bj]
+    it_l = [r["pcg_iters"] for r in le]
+    print(f"[lm-cg] CG iterations per LM step: block Jacobi {it_j}, learned {it_l}; final cost "
+          f"Cholesky {ref[-1]['cost']:.10g} learned CG {le[-1]['cost']:.10g}", flush=True)
+    assert not any(r["pcg_fallback"] for r in le)
+    assert sum(it_l) <= 0.8 * sum(it_j)
+    # PCG to 1e-8 reaches the exact-solve LM's final cost to 1e-6
+    assert abs(le[-1]["cost"] - ref[-1]["cost"]) <= 1e-6 * ref[-1]["cost"]
