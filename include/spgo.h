@@ -54,8 +54,8 @@ extern "C" {
  * ENVELOPE for short narrow ones and DENSE for wide ones.  PCG: conjugate
  * gradients preconditioned by the network set with spgo_set_preconditioner
  * (block Jacobi without one) to pcg_tol; when CG does not converge in
- * pcg_max_iter iterations the envelope Cholesky (as AUTO would pick it)
- * computes the step instead. */
+ * pcg_max_iter iterations the Cholesky AUTO would pick computes the step
+ * instead. */
 #define SPGO_FACTOR_AUTO 0
 #define SPGO_FACTOR_DENSE 1
 #define SPGO_FACTOR_ENVELOPE 2

@@ -18,6 +18,8 @@
 //   t_i = tr(A_i A_i^T) + 2 tr(A_i N_i B_i^T) + tr(B_i B_i^T),
 // with A_i = I + tril(out_i[0:21]), B_i = out_i[21:57], N_i = C_i H_{i,i-1}
 // C_{i-1}^T -- Adam, and PCG (one workgroup per system, FP64).
+#include <hip/hip_cooperative_groups.h>
+
 #include "spgo_device.h"
 #include "spgo_internal.h"
 #include "spgo_lpcg.h"
@@ -545,6 +547,133 @@ __global__ void k_lp_band_from_dense(int N, int W, const double* H, int Pp, doub
     Hb[t] = v;
 }
 
+// ---- PCG on one large system: every workgroup of a cooperative launch --------
+// Work item e = 6 i + a (node i, row a) in grid-stride loops; global sums are
+// block partials in part[slot][block], summed by every block in the same order
+// after a grid barrier, so all blocks take the same decisions.  Slots rotate
+// so that a slot is rewritten only after two more barriers.
+__global__ __launch_bounds__(256) void k_lp_pcg_coop(int T, int W, const double* Hb, const double* Gd,
+                                                     const double* Go, const double* g, double* x, double* r,
+                                                     double* z, double* p, double* q, double* tmp, double* part,
+                                                     double tol, int maxit, Control* ctl, const int* bad) {
+    namespace cgs = cooperative_groups;
+    cgs::grid_group grid = cgs::this_grid();
+    __shared__ double red[256];
+    const int tid = threadIdx.x, nblk = gridDim.x;
+    const int e0 = blockIdx.x * 256 + tid, es = nblk * 256, E = 6 * T;
+    auto block_sum = [&](double v) {
+        red[tid] = v;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (tid < w) red[tid] += red[tid + w];
+            __syncthreads();
+        }
+        const double s = red[0];
+        __syncthreads();
+        return s;
+    };
+    auto grid_sum = [&](double v, int slot) {
+        const double b = block_sum(v);
+        if (tid == 0) part[slot * nblk + blockIdx.x] = b;
+        grid.sync();
+        double s = 0.0;
+        for (int k = tid; k < nblk; k += 256) s += part[slot * nblk + k];
+        return block_sum(s);
+    };
+    // y_i[a] = (H v)_i[a] over the band
+    auto hrow = [&](int e, const double* v) {
+        const int i = e / 6, a = e - 6 * (e / 6);
+        double y = 0.0;
+        for (int d = 0; d <= W; ++d) {
+            if (i - d >= 0) {
+                const double* B = Hb + ((long long)i * (W + 1) + d) * 36 + a * 6;
+                const double* vv = v + 6LL * (i - d);
+                for (int b = 0; b < 6; ++b) y += B[b] * vv[b];
+            }
+            if (d > 0 && i + d < T) {
+                const double* B = Hb + ((long long)(i + d) * (W + 1) + d) * 36 + a;
+                const double* vv = v + 6LL * (i + d);
+                for (int b = 0; b < 6; ++b) y += B[6 * b] * vv[b];
+            }
+        }
+        return y;
+    };
+    auto prec = [&]() {   // tmp = G r, then z = G^T tmp
+        for (int e = e0; e < E; e += es) {
+            const int i = e / 6, a = e - 6 * i;
+            double s = 0.0;
+            for (int b = 0; b < 6; ++b) s += Gd[(long long)i * 36 + a * 6 + b] * r[6LL * i + b];
+            if (Go && i > 0)
+                for (int b = 0; b < 6; ++b) s += Go[(long long)i * 36 + a * 6 + b] * r[6LL * (i - 1) + b];
+            tmp[e] = s;
+        }
+        grid.sync();
+        for (int e = e0; e < E; e += es) {
+            const int i = e / 6, a = e - 6 * i;
+            double s = 0.0;
+            for (int b = 0; b < 6; ++b) s += Gd[(long long)i * 36 + b * 6 + a] * tmp[6LL * i + b];
+            if (Go && i + 1 < T)
+                for (int b = 0; b < 6; ++b) s += Go[(long long)(i + 1) * 36 + b * 6 + a] * tmp[6LL * (i + 1) + b];
+            z[e] = s;
+        }
+    };
+    double gg = 0.0;
+    for (int e = e0; e < E; e += es) {
+        x[e] = 0.0;
+        r[e] = g[e];
+        gg += g[e] * g[e];
+    }
+    const double g0 = sqrt(grid_sum(gg, 0));
+    bool broke = bad && *bad;
+    int it = 0;
+    double rn = g0;
+    if (!broke && g0 > 0.0) {
+        prec();
+        double rz = 0.0;
+        for (int e = e0; e < E; e += es) {
+            p[e] = z[e];
+            rz += r[e] * z[e];
+        }
+        rz = grid_sum(rz, 1);
+        while (it < maxit) {
+            double pq = 0.0;
+            for (int e = e0; e < E; e += es) {
+                const double y = hrow(e, p);
+                q[e] = y;
+                pq += p[e] * y;
+            }
+            pq = grid_sum(pq, 2);
+            if (!(pq > 0.0) || !(rz > 0.0)) {
+                broke = true;
+                break;
+            }
+            const double al = rz / pq;
+            double rr = 0.0;
+            for (int e = e0; e < E; e += es) {
+                x[e] += al * p[e];
+                r[e] -= al * q[e];
+                rr += r[e] * r[e];
+            }
+            rn = sqrt(grid_sum(rr, 3));
+            ++it;
+            if (!(rn > tol * g0)) break;
+            prec();
+            double rz2 = 0.0;
+            for (int e = e0; e < E; e += es) rz2 += r[e] * z[e];
+            rz2 = grid_sum(rz2, 0);
+            const double be = rz2 / rz;
+            rz = rz2;
+            for (int e = e0; e < E; e += es) p[e] = z[e] + be * p[e];
+            grid.sync();
+        }
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+        const bool ok = !broke && (rn <= tol * g0 || g0 == 0.0);
+        ctl->pcg_iters = it;
+        ctl->pcg_fail = ok ? 0 : 1;
+    }
+}
+
 __global__ void k_lp_zero_flag(int* p) { *p = 0; }
 
 // ---- launchers ------------------------------------------------------------------
@@ -620,6 +749,21 @@ void lp_launch_pcg(const LpSet& s, const double* Gd, const double* Go, double* x
                    Control* ctl, const int* bad) {
     hipLaunchKernelGGL(k_lp_pcg, dim3(s.S), dim3(256), 0, st, s.sys_begin, s.W, s.Hb, Gd, Go, s.g, x, r, z, p, q, tmp,
                        tol, maxit, iters, relres, ctl, bad);
+}
+
+int lp_coop_max_blocks() {
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lp_pcg_coop, 256, 0) != hipSuccess) return 0;
+    return per_cu * cus;
+}
+
+int lp_launch_pcg_coop(int T, int W, const double* Hb, const double* Gd, const double* Go, const double* g, double* x,
+                       double* r, double* z, double* p, double* q, double* tmp, double* part, int blocks, double tol,
+                       int maxit, Control* ctl, const int* bad, hipStream_t st) {
+    void* args[] = {&T, &W, &Hb, &Gd, &Go, &g, &x, &r, &z, &p, &q, &tmp, &part, &tol, &maxit, &ctl, &bad};
+    return (int)hipLaunchCooperativeKernel((const void*)k_lp_pcg_coop, dim3(blocks), dim3(256), args, 0, st);
 }
 
 void lp_launch_zero_flag(int* p, hipStream_t st) { hipLaunchKernelGGL(k_lp_zero_flag, dim3(1), dim3(1), 0, st, p); }

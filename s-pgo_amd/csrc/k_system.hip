@@ -118,6 +118,7 @@ __global__ __launch_bounds__(256) void k_chol_panel(double* H, double* Linv, int
     __shared__ double Lkk[NB][NB + 1];
     __shared__ double Li[NB][NB + 1];
     __shared__ double Ab[NB][NB + 1];
+    if (ctl->pcg_fail == 0) return;   // the CG solve of this iteration converged
     const int tid = threadIdx.x;
     const int k0 = k * NB;
     if (tid < 64) {
@@ -192,7 +193,8 @@ __global__ __launch_bounds__(256) void k_chol_panel(double* H, double* Linv, int
 
 // Trailing update A_ij -= L_ik L_jk^T for k < j <= i, one NB x NB tile per
 // workgroup, four 16x16 MFMA tiles per wavefront-quarter of the block.
-__global__ __launch_bounds__(256) void k_chol_update(double* H, int Pp, int k, int nblk) {
+__global__ __launch_bounds__(256) void k_chol_update(double* H, int Pp, int k, int nblk, const Control* ctl) {
+    if (ctl->pcg_fail == 0) return;   // the CG solve of this iteration converged
     const int n = nblk - k - 1;
     const int t = blockIdx.x;
     int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
@@ -233,7 +235,7 @@ void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t 
     for (int k = 0; k < nblk; ++k) {
         hipLaunchKernelGGL(k_chol_panel, dim3(nblk - k), dim3(256), 0, s, H, Linv, Pp, k, ctl);
         const int n = nblk - k - 1;
-        if (n > 0) hipLaunchKernelGGL(k_chol_update, dim3(n * (n + 1) / 2), dim3(256), 0, s, H, Pp, k, nblk);
+        if (n > 0) hipLaunchKernelGGL(k_chol_update, dim3(n * (n + 1) / 2), dim3(256), 0, s, H, Pp, k, nblk, ctl);
     }
 }
 
@@ -901,9 +903,10 @@ int chol_band_block() { return EB; }
 // L L^T x = g, one workgroup, right-hand side in LDS
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_chol_solve(const double* H, const double* Linv, const double* g,
-                                                     double* x, int Pp) {
+                                                     double* x, int Pp, const Control* ctl) {
     extern __shared__ double rhs[];
     __shared__ double y[NB];
+    if (ctl->pcg_fail == 0) return;   // the CG solve of this iteration converged
     const int tid = threadIdx.x;
     const int nblk = Pp / NB;
     for (int i = tid; i < Pp; i += 1024) rhs[i] = g[i];
@@ -947,8 +950,8 @@ __global__ __launch_bounds__(1024) void k_chol_solve(const double* H, const doub
 }
 
 void launch_chol_solve(const double* H, const double* Linv, const double* g, double* delta, int Pp,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), Pp * sizeof(double), s, H, Linv, g, delta, Pp);
+                       const Control* ctl, hipStream_t s) {
+    hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), Pp * sizeof(double), s, H, Linv, g, delta, Pp, ctl);
 }
 
 }  // namespace spgo

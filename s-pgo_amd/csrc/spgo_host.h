@@ -108,6 +108,7 @@ struct spgo_solver {
     spgo_lpcg* net = nullptr;   // its preconditioner network (nullptr: block Jacobi)
     const float* net_params = nullptr;   // the network's device parameters for this step
     spgo::LpSolve lps;          // CG buffers
+    int* pcg_flag_host = nullptr;   // pinned: CG convergence flag read back each iteration
     Control* ctl = nullptr;
     IterRecord* rec = nullptr;
     int rec_cap = 0;

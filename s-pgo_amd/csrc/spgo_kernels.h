@@ -39,7 +39,7 @@ int prepare_chol_band(int Dmax, size_t lds);
 int chol_band_block();
 size_t chol_band_lds(int Pp, int D);
 void launch_chol_solve(const double* H, const double* Linv, const double* g, double* delta, int Pp,
-                       hipStream_t s);
+                       const Control* ctl, hipStream_t s);
 
 template <typename T>
 void launch_pose_update(const Layout& lay, const Camera& cam, StateBuf<T>* bufs, const Control* ctl,

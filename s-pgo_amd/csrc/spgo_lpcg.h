@@ -61,6 +61,13 @@ void lp_launch_pcg(const LpSet& s, const double* Gd, const double* Go, double* x
                    Control* ctl = nullptr, const int* bad = nullptr);
 void lp_launch_band_from_dense(int N, int W, const double* H, int Pp, double* Hb, hipStream_t st);
 void lp_launch_zero_flag(int* p, hipStream_t st);   // a kernel, so a captured graph clears it too
+// PCG on one system with a cooperative launch of `blocks` workgroups (at most
+// lp_coop_max_blocks(), all resident); part holds 4 x blocks doubles.  Returns
+// the launch status.  Sets ctl->pcg_iters and ctl->pcg_fail.
+int lp_coop_max_blocks();
+int lp_launch_pcg_coop(int T, int W, const double* Hb, const double* Gd, const double* Go, const double* g, double* x,
+                       double* r, double* z, double* p, double* q, double* tmp, double* part, int blocks, double tol,
+                       int maxit, Control* ctl, const int* bad, hipStream_t st);
 
 // per-node buffers of one network forward pass
 struct LpNet {
@@ -87,6 +94,8 @@ struct LpSolve {
     double *Hb = nullptr, *C = nullptr, *N1 = nullptr, *sfeat = nullptr, *wsum = nullptr, *Gd = nullptr, *Go = nullptr;
     double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *tmp = nullptr;
     int *bad = nullptr, *node_sys = nullptr, *sys_begin = nullptr;
+    double* part = nullptr;   // cooperative PCG: 4 x coop_blocks partial sums
+    int coop_blocks = 0;
     std::vector<void*> ptrs;
     int setup(int n_poses, int band_blocks);   // 0 or a HIP error
     void release();
@@ -94,7 +103,7 @@ struct LpSolve {
 // x = CG solution of the leading 6N x 6N block of the dense H (lower triangle,
 // row-major Pp x Pp) with right-hand side g; P: network parameters or nullptr
 // (block Jacobi).  Writes ctl->pcg_iters and ctl->pcg_fail.
-void lp_solve_launch(const LpSolve& b, const double* H, int Pp, const double* g, double* x, const float* P,
-                     double tol, int maxit, Control* ctl, hipStream_t st);
+int lp_solve_launch(const LpSolve& b, const double* H, int Pp, const double* g, double* x, const float* P,
+                    double tol, int maxit, Control* ctl, hipStream_t st);   // 0 or a HIP error
 
 }  // namespace spgo

@@ -7,6 +7,7 @@
 // in the k_lp_* kernels on the handle's stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -219,7 +220,12 @@ int LpSolve::setup(int n_poses, int band_blocks) {
     net.OUT = (float*)take(T * LP_NOUT * 4);
     net.sfeat = sfeat;
     net.wsum = wsum;
-    if (ptrs.size() != 16 + 2 * (LP_NLAYER + 1) + LP_NLAYER + 1) return (int)hipErrorOutOfMemory;   // one failed
+    // the cooperative PCG: one work item per unknown, at most all resident workgroups
+    const int maxb = lp_coop_max_blocks();
+    coop_blocks = std::min(maxb, (int)((6 * T + 255) / 256));
+    part = (double*)take((size_t)4 * std::max(coop_blocks, 1) * 8);
+    if (coop_blocks < 1) return (int)hipErrorLaunchFailure;
+    if (ptrs.size() != 17 + 2 * (LP_NLAYER + 1) + LP_NLAYER + 1) return (int)hipErrorOutOfMemory;   // one failed
     std::vector<int> ns(T, 0);
     const int sb[2] = {0, N};
     hipError_t e = hipMemcpy(node_sys, ns.data(), T * 4, hipMemcpyHostToDevice);
@@ -235,8 +241,8 @@ void LpSolve::release() {
     ptrs.clear();
 }
 
-void lp_solve_launch(const LpSolve& b, const double* H, int Pp, const double* g, double* x, const float* P,
-                     double tol, int maxit, Control* ctl, hipStream_t st) {
+int lp_solve_launch(const LpSolve& b, const double* H, int Pp, const double* g, double* x, const float* P,
+                    double tol, int maxit, Control* ctl, hipStream_t st) {
     LpSet s = b.set;
     s.g = g;
     lp_launch_band_from_dense(b.N, b.W, H, Pp, b.Hb, st);
@@ -245,10 +251,9 @@ void lp_solve_launch(const LpSolve& b, const double* H, int Pp, const double* g,
     if (P) {
         lp_forward(s, P, b.net, st);
         lp_launch_buildG(s, b.net.OUT, b.C, b.Gd, b.Go, st);
-        lp_launch_pcg(s, b.Gd, b.Go, x, b.r, b.z, b.p, b.q, b.tmp, tol, maxit, nullptr, nullptr, st, ctl, b.bad);
-    } else {
-        lp_launch_pcg(s, b.C, nullptr, x, b.r, b.z, b.p, b.q, b.tmp, tol, maxit, nullptr, nullptr, st, ctl, b.bad);
     }
+    return lp_launch_pcg_coop(b.N, b.W, b.Hb, P ? b.Gd : b.C, P ? b.Go : nullptr, g, x, b.r, b.z, b.p, b.q, b.tmp,
+                              b.part, b.coop_blocks, tol, maxit, ctl, b.bad, st);
 }
 
 const float* lpcg_device_params(spgo_lpcg* h) {

@@ -296,10 +296,7 @@ static int factor_path(const spgo_solver* h, const HostLayout& L, std::vector<in
     case SPGO_FACTOR_DENSE: return SPGO_FACTOR_DENSE;
     case SPGO_FACTOR_ENVELOPE: return fits ? SPGO_FACTOR_ENVELOPE : -1;
     case SPGO_FACTOR_ENVELOPE_SPLIT: return split_fits ? SPGO_FACTOR_ENVELOPE_SPLIT : -1;
-    case SPGO_FACTOR_PCG:   // the envelope Cholesky behind CG, as AUTO picks it
-        if (split_fits && nbk >= 2 * depth + 6) return SPGO_FACTOR_ENVELOPE_SPLIT;
-        return fits ? SPGO_FACTOR_ENVELOPE : -1;
-    default:
+    default:   // AUTO, and the Cholesky behind CG
         if (split_fits && nbk >= 2 * depth + 6) return SPGO_FACTOR_ENVELOPE_SPLIT;
         return fits ? SPGO_FACTOR_ENVELOPE : SPGO_FACTOR_DENSE;
     }
@@ -413,6 +410,11 @@ int setup_layout(spgo_solver* h) {
                 return h->fail(SPGO_E_LAYOUT, "landmark spans need a band of " + std::to_string(need) +
                                                   " blocks; the preconditioner network covers " + std::to_string(W));
             if (h->lps.setup(h->N, W) != 0) return h->fail(SPGO_E_DEVICE, "device allocation failed (CG buffers)");
+            if (!h->pcg_flag_host &&
+                hipHostMalloc((void**)&h->pcg_flag_host, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+                h->pcg_flag_host = nullptr;
+                return h->fail(SPGO_E_DEVICE, "pinned host allocation failed");
+            }
         }
         if (h->band && !(h->Lpan = h->mem.alloc<double>(chol_panel_doubles(h->Pp))))
             return h->fail(SPGO_E_DEVICE, "device allocation failed");
@@ -524,22 +526,36 @@ static void launch_chol(spgo_solver* h, hipStream_t s) {
 }
 
 template <typename T>
-void launch_iteration(spgo_solver* h, StateBuf<T>* b) {
+int launch_iteration(spgo_solver* h, StateBuf<T>* b) {
     hipStream_t s = h->stream;
     launch_form_system<T>(h->lay, b, h->ctl, h->H, h->gvec, s);
-    if (h->pcg)
-        lp_solve_launch(h->lps, h->H, h->Pp, h->gvec, h->delta, h->net_params, h->opt.pcg_tol, h->opt.pcg_max_iter,
-                        h->ctl, s);
-    if (h->band) {
-        launch_chol(h, s);
-    } else {
-        launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
-        launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, s);
+    bool factor = true;
+    if (h->pcg) {
+        // CG first; the host reads its convergence flag and launches the
+        // Cholesky only when CG did not converge
+        int rc = check_hip(h, (hipError_t)lp_solve_launch(h->lps, h->H, h->Pp, h->gvec, h->delta, h->net_params,
+                                                          h->opt.pcg_tol, h->opt.pcg_max_iter, h->ctl, s),
+                           "cooperative CG launch");
+        if (rc) return rc;
+        if ((rc = check_hip(h, hipMemcpyAsync(h->pcg_flag_host, &h->ctl->pcg_fail, sizeof(int),
+                                              hipMemcpyDeviceToHost, s), "hipMemcpyAsync")) ||
+            (rc = check_hip(h, hipStreamSynchronize(s), "CG")))
+            return rc;
+        factor = *h->pcg_flag_host != 0;
+    }
+    if (factor) {
+        if (h->band) {
+            launch_chol(h, s);
+        } else {
+            launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
+            launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, h->ctl, s);
+        }
     }
     launch_pose_update<T>(h->lay, make_camera(h), b, h->ctl, h->delta, (double)h->step_size, 0, s);
     launch_assemble<T>(make_asm(h, 0), b, s);
     launch_accept(b[0].cost, b[1].cost, h->lay, h->ctl, h->rec, h->opt.mode == SPGO_MODE_REFERENCE,
                   h->opt.lam_up, h->opt.lam_down, h->opt.lam_min, h->opt.lam_max, 0, s);
+    return SPGO_OK;
 }
 
 template <typename T>
@@ -649,15 +665,19 @@ int run_step(spgo_solver* h, int iterations) {
     }
     rc = check_hip(h, hipGetLastError(), "kernel launch");
     if (rc) return rc;
-    if (h->opt.use_graph && iterations > 0 && h->graph_dirty) {
+    // CG decides on the host whether the Cholesky runs: no graph for it
+    const bool graph = h->opt.use_graph && !h->pcg;
+    if (graph && iterations > 0 && h->graph_dirty) {
         destroy_graph(h);
         rc = check_hip(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         if (rc) return rc;
         if (mixed)
-            launch_iteration<float>(h, h->bf);
+            rc = launch_iteration<float>(h, h->bf);
         else
-            launch_iteration<double>(h, h->bd);
-        rc = check_hip(h, hipStreamEndCapture(h->stream, &h->graph), "hipStreamEndCapture");
+            rc = launch_iteration<double>(h, h->bd);
+        const int rc2 = check_hip(h, hipStreamEndCapture(h->stream, &h->graph), "hipStreamEndCapture");
+        if (rc) return rc;
+        rc = rc2;
         if (rc) return rc;
         rc = check_hip(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
         if (rc) return rc;
@@ -666,14 +686,14 @@ int run_step(spgo_solver* h, int iterations) {
     rc = check_hip(h, hipEventRecord(h->ev0, h->stream), "hipEventRecord");
     if (rc) return rc;
     for (int it = 0; it < iterations; ++it) {
-        if (h->opt.use_graph) {
+        if (graph) {
             rc = check_hip(h, hipGraphLaunch(h->gexec, h->stream), "hipGraphLaunch");
-            if (rc) return rc;
         } else if (mixed) {
-            launch_iteration<float>(h, h->bf);
+            rc = launch_iteration<float>(h, h->bf);
         } else {
-            launch_iteration<double>(h, h->bd);
+            rc = launch_iteration<double>(h, h->bd);
         }
+        if (rc) return rc;
     }
     rc = check_hip(h, hipEventRecord(h->ev1, h->stream), "hipEventRecord");
     if (rc) return rc;
@@ -784,6 +804,7 @@ int spgo_destroy(spgo_solver* h) {
     destroy_graph(h);
     h->mem.release();
     h->lps.release();
+    if (h->pcg_flag_host) (void)hipHostFree(h->pcg_flag_host);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1106,16 +1127,26 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
         if (mixed) launch_form_system<float>(h->lay, h->bf, h->ctl, h->H, h->gvec, s);
         else launch_form_system<double>(h->lay, h->bd, h->ctl, h->H, h->gvec, s);
         (void)hipEventRecord(ev[1], s);
-        if (h->pcg)
-            lp_solve_launch(h->lps, h->H, h->Pp, h->gvec, h->delta, h->net_params, h->opt.pcg_tol,
-                            h->opt.pcg_max_iter, h->ctl, s);
-        if (h->band) {
+        bool factor = true;
+        if (h->pcg) {
+            rc = check_hip(h, (hipError_t)lp_solve_launch(h->lps, h->H, h->Pp, h->gvec, h->delta, h->net_params,
+                                                          h->opt.pcg_tol, h->opt.pcg_max_iter, h->ctl, s),
+                           "cooperative CG launch");
+            if (!rc) rc = check_hip(h, hipMemcpyAsync(h->pcg_flag_host, &h->ctl->pcg_fail, sizeof(int),
+                                                      hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+            if (!rc) rc = check_hip(h, hipStreamSynchronize(s), "CG");
+            if (rc) break;
+            factor = *h->pcg_flag_host != 0;
+        }
+        if (!factor) {
+            (void)hipEventRecord(ev[2], s);
+        } else if (h->band) {
             launch_chol(h, s);
             (void)hipEventRecord(ev[2], s);
         } else {
             launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
             (void)hipEventRecord(ev[2], s);
-            launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, s);
+            launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, h->ctl, s);
         }
         (void)hipEventRecord(ev[3], s);
         if (mixed) launch_pose_update<float>(h->lay, make_camera(h), h->bf, h->ctl, h->delta, h->step_size, 0, s);

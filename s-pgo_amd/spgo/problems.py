@@ -294,6 +294,88 @@ CONFIGS = {
 }
 
 
+def make_long_problem(n_poses: int, n_landmarks: int, obs_per_landmark: int, seed: int,
+                      fx: float = 500.0, fy: float = 500.0, cx: float = 320.0, cy: float = 240.0,
+                      width: int = 640, height: int = 480, baseline: float = 0.12, radius: float = 2.0,
+                      depth_range=(1.0, 4.0), noise_px: float = 1.0, rot_perturb: float = 0.05,
+                      trans_perturb: float = 0.1, alpha_perturb: float = 0.1, min_depth: float = 0.2,
+                      name: str = "") -> Problem:
+    """Seeded stereo problem for long trajectories (memory linear in the
+    landmark count).  Same circle of ``n_poses + 1`` keyframes looking at the
+    origin as ``make_problem``; each landmark is drawn in its host frame's view
+    (pixel uniform in the image, depth uniform in ``depth_range``) and kept if
+    both cameras see it in the ``obs_per_landmark`` consecutive frames from the
+    host on.  Noise and initial perturbations as in ``make_problem``."""
+    rng = np.random.default_rng(seed)
+    N = int(n_poses)
+    F = N + 1
+    K = int(obs_per_landmark)
+    if K < 1 or K > F:
+        raise ValueError("obs_per_landmark must be in [1, n_poses + 1]")
+    theta = 2.0 * math.pi * np.arange(F) / F
+    G = np.stack([look_at_origin(np.array([radius * math.cos(a), radius * math.sin(a), 0.0])) for a in theta])
+    T_rl = np.eye(4)
+    T_rl[0, 3] = baseline
+    Ginv = _inv(G)
+    GRinv = _inv(G @ T_rl)
+
+    def proj(Tinv, P):
+        Pc = np.einsum('nij,nj->ni', Tinv[:, :3, :3], P) + Tinv[:, :3, 3]
+        z = Pc[:, 2]
+        zs = np.where(np.abs(z) < 1e-12, 1e-12, z)
+        return np.stack([fx * Pc[:, 0] / zs + cx, fy * Pc[:, 1] / zs + cy], -1), z
+
+    pts, hosts = [], []
+    need = int(n_landmarks)
+    while need > 0:
+        m = max(2 * need, 1024)
+        a = rng.integers(0, F - K + 1, size=m)
+        u = rng.uniform(0, width, size=m)
+        v = rng.uniform(0, height, size=m)
+        d = rng.uniform(depth_range[0], depth_range[1], size=m)
+        Pc = np.stack([(u - cx) / fx * d, (v - cy) / fy * d, d], -1)
+        P = np.einsum('nij,nj->ni', G[a, :3, :3], Pc) + G[a, :3, 3]
+        ok = np.ones(m, bool)
+        for k in range(K):
+            t = a + k
+            pl, zl = proj(Ginv[t], P)
+            pr, zr = proj(GRinv[t], P)
+            ok &= ((zl > min_depth) & (zr > min_depth) & (pl[:, 0] >= 0) & (pl[:, 0] < width)
+                   & (pl[:, 1] >= 0) & (pl[:, 1] < height) & (pr[:, 0] >= 0) & (pr[:, 0] < width)
+                   & (pr[:, 1] >= 0) & (pr[:, 1] < height))
+        idx = np.nonzero(ok)[0][:need]
+        pts.append(P[idx])
+        hosts.append(a[idx])
+        need -= idx.size
+    P = np.concatenate(pts)
+    host = np.concatenate(hosts).astype(np.int64)
+    order = np.argsort(host, kind="stable")
+    P, host = P[order], host[order]
+    L = P.shape[0]
+    meas_l = np.repeat(np.arange(L), K)
+    meas_a = np.repeat(host, K)
+    meas_t = meas_a + np.tile(np.arange(K), L)
+    pl, _ = proj(Ginv[meas_t], P[meas_l])
+    pr, _ = proj(GRinv[meas_t], P[meas_l])
+    uv = np.concatenate([pl, pr], 1) + rng.normal(0.0, noise_px, size=(meas_l.size, 4))
+    z_host = (np.einsum('nij,nj->ni', Ginv[host, :3, :3], P) + Ginv[host, :3, 3])[:, 2]
+    M = int(math.ceil(L / N))
+    alpha_true = np.ones(N * M)
+    alpha_true[:L] = 1.0 / z_host
+    X_true = Ginv[1:] @ G[:-1]
+    xi = np.concatenate([rng.normal(0.0, rot_perturb / math.sqrt(3.0), size=(N, 3)),
+                         rng.normal(0.0, trans_perturb / math.sqrt(3.0), size=(N, 3))], axis=1)
+    X0 = X_true @ _exp_se3(xi)
+    alpha0 = alpha_true.copy()
+    alpha0[:L] *= 1.0 + rng.uniform(-alpha_perturb, alpha_perturb, size=L)
+    intr = np.array([fx, fy, cx, cy, fx, fy, cx, cy], dtype=np.float32)
+    return Problem(n_poses=N, m_per_frame=M, intrinsics=intr, T_r_to_l=T_rl.reshape(16).astype(np.float32),
+                   meas_a=meas_a.astype(np.int32), meas_t=meas_t.astype(np.int32), meas_l=meas_l.astype(np.int32),
+                   uv=uv.astype(np.float32), X0=X0.astype(np.float32), alpha0=alpha0.astype(np.float32),
+                   X_true=X_true, alpha_true=alpha_true, n_landmarks=L,
+                   outlier=np.zeros(meas_l.size, dtype=bool), name=name)
+
+
 # Batch of independent sliding windows: keyframes per window uniform in 8..32,
 # landmarks uniform in 200..2000, observations per landmark uniform in 3..8.
 WINDOW_BATCH = dict(n_windows=512, n_range=(8, 32), l_range=(200, 2000), k_range=(3, 8), seed=4004)
