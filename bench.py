@@ -141,6 +141,10 @@ def main() -> int:
                          "FP32 assembly, 10 LM iterations per solve (a step is one batched solve); c5: 1024 "
                          "keyframes, 400k landmarks, 12 observations each, LM with CG preconditioned by the "
                          "trained network against the Cholesky path")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="c4: skip the one-window-at-a-time FP64 solver and the FP64 batch (profiling runs)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="c4: use the tile shape given by --chunk-landmarks/--group-chunks instead of the autotuner")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 256 MiB five times (counter calibration)")
     args = ap.parse_args()
@@ -324,24 +328,29 @@ def bench_c4(args) -> int:
     init = [(w.X0.astype(np.float64), w.alpha0.astype(np.float64)) for w in ws]
 
     # plain FP64 GPU solver, one window at a time (buffers built by a warm-up step)
-    fo = make_options(mode="lm")
-    lone = [solver_from_problem(w, fo) for w in ws]
-    for s in lone:
-        s.step(1)
-    for s, (X0, a0) in zip(lone, init):
-        s.set_state(X0, a0)
-    t0 = time.perf_counter()
-    for s in lone:
-        s.step(iters)
-    t_lone = time.perf_counter() - t0
-    lone_final = np.array([s.stats()[1][-1]["cost"] for s in lone])
-    lone_c0 = np.array([s.stats()[0] for s in lone])
-    del lone
+    t_lone = None
+    if not args.no_plain:
+        fo = make_options(mode="lm")
+        lone = [solver_from_problem(w, fo) for w in ws]
+        for s in lone:
+            s.step(1)
+        for s, (X0, a0) in zip(lone, init):
+            s.set_state(X0, a0)
+        t0 = time.perf_counter()
+        for s in lone:
+            s.step(iters)
+        t_lone = time.perf_counter() - t0
+        lone_final = np.array([s.stats()[1][-1]["cost"] for s in lone])
+        lone_c0 = np.array([s.stats()[0] for s in lone])
+        del lone
 
     def run_batch(precision):
-        o = make_options(mode="lm", precision=precision, use_graph=1)
+        kw = {}
+        if args.no_autotune:
+            kw = {"chunk_landmarks": args.chunk_landmarks or 64, "group_chunks": args.group_chunks or 2}
+        o = make_options(mode="lm", precision=precision, use_graph=1, **kw)
         bt = Batch([solver_from_problem(w, o) for w in ws], o)
-        tune = bt.autotune(3)
+        tune = dict(kw, ms=None) if args.no_autotune else bt.autotune(3)
         for _ in range(max(args.warmup, 1)):
             bt.reset_states()
             bt.step(iters)
@@ -361,14 +370,25 @@ def bench_c4(args) -> int:
     bt, tune, wall, dev_ms, final, c0, phases = run_batch("mixed")
     info = bt.info()
     value = args.steps * nw / wall
-    rel = np.abs(final - lone_final) / lone_final
-    _, tune64, wall64, dev64, final64, _, phases64 = run_batch("fp64")
-    rel64 = np.abs(final64 - lone_final) / lone_final
+    plain = {}
+    if t_lone is not None:
+        rel = np.abs(final - lone_final) / lone_final
+        _, tune64, wall64, dev64, final64, _, phases64 = run_batch("fp64")
+        rel64 = np.abs(final64 - lone_final) / lone_final
+        plain = {
+            "plain_fp64_gpu": {"value": nw / t_lone, "unit": "solves/s (one window at a time)",
+                               "ms_per_batch": t_lone * 1e3},
+            "speedup_vs_plain_fp64_gpu": value / (nw / t_lone),
+            "final_cost_rel_diff_vs_plain_fp64": {"max": float(rel.max()), "median": float(np.median(rel))},
+            "initial_cost_rel_diff": float(np.max(np.abs(c0 - lone_c0) / lone_c0)),
+            "batched_fp64": {"value": args.steps * nw / wall64, "device_ms_per_step": dev64, "autotune": tune64,
+                             "final_cost_rel_diff_max": float(rel64.max()), "phases_ms": phases64},
+        }
 
     nbytes = int(sum(assembly_bytes(w, 4) for w in ws))
     ach = nbytes / (phases["assemble"] * 1e-3) / 1e9
     roof = {"kernel": "k_assemble<float>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": ach / HBM_PEAK_GBS, "traffic": profile_traffic("k_assemble_batch"), "algorithmic_bytes": nbytes}
+            "frac": ach / HBM_PEAK_GBS, "traffic": profile_traffic("k_assemble_c4"), "algorithmic_bytes": nbytes}
     dom = max(phases, key=phases.get)
     if dom != "assemble":
         roof["note"] = f"dominant phase is {dom}; roofline given for the assembly kernel"
@@ -409,13 +429,7 @@ def bench_c4(args) -> int:
                    "measurements": info["n_measurements"], "groups": info["n_groups"],
                    "max_profile_depth": info["max_depth"]},
         "autotune": tune,
-        "plain_fp64_gpu": {"value": nw / t_lone, "unit": "solves/s (one window at a time)",
-                           "ms_per_batch": t_lone * 1e3},
-        "speedup_vs_plain_fp64_gpu": value / (nw / t_lone),
-        "final_cost_rel_diff_vs_plain_fp64": {"max": float(rel.max()), "median": float(np.median(rel))},
-        "initial_cost_rel_diff": float(np.max(np.abs(c0 - lone_c0) / lone_c0)),
-        "batched_fp64": {"value": args.steps * nw / wall64, "device_ms_per_step": dev64, "autotune": tune64,
-                         "final_cost_rel_diff_max": float(rel64.max()), "phases_ms": phases64},
+        **plain,
         "phases_ms": phases,
         "roofline": roof,
         "generator_s": t_gen,

@@ -132,7 +132,10 @@ __device__ __forceinline__ MeasIn meas_load(const Layout& lay, int ob0, int o, b
     return m;
 }
 
-template <typename T, int KT>
+// MC: some group has several chunks, so the Schur tiles accumulate across the
+// chunk loop in registers; with one chunk per group they live only from phase 3
+// on, which keeps them out of the register peak of phase 1
+template <typename T, int KT, bool MC>
 __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, StateBuf<T> B0, StateBuf<T> B1) {
     using SH = AsmShape<KT>;
     const Layout& lay = A.lay;
@@ -181,9 +184,11 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
     __shared__ double red[NWAVE];
     __shared__ double Adk[KT][36];              // Ad(G_a^-1 G_{a+k}) of the state being linearised
     __shared__ int soffL[KMAX + 2];             // slot offsets of the current chunk
+    __shared__ double alnew[CHUNK_LM];          // new inverse depths, stored in phase 2
+    __shared__ int lidl[CHUNK_LM];              // their slots
 
     // ---- group set-up: relative poses, adjoints, the update mapped to the anchor
-    // frame (run in the first chunk, once its measurement loads are in flight)
+    // frame
     auto group_setup = [&]() __attribute__((always_inline)) {
         for (int i = tid; i < (KT + 1) * 27; i += WG) (&S[0][0])[i] = (T)0;
         if (tid < 2 * (K + 1)) {
@@ -219,14 +224,17 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
     };
 
     typename Vec4<T>::type acc[SH::MAXTILE];
+    if (MC) {
 #pragma unroll
-    for (int i = 0; i < SH::MAXTILE; ++i) acc[i] = (T)0;
+        for (int i = 0; i < SH::MAXTILE; ++i) acc[i] = (T)0;
+    }
     double cost = 0.0;
     T gl_grad = (T)0;
 
     const T fx0 = (T)A.cam.fx[0], fy0 = (T)A.cam.fy[0], cx0 = (T)A.cam.cx[0], cy0 = (T)A.cam.cy[0];
     const T inv_param = A.rb.param != 0.0 ? (T)(1.0 / A.rb.param) : (T)0;
 
+    group_setup();
     for (int ci = chunk_begin; ci < chunk_end; ++ci) {
         const bool first = ci == chunk_begin;
         const int4 ch = first ? chunk0 : lay.chunk[ci];
@@ -251,7 +259,6 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
             lray = lay.lm_ray[lm0 + tid];
             lps = lay.lm_pos[lm0 + tid];
         }
-        if (first) group_setup();
         // ---- phase 0: back-substitution and anchor-frame point per landmark ----
         // the chunk's B columns (nl x 6K, contiguous) into LDS with coalesced loads
         const T* vin = bin.v + voff + (long long)(lm0 - glm0) * K6;
@@ -260,7 +267,7 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
             // loads in flight in batches of NB before their stores (two round trips
             // for spans up to 8); indices clamped so every load is unconditional
             constexpr int NV = (CHUNK_LM * 6 * KT + WG - 1) / WG;
-            constexpr int NBV = 6;
+            constexpr int NBV = sizeof(T) == 8 ? 4 : 6;   // FP64: fewer values in flight, no spills
 #pragma unroll
             for (int r0 = 0; r0 < NV; r0 += NBV) {
                 T vv[NBV];
@@ -278,12 +285,11 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
         }
         for (int i = tid; i < CHUNK_LM * (6 * KT + 1); i += WG) U[i] = (T)0;
         lds_barrier();   // also orders the set-up above
-        double al = 0.0;   // the landmark's new inverse depth, stored with C and g_alpha
         if (tid < nl) {
             const int act = lm0 + tid;
             const int id = lid;
             lpos[tid] = lps;
-            al = bin.alpha[id];
+            double al = bin.alpha[id];
             if (update) {
                 double dal = 0.0;
                 if (chol_ok) {
@@ -296,6 +302,9 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
                 }
                 al = fmin(fmax(al + dal, A.alpha_min), A.alpha_max);
             }
+            // kept in LDS until phase 2 stores it with C and g_alpha
+            alnew[tid] = al;
+            lidl[tid] = id;
             const float2 ray = lray;
             const double x = ((double)ray.x - A.cam.cx[0]) / A.cam.fx[0];
             const double y = ((double)ray.y - A.cam.cy[0]) / A.cam.fy[0];
@@ -452,7 +461,7 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
             const int act = lm0 + tid;
             const T Ct = C + (T)A.eps;
             const T isc = (T)1 / sqrt(Ct);
-            if (update) bout.alpha[lid] = al;
+            if (update) bout.alpha[lidl[tid]] = alnew[tid];
             bout.Ct[act] = Ct;
             bout.g[act] = G;
             iscl[tid] = isc;
@@ -476,6 +485,10 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
         lds_barrier();
 
         // ---- phase 3: [V; g]^T-products on MFMA: Y = V V^T and y = V g -----------
+        if (!MC) {
+#pragma unroll
+            for (int i = 0; i < SH::MAXTILE; ++i) acc[i] = (T)0;
+        }
         const int kend = (nl + 3) & ~3;
 #pragma unroll
         for (int ti = 0; ti < SH::MAXTILE; ++ti) {
@@ -584,10 +597,14 @@ __global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, Sta
 template <typename T>
 void launch_assemble(const AsmArgs& a, const StateBuf<T>* bufs, hipStream_t s) {
     if (a.lay.n_groups == 0) return;
-    if (a.lay.kmax <= 8)
-        hipLaunchKernelGGL((k_assemble<T, 8>), dim3(a.lay.n_groups), dim3(WG), 0, s, a, bufs[0], bufs[1]);
-    else
-        hipLaunchKernelGGL((k_assemble<T, KMAX>), dim3(a.lay.n_groups), dim3(WG), 0, s, a, bufs[0], bufs[1]);
+    const dim3 grid(a.lay.n_groups), block(WG);
+    if (a.lay.kmax <= 8) {
+        if (a.lay.multi_chunk) hipLaunchKernelGGL((k_assemble<T, 8, true>), grid, block, 0, s, a, bufs[0], bufs[1]);
+        else hipLaunchKernelGGL((k_assemble<T, 8, false>), grid, block, 0, s, a, bufs[0], bufs[1]);
+    } else {
+        if (a.lay.multi_chunk) hipLaunchKernelGGL((k_assemble<T, KMAX, true>), grid, block, 0, s, a, bufs[0], bufs[1]);
+        else hipLaunchKernelGGL((k_assemble<T, KMAX, false>), grid, block, 0, s, a, bufs[0], bufs[1]);
+    }
 }
 
 template void launch_assemble<double>(const AsmArgs&, const StateBuf<double>*, hipStream_t);

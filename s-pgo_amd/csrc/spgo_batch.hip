@@ -309,6 +309,8 @@ int setup(spgo_batch* b) {
     lay.n_chunks = (int)chunk_base;
     lay.n_obs = (int)obs_base;
     lay.kmax = std::max(b->max_span, 1);
+    lay.multi_chunk = 0;
+    for (const GroupDesc& d : C.grp_desc) lay.multi_chunk |= d.chunk_end - d.chunk_begin > 1;
     lay.n_probs = np;
     int rc;
     const ProbDesc* probs = nullptr;

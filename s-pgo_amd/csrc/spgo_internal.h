@@ -77,6 +77,7 @@ struct Layout {
     int N, F, L;                  // poses, frames, landmark slots
     int n_act;                    // active landmarks (with measurements)
     int n_groups, n_chunks, n_obs, kmax;
+    int multi_chunk;              // some group holds more than one chunk
     // measurements, chunk/slot-major order inside each group
     const int* ob_t;              // target frame
     const uint32_t* ob_meta;      // bits 0-15: slot-major -> landmark-major position in chunk,
