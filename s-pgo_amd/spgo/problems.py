@@ -376,6 +376,35 @@ def make_long_problem(n_poses: int, n_landmarks: int, obs_per_landmark: int, see
                    outlier=np.zeros(meas_l.size, dtype=bool), name=name)
 
 
+# --------------------------------------------------------------------------
+# on-disk format: one .npz of plain arrays (no pickled objects)
+# --------------------------------------------------------------------------
+_NPZ_FIELDS = ("meas_a", "meas_t", "meas_l", "uv", "intrinsics", "T_r_to_l", "X0", "alpha0", "X_true", "alpha_true")
+NPZ_VERSION = 1
+
+
+def save_problem(path: str, p: Problem, seed: Optional[int] = None) -> None:
+    """Write a problem as an .npz: measurement ids (int32 SoA), pixels (float32
+    O x 4: uL vL uR vR), calibration, initial and true states, sizes and the
+    generator seed.  ``load_problem`` reads it back without unpickling."""
+    arrays = {k: np.asarray(getattr(p, k)) for k in _NPZ_FIELDS}
+    arrays["outlier"] = np.asarray(p.outlier if p.outlier is not None else np.zeros(p.n_obs, bool))
+    arrays["sizes"] = np.array([NPZ_VERSION, p.n_poses, p.m_per_frame, p.n_landmarks,
+                                -1 if seed is None else int(seed)], np.int64)
+    arrays["name"] = np.frombuffer(p.name.encode(), np.uint8)
+    np.savez(path, **arrays)
+
+
+def load_problem(path: str) -> Problem:
+    with np.load(path, allow_pickle=False) as z:
+        sizes = z["sizes"]
+        if int(sizes[0]) != NPZ_VERSION:
+            raise ValueError(f"{path}: format version {int(sizes[0])}, expected {NPZ_VERSION}")
+        kw = {k: z[k] for k in _NPZ_FIELDS}
+        return Problem(n_poses=int(sizes[1]), m_per_frame=int(sizes[2]), n_landmarks=int(sizes[3]),
+                       outlier=z["outlier"], name=bytes(z["name"]).decode(), **kw)
+
+
 # Batch of independent sliding windows: keyframes per window uniform in 8..32,
 # landmarks uniform in 200..2000, observations per landmark uniform in 3..8.
 WINDOW_BATCH = dict(n_windows=512, n_range=(8, 32), l_range=(200, 2000), k_range=(3, 8), seed=4004)

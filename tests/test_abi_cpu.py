@@ -11,7 +11,7 @@ import pytest
 
 from spgo import _lib
 from spgo.problems import make_problem
-from spgo.solver import Solver, make_options
+from spgo.solver import Solver, make_options, solver_from_problem
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -176,3 +176,20 @@ def test_dropin_module_surface():
     assert cs.getInverseDepths().shape == (12,)
     with pytest.raises(ValueError, match="dtype"):
         cs.loadInverseDepths(np.ones(12))                       # float64 buffer, as Cython's float[:] check
+
+
+def test_problem_npz_round_trip_and_ingestion(tmp_path):
+    from spgo.problems import load_problem, make_problem, save_problem
+    p = make_problem(8, 300, 4, seed=17, outlier_frac=0.05, name="npz")
+    f = str(tmp_path / "p.npz")
+    save_problem(f, p, seed=17)
+    q = load_problem(f)
+    assert (q.n_poses, q.m_per_frame, q.n_landmarks, q.name) == (p.n_poses, p.m_per_frame, p.n_landmarks, "npz")
+    for k in ("meas_a", "meas_t", "meas_l", "uv", "intrinsics", "T_r_to_l", "X0", "alpha0", "X_true",
+              "alpha_true", "outlier"):
+        a, b = getattr(p, k), getattr(q, k)
+        assert a.dtype == b.dtype and np.array_equal(a, b), k
+    # the loaded problem ingests into the same host layout
+    a = solver_from_problem(p).layout_info()
+    b = solver_from_problem(q).layout_info()
+    assert a == b
