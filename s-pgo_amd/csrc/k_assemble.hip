@@ -136,7 +136,7 @@ __device__ __forceinline__ MeasIn meas_load(const Layout& lay, int ob0, int o, b
 // chunk loop in registers; with one chunk per group they live only from phase 3
 // on, which keeps them out of the register peak of phase 1
 template <typename T, int KT, bool MC>
-__global__ __launch_bounds__(WG, KT <= 8 ? 2 : 1) void k_assemble(AsmArgs A, StateBuf<T> B0, StateBuf<T> B1) {
+__global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 8 ? 2 : 1)) void k_assemble(AsmArgs A, StateBuf<T> B0, StateBuf<T> B1) {
     using SH = AsmShape<KT>;
     const Layout& lay = A.lay;
     const int g = blockIdx.x;

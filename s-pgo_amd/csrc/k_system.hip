@@ -32,21 +32,24 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 // ---------------------------------------------------------------------------
 constexpr int FS_MAXG = 512;   // groups whose span can reach one pose row
 
-template <typename T>
+// R rows per workgroup (1, or the 6 rows of one pose, which share the covering
+// list): one row per workgroup fills the device for small systems, one pose per
+// workgroup stages each list once when there are many rows
+template <typename T, int R>
 __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0, StateBuf<T> B1, Control* ctls,
                                                      double* H, double* gvec) {
-    // the covering groups of this row's pose block (precomputed per layout, anchors
-    // ascending) with the offset of row `row` inside each group's block
-    __shared__ long long rowoff[FS_MAXG];
-    __shared__ int canc[FS_MAXG], gdo[FS_MAXG];
-    const int row_g = blockIdx.x;                 // row of all problems' poses
+    // the covering groups of this pose block (precomputed per layout, anchors
+    // ascending); row r of a group's block starts at base + r K6
+    __shared__ long long base[FS_MAXG];
+    __shared__ int canc[FS_MAXG], k6s[FS_MAXG], gdo[FS_MAXG];
+    const int row_g0 = blockIdx.x * R;            // first row, of all problems' poses
     const int tid = threadIdx.x;
-    const int pg = row_g / 6;
+    const int pg = row_g0 / 6;
     const ProbDesc pd = lay.probs[lay.pose_prob[pg]];
     Control* ctl = ctls + lay.pose_prob[pg];
-    const int row = row_g - 6 * pd.pose_base;    // row inside its problem
+    const int row0 = row_g0 - 6 * pd.pose_base;  // first row inside its problem
     const int cbase = 6 * pd.pose_base;
-    if (row == 0 && tid == 0) {
+    if (row0 == 0 && tid == 0) {
         // reset for this iteration's factorisation: without a converged CG solve
         // the Cholesky computes the step
         ctl->chol_ok = 1;
@@ -59,14 +62,20 @@ __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0,
     for (int e = tid; e < nc; e += 256) {
         const FsEntry f = lay.fs_list[e0 + e];
         canc[e] = f.anchor;
-        rowoff[e] = f.D_off + (long long)(row_g - 6 * f.anchor) * f.K6 - 6 * f.anchor;
-        gdo[e] = f.d_off + row_g - 6 * f.anchor;
+        k6s[e] = f.K6;
+        base[e] = f.D_off - (long long)(6 * f.anchor) * f.K6 - 6 * f.anchor;
+        gdo[e] = f.d_off - 6 * f.anchor;
     }
     __syncthreads();
-    double* Hr = H + pd.H_off + (long long)row * pd.Pp;
-    // every lower entry of the row is written (zeros outside the profile), so a
-    // failed factorisation cannot leave values behind for the next one
-    for (int col = tid; col <= row; col += 256) {
+    // items (r, col), col <= row0 + r: every lower entry of the rows is written
+    // (zeros outside the profile), so a failed factorisation cannot leave values
+    // behind for the next one
+    const int items = R * (row0 + 1) + R * (R - 1) / 2;
+    for (int t = tid; t < items; t += 256) {
+        int r = 0;
+        while (r + 1 < R && t >= (r + 1) * (row0 + 1) + (r + 1) * r / 2) ++r;
+        const int col = t - (r * (row0 + 1) + r * (r - 1) / 2);
+        const int row = row0 + r, row_g = row_g0 + r;
         const int colg = col + cbase, bj = colg / 6;
         // covering groups with anchor <= bj form a prefix (anchors ascending):
         // their blocks reach column col
@@ -83,7 +92,8 @@ __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0,
         for (int eb = 0; eb < ne; eb += 16) {
             double v[16];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) v[i] = (eb + i < ne) ? (double)bs.D[rowoff[eb + i] + colg] : 0.0;
+            for (int i = 0; i < 16; ++i)
+                v[i] = (eb + i < ne) ? (double)bs.D[base[eb + i] + (long long)row_g * k6s[eb + i] + colg] : 0.0;
 #pragma unroll
             for (int i = 0; i < 16; i += 2) {
                 h0 += v[i];
@@ -94,17 +104,21 @@ __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0,
         if (col == row) {
             h += ctl->lambda;
             double gs = 0.0;
-            for (int q = 0; q < nc; ++q) gs += (double)bs.d[gdo[q]];
+            for (int q = 0; q < nc; ++q) gs += (double)bs.d[gdo[q] + row_g];
             gvec[pd.vec_off + row] = gs;
         }
-        Hr[col] = h;
+        H[pd.H_off + (long long)row * pd.Pp + col] = h;
     }
 }
 
 template <typename T>
 void launch_form_system(const Layout& lay, const StateBuf<T>* bufs, Control* ctls, double* H, double* g,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_form_system<T>, dim3(6 * lay.N), dim3(256), 0, s, lay, bufs[0], bufs[1], ctls, H, g);
+    if (6 * lay.N > 4096)
+        hipLaunchKernelGGL((k_form_system<T, 6>), dim3(lay.N), dim3(256), 0, s, lay, bufs[0], bufs[1], ctls, H, g);
+    else
+        hipLaunchKernelGGL((k_form_system<T, 1>), dim3(6 * lay.N), dim3(256), 0, s, lay, bufs[0], bufs[1], ctls, H,
+                           g);
 }
 template void launch_form_system<double>(const Layout&, const StateBuf<double>*, Control*, double*, double*,
                                          hipStream_t);

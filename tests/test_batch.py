@@ -171,3 +171,16 @@ def test_batch_autotune_keeps_results():
         assert c0 == pytest.approx(r0, rel=1e-12)
         # another tile shape changes only summation order
         assert recs[-1]["cost"] == pytest.approx(rrecs[-1]["cost"], rel=1e-9)
+
+
+@pytest.mark.gpu
+def test_batch_with_many_rows_matches_lone_solvers():
+    # more than 4096 pose rows: the reduced systems are formed one pose per
+    # workgroup, the lone solvers one row per workgroup
+    ws = make_window_batch(52, n_range=(12, 20), l_range=(150, 300), k_range=(3, 8), seed=66)
+    assert 6 * sum(w.n_poses for w in ws) > 4096
+    opts = make_options(mode="lm", factorization=2)
+    bt = Batch([solver_from_problem(w, opts) for w in ws], opts)
+    bt.step(5)
+    for p in (0, 17, 51):
+        _compare(bt, p, _lone(ws[p], opts, 5), f"rows-{p}")
