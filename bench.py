@@ -468,7 +468,8 @@ def bench_c5(args) -> int:
     a0 = p.alpha0.astype(np.float64)
     runs = {}
     for tag, kw, use_net in (("cholesky", {}, False), ("cg_block_jacobi", {"factorization": 4}, False),
-                             ("cg_learned", {"factorization": 4}, True)):
+                             ("cg_learned", {"factorization": 4}, True),
+                             ("cg_learned_mixed", {"factorization": 4, "precision": "mixed"}, True)):
         s = solver_from_problem(p, make_options(mode="lm", **kw))
         if use_net:
             s.set_preconditioner(net)
@@ -492,7 +493,12 @@ def bench_c5(args) -> int:
     ref = runs["cholesky"]["cost_final"]
     for r in runs.values():
         r["final_cost_rel_diff_vs_cholesky"] = abs(r["cost_final"] - ref) / ref
-    le = runs["cg_learned"]
+    # the headline form: the fastest learned-CG run that reaches the FP64
+    # Cholesky path's final cost to 1e-6
+    best = max((t for t in ("cg_learned", "cg_learned_mixed")
+                if runs[t]["final_cost_rel_diff_vs_cholesky"] <= 1e-6 and runs[t]["fallbacks"] == 0),
+               key=lambda t: runs[t]["value"], default="cg_learned")
+    le = runs[best]
     # CG kernel roofline: per CG iteration the band (N (W+1) 6x6 FP64 blocks), the
     # two preconditioner block rows and six vectors
     n_it = max(1, int(np.mean(le["cg_iterations"])))
@@ -503,7 +509,9 @@ def bench_c5(args) -> int:
         "metric": "LM iters/s & batched solves/s vs plain FP64 GPU solver at matched final cost",
         "value": le["value"], "unit": "LM iterations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": le["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (seeded long-trajectory generator: circle r=2 m, 1 px noise)",
+        "dtype": "f32 linearisation / f64 reduced system and CG" if best == "cg_learned_mixed" else "f64",
+        "form": best,
+        "data": "synthetic (seeded long-trajectory generator: circle r=2 m, 1 px noise)",
         "config": {"workload": "1024 keyframes (6144 x 6144 reduced system), 400000 landmarks, 12 stereo "
                                "observations each, LM with CG to 1e-8 preconditioned by the trained network, "
                                "Cholesky fallback",

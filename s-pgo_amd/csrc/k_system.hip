@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0,
 template <typename T>
 void launch_form_system(const Layout& lay, const StateBuf<T>* bufs, Control* ctls, double* H, double* g,
                         hipStream_t s) {
-    if (6 * lay.N > 4096)
+    if (lay.form_rows == 6)
         hipLaunchKernelGGL((k_form_system<T, 6>), dim3(lay.N), dim3(256), 0, s, lay, bufs[0], bufs[1], ctls, H, g);
     else
         hipLaunchKernelGGL((k_form_system<T, 1>), dim3(6 * lay.N), dim3(256), 0, s, lay, bufs[0], bufs[1], ctls, H,

@@ -311,6 +311,10 @@ int setup(spgo_batch* b) {
     lay.kmax = std::max(b->max_span, 1);
     lay.multi_chunk = 0;
     for (const GroupDesc& d : C.grp_desc) lay.multi_chunk |= d.chunk_end - d.chunk_begin > 1;
+    // many short rows: one pose (6 rows sharing a covering list) per workgroup
+    int max_pp = 0;
+    for (const ProbDesc& d : b->pd) max_pp = std::max(max_pp, d.Pp);
+    lay.form_rows = (6 * pose_base > 4096 && max_pp <= 512) ? 6 : 1;
     lay.n_probs = np;
     int rc;
     const ProbDesc* probs = nullptr;

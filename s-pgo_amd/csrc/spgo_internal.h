@@ -78,6 +78,7 @@ struct Layout {
     int n_act;                    // active landmarks (with measurements)
     int n_groups, n_chunks, n_obs, kmax;
     int multi_chunk;              // some group holds more than one chunk
+    int form_rows;                // reduced-system rows per workgroup of k_form_system (1 or 6)
     // measurements, chunk/slot-major order inside each group
     const int* ob_t;              // target frame
     const uint32_t* ob_meta;      // bits 0-15: slot-major -> landmark-major position in chunk,

@@ -375,6 +375,7 @@ int setup_layout(spgo_solver* h) {
     lay.kmax = L.kmax;
     lay.multi_chunk = 0;
     for (const GroupDesc& d : L.grp_desc) lay.multi_chunk |= d.chunk_end - d.chunk_begin > 1;
+    lay.form_rows = 1;   // one problem: one row per workgroup fills the device
     if ((rc = upload(h, L.ob_t, &lay.ob_t)) || (rc = upload(h, L.ob_meta, &lay.ob_meta)) ||
         (rc = upload(h, L.ob_uv, &lay.ob_uv)) || (rc = upload(h, L.lm_id, &lay.lm_id)) ||
         (rc = upload(h, L.lm_ray, &lay.lm_ray)) || (rc = upload(h, L.lm_pos, &lay.lm_pos)) ||
