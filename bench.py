@@ -347,7 +347,7 @@ def bench_c4(args) -> int:
     def run_batch(precision):
         kw = {}
         if args.no_autotune:
-            kw = {"chunk_landmarks": args.chunk_landmarks or 64, "group_chunks": args.group_chunks or 2}
+            kw = {"chunk_landmarks": args.chunk_landmarks or 64, "group_chunks": args.group_chunks or 1}
         o = make_options(mode="lm", precision=precision, use_graph=1, **kw)
         bt = Batch([solver_from_problem(w, o) for w in ws], o)
         tune = dict(kw, ms=None) if args.no_autotune else bt.autotune(3)

@@ -318,6 +318,10 @@ __device__ __forceinline__ void lds_barrier() {
 
 __host__ __device__ constexpr int env_window_slots(int D) { return (D + 3) * (D + 4) / 2; }
 
+// wavefronts that factor one block column: the diagonal block and up to three
+// blocks below it each
+__host__ __device__ constexpr int env_factor_waves(int m) { return m > 3 ? (m + 2) / 3 : 1; }
+
 // window slot of block (i, j) from the residues a = i mod W, b = j mod W
 __device__ __forceinline__ double* env_slot(double* win, int a, int b) {
     const int hi = a > b ? a : b, lo = a > b ? b : a;
@@ -346,10 +350,12 @@ __device__ __forceinline__ void env_tile(double* win, int ri, int rj, int rk, in
 }
 
 // Inverse of the lower-triangular diagonal factor held column-major (stride
-// EST) in LDS, with its inverse diagonal dinv: lane c < 16 forms column c of
-// L^-1 by forward substitution and stores it row-major to dst (16 x 16).
-__device__ __forceinline__ void env_invert(const double* Lc, const double* dinv, double* dst, int lane) {
-    const int c = lane & 15;
+// EST) in LDS, with its inverse diagonal dinv: every lane forms column
+// c = lane & 15 of W = L^-1 by forward substitution, lanes 0..15 write W
+// row-major over the factor, and every lane returns rows g + 4t
+// (g = lane >> 4, t = 0..3) of its column, wv[t] = W(g + 4t, c).
+__device__ __forceinline__ void env_invert(double* Lc, const double* dinv, double* wv, int lane) {
+    const int c = lane & 15, g = lane >> 4;
     double s[EB];
 #pragma unroll
     for (int r = 0; r < EB; ++r) s[r] = (r == c) ? 1.0 : 0.0;
@@ -359,11 +365,22 @@ __device__ __forceinline__ void env_invert(const double* Lc, const double* dinv,
 #pragma unroll
         for (int r = q + 1; r < EB; ++r) s[r] -= Lc[q * EST + r] * s[q];
     }
+    // no fences: the substitution's reads, these writes and the reads below
+    // may alias, so the compiler keeps their order, and LDS executes one
+    // wavefront's operations in order (fences here cost ~150 registers)
     if (lane < EB) {
 #pragma unroll
-        for (int r = 0; r < EB; ++r) dst[r * EB + c] = s[r];
+        for (int r = 0; r < EB; ++r) Lc[r * EST + c] = s[r];
     }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) wv[t] = Lc[(g + 4 * t) * EST + c];
 }
+
+// Backward-solve blocks in global memory are 16 x 16 and lane-major: lane
+// (c, g) = (lane & 15, lane >> 4) holds entries (g + 4t, c), t = 0..3, as four
+// consecutive doubles at 4 * lane.  That is the output layout of the f64
+// 16x16x4 MFMA and the operand layout of the backward solve.
+__device__ __forceinline__ int lm_index(int r, int c) { return 4 * (c + 16 * (r & 3)) + (r >> 2); }
 
 // Where a local block system lives in global memory: local block row i is
 // block row i of H (rev = 0), or block row nb - 1 - i with the system
@@ -382,6 +399,7 @@ __device__ __forceinline__ long long env_gaddr(const EnvSrc& s, int i, int j, in
 struct EnvCtx {
     double* win;    // env_window_slots(D) blocks
     double* ring;   // the last two diagonal factors, column-major
+    double* colb;   // per factoring wavefront: the diagonal rows' entries of each column (EB x EB)
     double* xs;     // right-hand side, then forward solution (EB * nloc)
     double* dv;     // 1 / L_ii (EB * nloc)
     int* mk;        // blocks below the diagonal per block column (nloc)
@@ -395,8 +413,8 @@ struct EnvCtx {
 };
 
 // Panel store: for slot s (one per factored block column), the EDMAX blocks
-// below the diagonal in the local system's orientation, row-major 16 x 16,
-// so the copy-out and the backward solve read and write contiguous rows.
+// Q_(k,i) = L_(k+i,k) L_kk^-1 below the diagonal in the local system's
+// orientation, lane-major 16 x 16.
 __device__ __forceinline__ double* env_panel(double* Lp, int slot, int i) {
     return Lp + ((long long)slot * EDMAX + (i - 1)) * EB * EB;
 }
@@ -472,20 +490,32 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
     if (w == 3 && D + 1 < nloc) prefetch(D + 1);
     __syncthreads();
 
-    // panel kp to global memory for the backward solve: the inverse of its
-    // diagonal factor and the blocks below it (one wavefront)
+    // panel kp to global memory for the backward solve (one wavefront): the
+    // inverse W of its diagonal factor and Q_i = L_(kp+i,kp) W, formed on the
+    // MFMA from the window blocks, both lane-major
     auto copy_out = [&](int kp, int rkp) {
         const int slot = env_wslot(c, kp);
-        env_invert(ring + (kp & 1) * EBLK, dv + EB * kp, winv + (long long)slot * EB * EB, lane);
+        double wv[4];
+        env_invert(ring + (kp & 1) * EBLK, dv + EB * kp, wv, lane);
+        double* wd = winv + (long long)slot * EB * EB + 4 * lane;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) wd[t] = wv[t];
         const int mp = mk[kp];
         for (int i = 1; i <= mp; ++i) {
-            const double* s = env_slot(win, wrap(rkp + i), rkp);
-            double* d = env_panel(Lp, slot, i);
+            // A(m, k) = L(m, g + 4s) from the window, B(k, n) = W(g + 4s, n)
+            const double* L = env_slot(win, wrap(rkp + i), rkp) + (lane & 15) * EST + (lane >> 4);
+            d4 acc = 0.0;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int e = lane + 64 * t, rr = e >> 4, cc = e & 15;
-                d[e] = s[rr * EST + cc];
-            }
+            for (int s4 = 0; s4 < 4; ++s4) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(L[4 * s4], wv[s4], acc, 0, 0, 0);
+            double* d = env_panel(Lp, slot, i) + 4 * lane;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) d[v] = acc[v];
+        }
+        // past the profile, up to the depth the backward solve reads: zeros
+        for (int i = mp + 1; i <= DT; ++i) {
+            double* d = env_panel(Lp, slot, i) + 4 * lane;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) d[v] = 0.0;
         }
     };
     // rest of column kp's trailing update: tiles (kp+i, kp+j), 2 <= j <= i <= mp,
@@ -505,33 +535,62 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
     int rk = 0;   // k mod W
     for (int k = 0; k < c.steps; ++k, rk = wrap(rk + 1)) {
         const int m = mk[k];
-        const int nf = m > 3 ? (m + 2) / 3 : 1;         // factoring wavefronts
+        const int nf = env_factor_waves(m);
         // ---------------- phase A
         if (w < nf) {
             const int bi = (lane < EB) ? 0 : 1 + 3 * w + ((lane - EB) >> 4);
             const int rr = lane & 15;
             const bool valid = bi <= m;
+            // lanes past the profile read the diagonal block instead: finite
+            // values that are never broadcast or stored
             const int bic = valid ? bi : 0;
             double a[EB];
             {
                 const double* s = env_slot(win, wrap(rk + bic), rk) + rr * EST;
 #pragma unroll
-                for (int cc = 0; cc < EB; ++cc) a[cc] = valid ? s[cc] : 0.0;
+                for (int cc = 0; cc < EB; ++cc) a[cc] = s[cc];
             }
-            double b = valid ? xs[EB * (k + bic) + rr] : 0.0;
+            double b = xs[EB * (k + bic) + rr];
+            double* cb = c.colb + w * EB * EB;
             double yv = 0.0, iv = 0.0;
-            bool bad = false;
             int rl = lane;
             asm volatile("" : "+v"(rl));
+            // Column j's entries u_q of the diagonal rows q > j reach every
+            // lane two ways: u_(j+1) by a lane read, the rest through LDS, read
+            // back as broadcasts and applied at the start of column j + 1, when
+            // they have arrived.  The pivot chain is kept short: pivot j + 1 is
+            // formed from t = A(j+1, j+1) before column j's update and u_(j+1)
+            // as t - (u_(j+1) / L_jj)^2, two operations after 1/L_jj.
+            double up[EB];
+            double scp = 0.0;
+            double piv = readlane_d(a[0], 0);
 #pragma unroll
             for (int j = 0; j < EB; ++j) {
+                if (j >= 1) {
+                    // applied here, not sunk into a chain in front of a later pivot
+#pragma unroll
+                    for (int q = j + 1; q < EB; ++q) {
+                        a[q] -= scp * up[q];
+                        asm volatile("" : "+v"(a[q]));
+                    }
+                }
                 // column j: u holds A(:, j) after the updates of columns < j.
                 // For lane j, u * inv = sqrt(pivot) and sc = 1, so the update
                 // clobbers only entries of rows <= j that are already final
                 // (their y_j and 1/L_jj are kept in yv, iv).
                 const double u = a[j];
-                const double piv = readlane_d(u, j);
-                bad |= !(piv > 0.0 && piv < __builtin_inf());
+                if (j + 2 < EB) {
+                    if (rl < EB) cb[EB * j + rl] = u;
+                    wave_lds_order();
+#pragma unroll
+                    for (int q = j + 2; q < EB; ++q) up[q] = cb[EB * j + q];
+                    wave_lds_order();
+                }
+                double t = 0.0, uq = 0.0;
+                if (j + 1 < EB) {
+                    t = readlane_d(a[j + 1], j + 1);
+                    uq = readlane_d(u, j + 1);
+                }
                 const double inv = rsqrt_fast(piv);
                 const double bj = readlane_d(b, j);
                 const double l = u * inv;
@@ -542,29 +601,28 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
                 }
                 a[j] = l;
                 b -= sc * bj;
-#pragma unroll
-                for (int q = j + 1; q < EB; ++q) a[q] -= sc * readlane_d(u, q);
+                if (j + 1 < EB) {
+                    a[j + 1] -= sc * uq;
+                    const double x = uq * inv;
+                    piv = fma(-x, x, t);
+                }
+                scp = sc;
             }
+            // a pivot outside (0, inf) leaves 1/L_jj outside it too
+            const bool bad = rl < EB && !(iv > 0.0 && iv < __builtin_inf());
             // the panel stays in LDS (the diagonal factor column-major in the
             // ring); wavefront 3 copies it to global memory in the next phase A,
             // so this wavefront never waits on global stores
-            if (valid) {
-                if (bi == 0) {
-                    if (w == 0) {
-                        double* rg = ring + (k & 1) * EBLK + rr;
+            if (valid && (bi != 0 || w == 0)) {
+                const bool dg = bi == 0;
+                double* dst = dg ? ring + (k & 1) * EBLK + rr : env_slot(win, wrap(rk + bi), rk) + rr * EST;
+                const int st = dg ? EST : 1, zc = dg ? rr : EB;
 #pragma unroll
-                        for (int cc = 0; cc < EB; ++cc) rg[cc * EST] = (cc <= rr) ? a[cc] : 0.0;
-                        dv[EB * k + rr] = iv;
-                        xs[EB * k + rr] = yv;
-                    }
-                } else {
-                    double* wl = env_slot(win, wrap(rk + bi), rk) + rr * EST;
-#pragma unroll
-                    for (int cc = 0; cc < EB; ++cc) wl[cc] = a[cc];
-                    xs[EB * (k + bi) + rr] = b;
-                }
+                for (int cc = 0; cc < EB; ++cc) dst[cc * st] = (cc > zc) ? 0.0 : a[cc];
+                if (dg) dv[EB * k + rr] = iv;
+                xs[EB * (k + bi) + rr] = dg ? yv : b;
             }
-            if (bad && lane == 0) *sbad = 1;
+            if (bad) *sbad = 1;
         } else {
             // Helper wavefronts nf .. 3.  Wavefront 3 moves the prefetched block
             // row k + D + 1 into its slot (row k - 2's, free since the last phase
@@ -605,74 +663,86 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
 }
 
 // Backward solve of the first `steps` block rows of a local system by one
-// wavefront, from the bottom: xs holds the forward solution of those rows and
-// the final solution of the rows after them.  Lane (c, g) sums rows g, g+4,
-// g+8, g+12 of column c over the blocks below the diagonal; z = y_k - that
-// sum goes through LDS so each lane can form its share of x_k = L_kk^-T z
-// from its four entries of column c of L_kk^-1.  The next block's loads are
-// issued PD blocks ahead (the panels come from other compute units' caches).
-template <int DT>
+// wavefront, from the bottom: xs holds the forward solution y of those rows and
+// the final solution of the rows after them.  With W_k = L_kk^-1 and
+// Q_(k,i) = L_(k+i,k) W_k,
+//     x_k = W_k^T y_k - sum_(i>=1) Q_(k,i)^T x_(k+i).
+// Lane (c, g) sums entries g, g+4, g+8, g+12 of column c of those blocks; a
+// sum over the four rows of the wavefront gives x_k(c).  Only the x_(k+1) term
+// is on the sequential chain: the rest of block k - 1's sum is formed while
+// block k finishes.  Each block's loads are issued PD blocks ahead (PD
+// buffers of 4 (DT + 1) doubles per lane).
+template <int DT, int PD>
 __device__ void env_backward(const EnvCtx c, const double* winv, const double* Lp, int lane) {
     const int cl = lane & 15, g4 = lane >> 4;
     const int* mk = c.mk;
     double* xs = c.xs;
-    constexpr int PD = DT <= 3 ? 3 : 2;    // blocks whose loads are in flight
-    double pl[PD][4 * DT], wl[PD][4];
-    auto load = [&](int k, double* P, double* Wc) {
-        const int kc = max(k, 0);
-        const int m = mk[kc];
-        const int slot = env_wslot(c, kc);
+    constexpr int NQ = 4 * (DT + 1);       // W_k, then Q_(k,1..DT)
+    double qb[PD][NQ];
+    auto load = [&](int k, double* Q) {
+        const int slot = env_wslot(c, max(k, 0));
+        const double* Wk = winv + (long long)slot * EB * EB + 4 * lane;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) Q[t] = Wk[t];
+        // blocks past the profile were stored as zeros
 #pragma unroll
         for (int i = 1; i <= DT; ++i) {
-            const double* Pk = env_panel((double*)Lp, slot, max(min(i, m), 1));
+            const double* Pk = env_panel((double*)Lp, slot, i) + 4 * lane;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) P[4 * (i - 1) + t] = Pk[(g4 + 4 * t) * EB + cl];
+            for (int t = 0; t < 4; ++t) Q[4 * i + t] = Pk[t];
         }
-        const double* Wk = winv + (long long)slot * EB * EB;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) Wc[t] = Wk[(g4 + 4 * t) * EB + cl];
     };
-    auto solve = [&](int k, const double* P, const double* Wc) {
-        const int m = mk[k];
-        double s = 0.0;
+    // entries g + 4t of row block r of xs, the row clamped to the local system
+    // (its zero blocks meet finite values there)
+    const int nl1 = c.nloc - 1;
+    auto rows = [&](int r, double* v) {
+        const double* xr = xs + EB * min(r, nl1) + g4;
 #pragma unroll
-        for (int i = 1; i <= DT; ++i) {
-            if (i <= m) {
+        for (int t = 0; t < 4; ++t) v[t] = xr[4 * t];
+    };
+    // block k's partial sum without the x_(k+1) term, four accumulators
+    auto offchain = [&](int k, const double* Q) {
+        double v[4], a[4];
+        rows(k, v);
 #pragma unroll
-                for (int t = 0; t < 4; ++t) s += P[4 * (i - 1) + t] * xs[EB * (k + i) + g4 + 4 * t];
-            }
+        for (int t = 0; t < 4; ++t) a[t] = Q[t] * v[t];
+#pragma unroll
+        for (int i = 2; i <= DT; ++i) {
+            rows(k + i, v);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) a[t] = fma(-Q[4 * i + t], v[t], a[t]);
         }
-        s += __shfl_xor(s, 16, 64);
-        s += __shfl_xor(s, 32, 64);
-        const double z = xs[EB * k + cl] - s;
+        return (a[0] + a[1]) + (a[2] + a[3]);
+    };
+    // x_k from its partial sum p, and the partial sum of block k - 1
+    auto step = [&](int k, const double* Q, double p, const double* Qn) {
+        double v[4];
+        rows(k + 1, v);
+        const double pn = k >= 1 ? offchain(k - 1, Qn) : 0.0;
+        const double ch = fma(Q[4], v[0], Q[5] * v[1]) + fma(Q[6], v[2], Q[7] * v[3]);
+        const double xk = row_sum4(p - ch);
         wave_lds_order();
-        if (lane < EB) xs[EB * k + cl] = z;
+        if (lane < EB) xs[EB * k + cl] = xk;
         wave_lds_order();
-        double xv = 0.0;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) xv += Wc[t] * xs[EB * k + g4 + 4 * t];
-        xv += __shfl_xor(xv, 16, 64);
-        xv += __shfl_xor(xv, 32, 64);
-        wave_lds_order();
-        if (lane < EB) xs[EB * k + cl] = xv;
-        wave_lds_order();
+        return pn;
     };
     if (c.steps <= 0) return;
     // buffer q holds block k - q; after block k - q is solved it is refilled
     // with block k - q - PD
 #pragma unroll
-    for (int q = 0; q < PD; ++q) load(c.steps - 1 - q, pl[q], wl[q]);
+    for (int q = 0; q < PD; ++q) load(c.steps - 1 - q, qb[q]);
     int k = c.steps - 1;
+    double p = offchain(k, qb[0]);
     for (; k >= PD - 1; k -= PD) {
 #pragma unroll
         for (int q = 0; q < PD; ++q) {
-            solve(k - q, pl[q], wl[q]);
-            load(k - q - PD, pl[q], wl[q]);
+            p = step(k - q, qb[q], p, qb[(q + 1) % PD]);
+            load(k - q - PD, qb[q]);
         }
     }
 #pragma unroll
     for (int q = 0; q < PD; ++q)
-        if (k - q >= 0) solve(k - q, pl[q], wl[q]);
+        if (k - q >= 0) p = step(k - q, qb[q], p, qb[(q + 1) % PD]);
 }
 
 // LDS carve-up of one context (window for depth D, nloc block rows)
@@ -680,7 +750,8 @@ __device__ __forceinline__ EnvCtx env_ctx(double* base, int D, int nloc, double*
     EnvCtx c;
     c.win = base;
     c.ring = c.win + env_window_slots(D) * EBLK;
-    c.xs = c.ring + 2 * EBLK;
+    c.colb = c.ring + 2 * EBLK;
+    c.xs = c.colb + env_factor_waves(D) * EB * EB;
     c.dv = c.xs + EB * nloc;
     c.mk = (int*)(c.dv + EB * nloc);
     c.fst = c.mk + nloc;
@@ -696,13 +767,14 @@ __device__ __forceinline__ EnvCtx env_ctx(double* base, int D, int nloc, double*
 }
 
 __host__ __device__ inline size_t env_ctx_bytes(int D, int nloc) {
-    return (size_t)(env_window_slots(D) + 2) * EBLK * sizeof(double) + (size_t)2 * EB * nloc * sizeof(double) +
+    return (size_t)(env_window_slots(D) + 2) * EBLK * sizeof(double) +
+           (size_t)env_factor_waves(D) * EB * EB * sizeof(double) + (size_t)2 * EB * nloc * sizeof(double) +
            (size_t)((nloc + 1) & ~1) * 2 * sizeof(int);
 }
 
 // ---- one workgroup per problem: its whole system ---------------------------
 template <int DT>
-__global__ __launch_bounds__(256) void k_chol_env(const ProbDesc* probs, double* Hb, const int* firstb,
+__global__ __launch_bounds__(256, 2) void k_chol_env(const ProbDesc* probs, double* Hb, const int* firstb,
                                                   const double* gb, double* xb, double* winvb, double* Lpb,
                                                   Control* ctls) {
     extern __shared__ double lds[];
@@ -728,7 +800,7 @@ __global__ __launch_bounds__(256) void k_chol_env(const ProbDesc* probs, double*
     const EnvSrc src{H, Pp, nb, 0};
     env_forward<DT>(src, c, winv, Lp, &sbad);
     __syncthreads();   // the factor and the inverses in global memory are complete
-    if (w == 0) env_backward<DT>(c, winv, Lp, lane);
+    if (w == 0) env_backward<DT, 2>(c, winv, Lp, lane);
     __syncthreads();
     for (int i = tid; i < Pp; i += 256) x[i] = c.xs[i];
     if (tid == 0 && sbad) ctl->chol_ok = 0;
@@ -844,7 +916,7 @@ __global__ __launch_bounds__(256) void k_chol_split2(int Pp, int D, int s0, int 
     env_counts(ct.fst, nT, D, ct.mk);
     env_counts(cb.fst, nB, D, cb.mk);
     __syncthreads();
-    if (w == 0) env_backward<DT>(cs, winvS, LpS, lane);
+    if (w == 0) env_backward<DT, DT <= 3 ? 4 : 2>(cs, winvS, LpS, lane);
     __syncthreads();
     // the two parts: forward solutions of their rows, the separator's solution after them
     for (int i = tid; i < EB * s0; i += 256) ct.xs[i] = x[i];
@@ -854,8 +926,8 @@ __global__ __launch_bounds__(256) void k_chol_split2(int Pp, int D, int s0, int 
         cb.xs[EB * (nb - s1) + t] = cs.xs[PS - 1 - t];
     }
     __syncthreads();
-    if (w == 0) env_backward<DT>(ct, winv, Lp, lane);
-    if (w == 1) env_backward<DT>(cb, winv, Lp, lane);
+    if (w == 0) env_backward<DT, DT <= 3 ? 4 : 2>(ct, winv, Lp, lane);
+    if (w == 1) env_backward<DT, DT <= 3 ? 4 : 2>(cb, winv, Lp, lane);
     __syncthreads();
     for (int i = tid; i < EB * s0; i += 256) x[i] = ct.xs[i];
     for (int t = tid; t < PS; t += 256) x[EB * s0 + t] = cs.xs[t];

@@ -124,6 +124,32 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+// Sum over the four 16-lane rows of a wavefront (lanes l, l^16, l^32, l^48),
+// the same in every lane, without LDS: v_permlane16_swap(v, v) returns the
+// values of rows (2i, 2i+1) of the lane's row pair, v_permlane32_swap(v, v)
+// those of the lower and upper half.  The additions are (r0 + r1) + (r2 + r3),
+// as with exchanges over 16 then 32 lanes.
+__device__ __forceinline__ float row_sum4(float v) {
+    const unsigned u = __float_as_uint(v);
+    const auto p = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    const float s = __uint_as_float(p[0]) + __uint_as_float(p[1]);
+    const unsigned w = __float_as_uint(s);
+    const auto q = __builtin_amdgcn_permlane32_swap(w, w, false, false);
+    return __uint_as_float(q[0]) + __uint_as_float(q[1]);
+}
+
+__device__ __forceinline__ double row_sum4(double v) {
+    auto mk = [](unsigned lo, unsigned hi) { return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo)); };
+    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    auto pl = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
+    auto ph = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+    const double s = mk(pl[0], ph[0]) + mk(pl[1], ph[1]);
+    b = (unsigned long long)__double_as_longlong(s);
+    pl = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+    ph = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+    return mk(pl[0], ph[0]) + mk(pl[1], ph[1]);
+}
+
 __device__ __forceinline__ int wave_min_int(int v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
