@@ -291,6 +291,7 @@ void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t 
 // block columns.
 // ---------------------------------------------------------------------------
 constexpr int EB = 16;            // block size: one MFMA tile
+typedef __attribute__((address_space(3))) double lds_double;
 constexpr int EDMAX = 7;          // deepest profile: blocks below the diagonal
 constexpr int EST = EB + 1;       // LDS row stride of a window block
 constexpr int EBLK = EB * EST;    // doubles per window block
@@ -551,7 +552,10 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
                 for (int cc = 0; cc < EB; ++cc) a[cc] = s[cc];
             }
             double b = xs[EB * (k + bic) + rr];
-            double* cb = c.colb + w * EB * EB;
+            // the column buffer as its own LDS base, so its accesses carry
+            // immediate offsets
+            lds_double* cb = (lds_double*)(c.colb + w * EB * EB);
+            asm volatile("" : "+v"(cb));
             double yv = 0.0, iv = 0.0;
             int rl = lane;
             asm volatile("" : "+v"(rl));
@@ -577,7 +581,8 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
                 // column j: u holds A(:, j) after the updates of columns < j.
                 // For lane j, u * inv = sqrt(pivot) and sc = 1, so the update
                 // clobbers only entries of rows <= j that are already final
-                // (their y_j and 1/L_jj are kept in yv, iv).
+                // (their y_j and 1/L_jj are kept in yv, iv; a select, not a
+                // store by one lane, which would split the loop into blocks).
                 const double u = a[j];
                 if (j + 2 < EB) {
                     if (rl < EB) cb[EB * j + rl] = u;
