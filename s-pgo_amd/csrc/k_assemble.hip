@@ -539,43 +539,34 @@ __global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 8 ? 2 
         }
     }
     lds_barrier();
-    // to incremental-pose coordinates, in place: D(i, j) <- Ad_i^T D(i, j) Ad_j with
-    // Ad_k = Ad(G_a^-1 G_{a+k}); first every row times the block-diagonal Ad ...
-    for (int it = tid; it < K6 * K; it += WG) {
-        const int row = it / K, bj = it - row * K;
-        T* d6 = Dl + row * K6 + 6 * bj;
-        double in[6];
+    // to incremental-pose coordinates, straight to the reduced blocks:
+    // D(i, j) <- Ad_i^T D(i, j) Ad_j with Ad_k = Ad(G_a^-1 G_{a+k}), for the
+    // lower blocks i >= j (the reduced system reads only those).  One thread
+    // forms row r of one block: t = (column r of Ad_i)^T D(i, j), then t Ad_j
+    // (6 K (K + 1) / 2 rows: more than one per thread for spans past 10).
+    T* Dg = bout.D + D_off;
+    const int nlow = K * (K + 1) / 2;
+    for (int it = tid; it < 6 * nlow; it += WG) {
+        const int blk = it / 6, r = it - 6 * blk;
+        int bi = 0;
+        while ((bi + 1) * (bi + 2) / 2 <= blk) ++bi;
+        const int bj = blk - bi * (bi + 1) / 2;
+        const T* Db = Dl + 6 * bi * K6 + 6 * bj;
+        double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int q = 0; q < 6; ++q) in[q] = (double)d6[q];
+        for (int p = 0; p < 6; ++p) {
+            const double a = Adk[bi][6 * p + r];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) t[q] += a * (double)Db[p * K6 + q];
+        }
+        T* out = Dg + (6 * bi + r) * K6 + 6 * bj;
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             double s = 0.0;
 #pragma unroll
-            for (int q = 0; q < 6; ++q) s += in[q] * Adk[bj][6 * q + c];
-            d6[c] = (T)s;
+            for (int q = 0; q < 6; ++q) s += t[q] * Adk[bj][6 * q + c];
+            out[c] = (T)s;
         }
-    }
-    lds_barrier();
-    // ... then the transposed block-diagonal Ad times every column
-    for (int it = tid; it < K * K6; it += WG) {
-        const int bi = it / K6, col = it - bi * K6;
-        double in[6];
-#pragma unroll
-        for (int p = 0; p < 6; ++p) in[p] = (double)Dl[(6 * bi + p) * K6 + col];
-#pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            double s = 0.0;
-#pragma unroll
-            for (int p = 0; p < 6; ++p) s += Adk[bi][6 * p + r] * in[p];
-            Dl[(6 * bi + r) * K6 + col] = (T)s;
-        }
-    }
-    lds_barrier();
-    // lower blocks out (the reduced system reads block rows i >= j)
-    T* Dg = bout.D + D_off;
-    for (int it = tid; it < K6 * K6; it += WG) {
-        const int row = it / K6, col = it - row * K6;
-        if (row / 6 >= col / 6) Dg[it] = Dl[it];
     }
     if (tid < K6) {
         const int k = tid / 6, r = tid % 6;
