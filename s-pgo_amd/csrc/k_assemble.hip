@@ -428,17 +428,17 @@ __global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 8 ? 2 
                 // then 27 lanes each add 32 partials
                 T* scr = Vsh + wv * SCRW * 27;
 #pragma unroll
-                for (int q = 0; q < 27; ++q) pr[q] += __shfl_xor(pr[q], 32, 64);
+                for (int q = 0; q < 27; ++q) pr[q] = half_sum(pr[q]);
                 if (lane < 32) {
 #pragma unroll
                     for (int q = 0; q < 27; ++q) scr[lane * 27 + q] = pr[q];
                 }
                 wave_order();
                 if (lane < 27) {
-                    T s = (T)0;
+                    T s4[4] = {(T)0, (T)0, (T)0, (T)0};
 #pragma unroll
-                    for (int jj = 0; jj < SCRW; ++jj) s += scr[jj * 27 + lane];
-                    S[seg][lane] += s;
+                    for (int jj = 0; jj < SCRW; ++jj) s4[jj & 3] += scr[jj * 27 + lane];
+                    S[seg][lane] += (s4[0] + s4[1]) + (s4[2] + s4[3]);
                 }
                 wave_order();
             }
