@@ -136,7 +136,7 @@ __device__ __forceinline__ MeasIn meas_load(const Layout& lay, int ob0, int o, b
 // chunk loop in registers; with one chunk per group they live only from phase 3
 // on, which keeps them out of the register peak of phase 1
 template <typename T, int KT, bool MC>
-__global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 8 ? 2 : 1)) void k_assemble(AsmArgs A, StateBuf<T> B0, StateBuf<T> B1) {
+__global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 2 : 1)) void k_assemble(AsmArgs A, StateBuf<T> B0, StateBuf<T> B1) {
     using SH = AsmShape<KT>;
     const Layout& lay = A.lay;
     const int g = blockIdx.x;
@@ -592,6 +592,9 @@ void launch_assemble(const AsmArgs& a, const StateBuf<T>* bufs, hipStream_t s) {
     if (a.lay.kmax <= 8) {
         if (a.lay.multi_chunk) hipLaunchKernelGGL((k_assemble<T, 8, true>), grid, block, 0, s, a, bufs[0], bufs[1]);
         else hipLaunchKernelGGL((k_assemble<T, 8, false>), grid, block, 0, s, a, bufs[0], bufs[1]);
+    } else if (a.lay.kmax <= 12) {
+        if (a.lay.multi_chunk) hipLaunchKernelGGL((k_assemble<T, 12, true>), grid, block, 0, s, a, bufs[0], bufs[1]);
+        else hipLaunchKernelGGL((k_assemble<T, 12, false>), grid, block, 0, s, a, bufs[0], bufs[1]);
     } else {
         if (a.lay.multi_chunk) hipLaunchKernelGGL((k_assemble<T, KMAX, true>), grid, block, 0, s, a, bufs[0], bufs[1]);
         else hipLaunchKernelGGL((k_assemble<T, KMAX, false>), grid, block, 0, s, a, bufs[0], bufs[1]);
