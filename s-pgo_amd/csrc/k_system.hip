@@ -88,12 +88,17 @@ __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0,
         const int ne = lo;
         double h0 = 0.0, h1 = 0.0;
         // loads in batches of 16 before their adds, so a long list is a few round
-        // trips; even entries into h0, odd into h1
+        // trips; even entries into h0, odd into h1.  Every load is issued (past the
+        // list it repeats the last entry, weighted by zero): a conditional load
+        // waits for its data before the next one starts.
         for (int eb = 0; eb < ne; eb += 16) {
             double v[16];
 #pragma unroll
-            for (int i = 0; i < 16; ++i)
-                v[i] = (eb + i < ne) ? (double)bs.D[base[eb + i] + (long long)row_g * k6s[eb + i] + colg] : 0.0;
+            for (int i = 0; i < 16; ++i) {
+                const int e = min(eb + i, ne - 1);
+                const T x = bs.D[base[e] + (long long)row_g * k6s[e] + colg];
+                v[i] = (eb + i < ne) ? (double)x : 0.0;
+            }
 #pragma unroll
             for (int i = 0; i < 16; i += 2) {
                 h0 += v[i];
