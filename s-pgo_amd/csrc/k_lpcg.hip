@@ -550,35 +550,56 @@ __global__ void k_lp_band_from_dense(int N, int W, const double* H, int Pp, doub
 // ---- PCG on one large system: every workgroup of a cooperative launch --------
 // Work item e = 6 i + a (node i, row a) in grid-stride loops; global sums are
 // block partials in part[slot][block], summed by every block in the same order
-// after a grid barrier, so all blocks take the same decisions.  Slots rotate
-// so that a slot is rewritten only after two more barriers.
+// after a grid barrier, so all blocks take the same decisions.  The pair sum
+// uses slots 0 and 1, the p.q sum slots 2 and 3: a block writes a slot again
+// only after passing the next barrier, which every reader of it has passed.
 __global__ __launch_bounds__(256) void k_lp_pcg_coop(int T, int W, const double* Hb, const double* Gd,
                                                      const double* Go, const double* g, double* x, double* r,
                                                      double* z, double* p, double* q, double* tmp, double* part,
                                                      double tol, int maxit, Control* ctl, const int* bad) {
+    // Two grid barriers per iteration.  After p.q gives alpha, every thread
+    // forms the new residual of the block rows it needs (i - 1 .. i + 1) from
+    // the old one and q, and with it z = G^T G r locally: no barrier inside the
+    // preconditioner.  One sum gives r.r and r.z.  Then p = z + beta p and
+    // q = H z + beta q (= H p): z was complete before that sum's barrier, so
+    // the next product needs none of its own.  r is double-buffered (the other
+    // copy lives in tmp) because neighbours read the old residual while its
+    // owner writes the new one.
     namespace cgs = cooperative_groups;
     cgs::grid_group grid = cgs::this_grid();
-    __shared__ double red[256];
-    const int tid = threadIdx.x, nblk = gridDim.x;
+    __shared__ double red[2][4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nblk = gridDim.x;
     const int e0 = blockIdx.x * 256 + tid, es = nblk * 256, E = 6 * T;
-    auto block_sum = [&](double v) {
-        red[tid] = v;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (tid < w) red[tid] += red[tid + w];
-            __syncthreads();
+    // two block sums, the same in every thread
+    auto block_sum2 = [&](double& a, double& b) {
+        a = wave_sum(a);
+        b = wave_sum(b);
+        if (lane == 0) {
+            red[0][wv] = a;
+            red[1][wv] = b;
         }
-        const double s = red[0];
         __syncthreads();
-        return s;
+        a = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        b = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+        __syncthreads();
     };
-    auto grid_sum = [&](double v, int slot) {
-        const double b = block_sum(v);
-        if (tid == 0) part[slot * nblk + blockIdx.x] = b;
+    // grid sums: block partials to slots s, s + 1 of part, one grid barrier,
+    // then every block adds all partials in the same order (so all blocks
+    // hold the same alpha and beta)
+    auto grid_sum2 = [&](double& a, double& b, int slot) {
+        block_sum2(a, b);
+        if (tid == 0) {
+            part[slot * nblk + blockIdx.x] = a;
+            part[(slot + 1) * nblk + blockIdx.x] = b;
+        }
         grid.sync();
-        double s = 0.0;
-        for (int k = tid; k < nblk; k += 256) s += part[slot * nblk + k];
-        return block_sum(s);
+        a = 0.0;
+        b = 0.0;
+        for (int k = tid; k < nblk; k += 256) {
+            a += part[slot * nblk + k];
+            b += part[(slot + 1) * nblk + k];
+        }
+        block_sum2(a, b);
     };
     // y_i[a] = (H v)_i[a] over the band
     auto hrow = [&](int e, const double* v) {
@@ -598,73 +619,109 @@ __global__ __launch_bounds__(256) void k_lp_pcg_coop(int T, int W, const double*
         }
         return y;
     };
-    auto prec = [&]() {   // tmp = G r, then z = G^T tmp
-        for (int e = e0; e < E; e += es) {
-            const int i = e / 6, a = e - 6 * i;
-            double s = 0.0;
-            for (int b = 0; b < 6; ++b) s += Gd[(long long)i * 36 + a * 6 + b] * r[6LL * i + b];
-            if (Go && i > 0)
-                for (int b = 0; b < 6; ++b) s += Go[(long long)i * 36 + a * 6 + b] * r[6LL * (i - 1) + b];
-            tmp[e] = s;
+    // block row i of G v, v given element-wise by vv(j, b)
+    auto gblock = [&](int i, auto&& vv, double* t) {
+        double vi[6], vm[6];
+        const bool off = Go != nullptr && i > 0;
+        const int im = off ? i - 1 : i;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            vi[b] = vv(i, b);
+            vm[b] = vv(im, b);
         }
-        grid.sync();
-        for (int e = e0; e < E; e += es) {
-            const int i = e / 6, a = e - 6 * i;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
             double s = 0.0;
-            for (int b = 0; b < 6; ++b) s += Gd[(long long)i * 36 + b * 6 + a] * tmp[6LL * i + b];
-            if (Go && i + 1 < T)
-                for (int b = 0; b < 6; ++b) s += Go[(long long)(i + 1) * 36 + b * 6 + a] * tmp[6LL * (i + 1) + b];
-            z[e] = s;
+#pragma unroll
+            for (int b = 0; b < 6; ++b) s += Gd[(long long)i * 36 + a * 6 + b] * vi[b];
+            if (off) {
+#pragma unroll
+                for (int b = 0; b < 6; ++b) s += Go[(long long)i * 36 + a * 6 + b] * vm[b];
+            }
+            t[a] = s;
         }
     };
-    double gg = 0.0;
+    // element (i, a) of G^T G v
+    auto mprec = [&](int i, int a, auto&& vv) {
+        double t[6];
+        gblock(i, vv, t);
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) s += Gd[(long long)i * 36 + b * 6 + a] * t[b];
+        if (Go != nullptr && i + 1 < T) {
+            gblock(i + 1, vv, t);
+#pragma unroll
+            for (int b = 0; b < 6; ++b) s += Go[(long long)(i + 1) * 36 + b * 6 + a] * t[b];
+        }
+        return s;
+    };
+    double* rA = r;     // residual read this iteration
+    double* rB = tmp;   // residual written this iteration
+    // r = g, z = M g, p = z
+    double gg = 0.0, rz = 0.0;
     for (int e = e0; e < E; e += es) {
+        const int i = e / 6, a = e - 6 * i;
+        const double zv = mprec(i, a, [&](int j, int b) { return g[6 * j + b]; });
+        const double gv = g[e];
         x[e] = 0.0;
-        r[e] = g[e];
-        gg += g[e] * g[e];
+        rA[e] = gv;
+        z[e] = zv;
+        p[e] = zv;
+        gg += gv * gv;
+        rz += gv * zv;
     }
-    const double g0 = sqrt(grid_sum(gg, 0));
+    grid_sum2(gg, rz, 0);
+    const double g0 = sqrt(gg);
     bool broke = bad && *bad;
     int it = 0;
     double rn = g0;
     if (!broke && g0 > 0.0) {
-        prec();
-        double rz = 0.0;
+        // q = H p with p = z, complete after the barrier
+        double pq = 0.0, unused = 0.0;
         for (int e = e0; e < E; e += es) {
-            p[e] = z[e];
-            rz += r[e] * z[e];
+            const double y = hrow(e, z);
+            q[e] = y;
+            pq += z[e] * y;
         }
-        rz = grid_sum(rz, 1);
+        grid_sum2(pq, unused, 2);
         while (it < maxit) {
-            double pq = 0.0;
-            for (int e = e0; e < E; e += es) {
-                const double y = hrow(e, p);
-                q[e] = y;
-                pq += p[e] * y;
-            }
-            pq = grid_sum(pq, 2);
             if (!(pq > 0.0) || !(rz > 0.0)) {
                 broke = true;
                 break;
             }
             const double al = rz / pq;
-            double rr = 0.0;
+            const double* ro = rA;
+            auto rnew = [&](int j, int b) { return fma(-al, q[6 * j + b], ro[6 * j + b]); };
+            double rr = 0.0, rz2 = 0.0;
             for (int e = e0; e < E; e += es) {
+                const int i = e / 6, a = e - 6 * i;
+                const double rv = rnew(i, a);
+                const double zv = mprec(i, a, rnew);
                 x[e] += al * p[e];
-                r[e] -= al * q[e];
-                rr += r[e] * r[e];
+                rB[e] = rv;
+                z[e] = zv;
+                rr += rv * rv;
+                rz2 += rv * zv;
             }
-            rn = sqrt(grid_sum(rr, 3));
+            grid_sum2(rr, rz2, 0);
             ++it;
+            rn = sqrt(rr);
+            double* t = rA;
+            rA = rB;
+            rB = t;
             if (!(rn > tol * g0)) break;
-            prec();
-            double rz2 = 0.0;
-            for (int e = e0; e < E; e += es) rz2 += r[e] * z[e];
-            rz2 = grid_sum(rz2, 0);
             const double be = rz2 / rz;
             rz = rz2;
-            for (int e = e0; e < E; e += es) p[e] = z[e] + be * p[e];
-            grid.sync();
+            pq = 0.0;
+            unused = 0.0;
+            for (int e = e0; e < E; e += es) {
+                const double pv = z[e] + be * p[e];
+                const double qv = hrow(e, z) + be * q[e];
+                p[e] = pv;
+                q[e] = qv;
+                pq += pv * qv;
+            }
+            grid_sum2(pq, unused, 2);
         }
     }
     if (blockIdx.x == 0 && tid == 0) {
