@@ -601,20 +601,43 @@ __global__ __launch_bounds__(256) void k_lp_pcg_coop(int T, int W, const double*
         }
         block_sum2(a, b);
     };
-    // y_i[a] = (H v)_i[a] over the band
+    // y_i[a] = (H v)_i[a] over the band, four block distances at a time with
+    // every load issued before the products (a load under a condition waits
+    // for its data before the next one starts): indices past the band are
+    // clamped into it and their terms weighted by zero
     auto hrow = [&](int e, const double* v) {
         const int i = e / 6, a = e - 6 * (e / 6);
         double y = 0.0;
-        for (int d = 0; d <= W; ++d) {
-            if (i - d >= 0) {
-                const double* B = Hb + ((long long)i * (W + 1) + d) * 36 + a * 6;
-                const double* vv = v + 6LL * (i - d);
-                for (int b = 0; b < 6; ++b) y += B[b] * vv[b];
+        for (int d0 = 0; d0 <= W; d0 += 4) {
+            double bl[4][6], vl[4][6], bu[4][6], vu[4][6], wl[4], wu[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int d = d0 + k;
+                const bool lo = d <= W && i - d >= 0, up = d > 0 && d <= W && i + d < T;
+                wl[k] = lo ? 1.0 : 0.0;
+                wu[k] = up ? 1.0 : 0.0;
+                const int dl = lo ? d : 0, du = up ? d : 0;
+                const double* Bl = Hb + ((long long)i * (W + 1) + dl) * 36 + a * 6;
+                const double* Vl = v + 6LL * (i - dl);
+                const double* Bu = Hb + ((long long)(i + du) * (W + 1) + du) * 36 + a;
+                const double* Vu = v + 6LL * (i + du);
+#pragma unroll
+                for (int b = 0; b < 6; ++b) {
+                    bl[k][b] = Bl[b];
+                    vl[k][b] = Vl[b];
+                    bu[k][b] = Bu[6 * b];
+                    vu[k][b] = Vu[b];
+                }
             }
-            if (d > 0 && i + d < T) {
-                const double* B = Hb + ((long long)(i + d) * (W + 1) + d) * 36 + a;
-                const double* vv = v + 6LL * (i + d);
-                for (int b = 0; b < 6; ++b) y += B[6 * b] * vv[b];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                double sl = 0.0, su = 0.0;
+#pragma unroll
+                for (int b = 0; b < 6; ++b) {
+                    sl += bl[k][b] * vl[k][b];
+                    su += bu[k][b] * vu[k][b];
+                }
+                y += wl[k] * sl + wu[k] * su;
             }
         }
         return y;
@@ -641,19 +664,22 @@ __global__ __launch_bounds__(256) void k_lp_pcg_coop(int T, int W, const double*
             t[a] = s;
         }
     };
-    // element (i, a) of G^T G v
+    // element (i, a) of G^T G v (the second block row's loads are issued with
+    // the first: at the last node it repeats row i, weighted by zero)
     auto mprec = [&](int i, int a, auto&& vv) {
-        double t[6];
+        const bool nx = Go != nullptr && i + 1 < T;
+        const int in = nx ? i + 1 : i;
+        double t[6], tn[6];
         gblock(i, vv, t);
-        double s = 0.0;
+        gblock(in, vv, tn);
+        double s = 0.0, sn = 0.0;
 #pragma unroll
         for (int b = 0; b < 6; ++b) s += Gd[(long long)i * 36 + b * 6 + a] * t[b];
-        if (Go != nullptr && i + 1 < T) {
-            gblock(i + 1, vv, t);
+        if (Go != nullptr) {
 #pragma unroll
-            for (int b = 0; b < 6; ++b) s += Go[(long long)(i + 1) * 36 + b * 6 + a] * t[b];
+            for (int b = 0; b < 6; ++b) sn += Go[(long long)in * 36 + b * 6 + a] * tn[b];
         }
-        return s;
+        return nx ? s + sn : s;
     };
     double* rA = r;     // residual read this iteration
     double* rB = tmp;   // residual written this iteration
