@@ -428,7 +428,12 @@ __global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 2
                 // then 27 lanes each add 32 partials
                 T* scr = Vsh + wv * SCRW * 27;
 #pragma unroll
-                for (int q = 0; q < 27; ++q) pr[q] = half_sum(pr[q]);
+                for (int q = 0; q < 27; ++q) {
+                    // FP32: permlane swap; FP64 keeps the LDS exchange, whose
+                    // fewer temporaries keep the 8-pose shape free of spills
+                    if constexpr (sizeof(T) == 4) pr[q] = half_sum(pr[q]);
+                    else pr[q] += __shfl_xor(pr[q], 32, 64);
+                }
                 if (lane < 32) {
 #pragma unroll
                     for (int q = 0; q < 27; ++q) scr[lane * 27 + q] = pr[q];
