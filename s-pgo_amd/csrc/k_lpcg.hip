@@ -144,8 +144,9 @@ __global__ void k_lp_feat2(int T, int W, const double* s, const int* node_sys, c
 // ---- network --------------------------------------------------------------
 // z[n][o] = b[o] + W1[o][:] in1[n][:] + W2[o][:] in2[n][:] for o < O (O <= 64);
 // h = relu(z) when h != nullptr.  A block stages W1 and W2 in LDS (padded rows)
-// and handles LP_LIN_NODES nodes, a thread one output of four nodes at a time.
-constexpr int LP_LIN_NODES = 32;
+// and handles LP_LIN_NODES nodes, a thread one output of four nodes at a time,
+// each dot product in four independent partial sums (short dependency chains).
+constexpr int LP_LIN_NODES = 8;
 __global__ __launch_bounds__(256) void k_lp_linear(int T, int O, int I1, const float* in1, const float* W1, int I2,
                                                    const float* in2, const float* W2, const float* b, float* z,
                                                    float* h) {
@@ -161,13 +162,14 @@ __global__ __launch_bounds__(256) void k_lp_linear(int T, int O, int I1, const f
     for (int nn = q; nn < LP_LIN_NODES; nn += 4) {
         const int n = n0 + nn;
         if (n >= T) break;
-        float a = b[o];
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
         const float* x1 = in1 + (long long)n * I1;
-        for (int k = 0; k < I1; ++k) a += wl[o * (I1 + 1) + k] * x1[k];
+        for (int k = 0; k < I1; ++k) acc[k & 3] += wl[o * (I1 + 1) + k] * x1[k];
         if (I2 > 0) {
             const float* x2 = in2 + (long long)n * I2;
-            for (int k = 0; k < I2; ++k) a += w2l[o * (I2 + 1) + k] * x2[k];
+            for (int k = 0; k < I2; ++k) acc[k & 3] += w2l[o * (I2 + 1) + k] * x2[k];
         }
+        const float a = b[o] + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
         z[(long long)n * O + o] = a;
         if (h) h[(long long)n * O + o] = a > 0.f ? a : 0.f;
     }
