@@ -206,10 +206,19 @@ __global__ void k_lp_agg(int T, int W, const double* s, const double* wsum, cons
     if (t >= T * LP_HID) return;
     const int i = t / LP_HID, k = t - i * LP_HID;
     const int lo = sys_lo(sys_begin, node_sys, i), hi = sys_hi(sys_begin, node_sys, i);
+    // every load is issued (outside the system's range the indices fall back to
+    // node i and the term is not added): a load under a condition waits for its
+    // data before the next one starts
     double a = 0.0;
+#pragma unroll 4
     for (int d = 1; d <= W; ++d) {
-        if (i - d >= lo) a += s[(long long)i * (W + 1) + d] * hin[(long long)(i - d) * LP_HID + k];
-        if (i + d < hi) a += s[(long long)(i + d) * (W + 1) + d] * hin[(long long)(i + d) * LP_HID + k];
+        const bool dn = i - d >= lo, up = i + d < hi;
+        const double sl = s[(long long)i * (W + 1) + d];
+        const double su = s[(long long)(up ? i + d : i) * (W + 1) + d];
+        const float hl = hin[(long long)(dn ? i - d : i) * LP_HID + k];
+        const float hu = hin[(long long)(up ? i + d : i) * LP_HID + k];
+        a = dn ? a + sl * hl : a;
+        a = up ? a + su * hu : a;
     }
     m[t] = (float)(a / wsum[i]);
 }
