@@ -24,6 +24,16 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
     const Control* ctl = ctls + blockIdx.x;
     const int N = pd.N;
     const double* dlt = delta + pd.vec_off;
+    // one pose per thread (N <= WG): its pose in both buffers and its step are
+    // requested before the control block that selects between them arrives
+    double Xa[12], Xb[12], dpre[6];
+    const bool one = N <= WG && tid < N;
+    if (one) {
+        pose_load(B0.X + 12 * (pd.pose_base + tid), Xa);
+        pose_load(B1.X + 12 * (pd.pose_base + tid), Xb);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) dpre[q] = init ? 0.0 : dlt[6 * tid + q];
+    }
     const int cur = ctl->cur;
     const StateBuf<T> bin = cur ? B1 : B0;
     const StateBuf<T> bout = init ? bin : (cur ? B0 : B1);
@@ -36,11 +46,16 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
     double Xkeep[12];   // with one pose per thread it stays in registers for the second pass
     for (int k = k0; k < k1; ++k) {
         double X[12], Xi[12], tmp[12];
-        pose_load(bin.X + 12 * (pd.pose_base + k), X);
+        if (one) {
+#pragma unroll
+            for (int q = 0; q < 12; ++q) X[q] = cur ? Xb[q] : Xa[q];
+        } else {
+            pose_load(bin.X + 12 * (pd.pose_base + k), X);
+        }
         if (!init) {
             double xi[6], E[12];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) xi[q] = ok ? step * dlt[6 * k + q] : 0.0;
+            for (int q = 0; q < 6; ++q) xi[q] = ok ? step * (one ? dpre[q] : dlt[6 * k + q]) : 0.0;
             exp_se3(xi, E);
             pose_mul(X, E, tmp);
 #pragma unroll
