@@ -926,7 +926,7 @@ __global__ __launch_bounds__(256) void k_chol_split2(int Pp, int D, int s0, int 
     env_counts(ct.fst, nT, D, ct.mk);
     env_counts(cb.fst, nB, D, cb.mk);
     __syncthreads();
-    if (w == 0) env_backward<DT, DT <= 3 ? 4 : 2>(cs, winvS, LpS, lane);
+    if (w == 0) env_backward<DT, DT <= 3 ? 4 : (DT <= 4 ? 3 : 2)>(cs, winvS, LpS, lane);
     __syncthreads();
     // the two parts: forward solutions of their rows, the separator's solution after them
     for (int i = tid; i < EB * s0; i += 256) ct.xs[i] = x[i];
@@ -936,8 +936,8 @@ __global__ __launch_bounds__(256) void k_chol_split2(int Pp, int D, int s0, int 
         cb.xs[EB * (nb - s1) + t] = cs.xs[PS - 1 - t];
     }
     __syncthreads();
-    if (w == 0) env_backward<DT, DT <= 3 ? 4 : 2>(ct, winv, Lp, lane);
-    if (w == 1) env_backward<DT, DT <= 3 ? 4 : 2>(cb, winv, Lp, lane);
+    if (w == 0) env_backward<DT, DT <= 3 ? 4 : (DT <= 4 ? 3 : 2)>(ct, winv, Lp, lane);
+    if (w == 1) env_backward<DT, DT <= 3 ? 4 : (DT <= 4 ? 3 : 2)>(cb, winv, Lp, lane);
     __syncthreads();
     for (int i = tid; i < EB * s0; i += 256) x[i] = ct.xs[i];
     for (int t = tid; t < PS; t += 256) x[EB * s0 + t] = cs.xs[t];
@@ -953,39 +953,54 @@ size_t chol_split_lds(int Pp, int D, int part) {
     return env_ctx_bytes(nS - 1, nS) + env_ctx_bytes(0, nb) * 2;
 }
 
+// compiled profile depths: 3 (C2-like), 4 (ten-frame spans) and EDMAX; the
+// depth sizes the row prefetch, the unrolled window loads and the backward
+// solve's buffers
+template <int DT>
+static void band_launch(const ProbDesc* probs, int n_probs, size_t lds, double* H, const int* first,
+                        const double* g, double* x, double* winv, double* Lp, Control* ctl, hipStream_t s) {
+    hipLaunchKernelGGL(k_chol_env<DT>, dim3(n_probs), dim3(256), lds, s, probs, H, first, g, x, winv, Lp, ctl);
+}
+
 void launch_chol_band(const ProbDesc* probs, int n_probs, int Dmax, size_t lds, double* H, const int* first,
                       const double* g, double* x, double* winv, double* Lp, Control* ctl, hipStream_t s) {
-    if (Dmax <= 3)
-        hipLaunchKernelGGL(k_chol_env<3>, dim3(n_probs), dim3(256), lds, s, probs, H, first, g, x, winv, Lp, ctl);
-    else
-        hipLaunchKernelGGL(k_chol_env<EDMAX>, dim3(n_probs), dim3(256), lds, s, probs, H, first, g, x, winv, Lp,
-                           ctl);
+    if (Dmax <= 3) band_launch<3>(probs, n_probs, lds, H, first, g, x, winv, Lp, ctl, s);
+    else if (Dmax == 4) band_launch<4>(probs, n_probs, lds, H, first, g, x, winv, Lp, ctl, s);
+    else band_launch<EDMAX>(probs, n_probs, lds, H, first, g, x, winv, Lp, ctl, s);
+}
+
+template <int DT>
+static void split_launch(double* H, int Pp, int D, int s0, int s1, const int* firstT, const int* firstB,
+                         const double* g, double* x, double* winv, double* Lp, double* S, double* Sm, double* winvS,
+                         double* LpS, Control* ctl, hipStream_t s) {
+    hipLaunchKernelGGL(k_chol_split1<DT>, dim3(2), dim3(256), chol_split_lds(Pp, D, 1), s, H, Pp, D, s0, s1, firstT,
+                       firstB, g, x, winv, Lp, S, ctl);
+    hipLaunchKernelGGL(k_chol_split2<DT>, dim3(1), dim3(256), chol_split_lds(Pp, D, 2), s, Pp, D, s0, s1, firstT,
+                       firstB, x, winv, Lp, S, Sm, winvS, LpS, ctl);
 }
 
 void launch_chol_split(double* H, int Pp, int D, int s0, int s1, const int* firstT, const int* firstB,
                        const double* g, double* x, double* winv, double* Lp, double* S, double* Sm, double* winvS,
                        double* LpS, Control* ctl, hipStream_t s) {
-    if (D <= 3) {
-        hipLaunchKernelGGL(k_chol_split1<3>, dim3(2), dim3(256), chol_split_lds(Pp, D, 1), s, H, Pp, D, s0, s1,
-                           firstT, firstB, g, x, winv, Lp, S, ctl);
-        hipLaunchKernelGGL(k_chol_split2<3>, dim3(1), dim3(256), chol_split_lds(Pp, D, 2), s, Pp, D, s0, s1,
-                           firstT, firstB, x, winv, Lp, S, Sm, winvS, LpS, ctl);
-    } else {
-        hipLaunchKernelGGL(k_chol_split1<EDMAX>, dim3(2), dim3(256), chol_split_lds(Pp, D, 1), s, H, Pp, D, s0, s1,
-                           firstT, firstB, g, x, winv, Lp, S, ctl);
-        hipLaunchKernelGGL(k_chol_split2<EDMAX>, dim3(1), dim3(256), chol_split_lds(Pp, D, 2), s, Pp, D, s0, s1,
-                           firstT, firstB, x, winv, Lp, S, Sm, winvS, LpS, ctl);
-    }
+    if (D <= 3) split_launch<3>(H, Pp, D, s0, s1, firstT, firstB, g, x, winv, Lp, S, Sm, winvS, LpS, ctl, s);
+    else if (D == 4) split_launch<4>(H, Pp, D, s0, s1, firstT, firstB, g, x, winv, Lp, S, Sm, winvS, LpS, ctl, s);
+    else split_launch<EDMAX>(H, Pp, D, s0, s1, firstT, firstB, g, x, winv, Lp, S, Sm, winvS, LpS, ctl, s);
 }
 
 int prepare_chol_band(int Dmax, size_t lds) {
-    const void* f = Dmax <= 3 ? (const void*)k_chol_env<3> : (const void*)k_chol_env<EDMAX>;
+    const void* f = Dmax <= 3    ? (const void*)k_chol_env<3>
+                    : Dmax == 4 ? (const void*)k_chol_env<4>
+                                : (const void*)k_chol_env<EDMAX>;
     return (int)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 int prepare_chol_split(int Pp, int D) {
-    const void* f1 = D <= 3 ? (const void*)k_chol_split1<3> : (const void*)k_chol_split1<EDMAX>;
-    const void* f2 = D <= 3 ? (const void*)k_chol_split2<3> : (const void*)k_chol_split2<EDMAX>;
+    const void* f1 = D <= 3    ? (const void*)k_chol_split1<3>
+                     : D == 4 ? (const void*)k_chol_split1<4>
+                              : (const void*)k_chol_split1<EDMAX>;
+    const void* f2 = D <= 3    ? (const void*)k_chol_split2<3>
+                     : D == 4 ? (const void*)k_chol_split2<4>
+                              : (const void*)k_chol_split2<EDMAX>;
     int rc = (int)hipFuncSetAttribute(f1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)chol_split_lds(Pp, D, 1));
     if (rc) return rc;
     return (int)hipFuncSetAttribute(f2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)chol_split_lds(Pp, D, 2));
