@@ -279,7 +279,8 @@ def bench_c3(args) -> int:
     p = make_config("c3_mixed")
     out = {}
     for prec in ("fp64", "mixed"):
-        s = solver_from_problem(p, make_options(mode="lm", robust="huber", robust_param=2.0, precision=prec))
+        tile = {k: v for k, v in (("chunk_landmarks", args.chunk_landmarks), ("group_chunks", args.group_chunks)) if v}
+        s = solver_from_problem(p, make_options(mode="lm", robust="huber", robust_param=2.0, precision=prec, **tile))
         if args.warmup > 0:
             s.step(args.warmup)
         s.set_state(p.X0.astype(np.float64), p.alpha0.astype(np.float64))
