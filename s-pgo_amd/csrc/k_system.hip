@@ -439,10 +439,11 @@ __device__ __forceinline__ void env_counts(const int* fst, int nloc, int D, int*
 // Factor the first c.steps block columns of the local system and solve
 // forward.  Called by all 256 threads after c.fst and c.xs are staged and a
 // barrier.  On return (after a barrier) the window holds the blocks of rows
-// and columns >= c.steps with every update applied, c.xs the forward
-// solution of rows < EB * steps and the updated right-hand side of the rest,
-// and L below the diagonal and the inverse diagonal blocks are stored to
-// global memory (not yet waited for).
+// and columns >= c.steps with every update applied and c.xs the forward
+// solution of rows < EB * steps and the updated right-hand side of the rest.
+// The panels and inverse diagonal blocks go to global memory; those of the
+// last column may still be in progress on wavefront 3 (it also writes the
+// ring), so callers synchronise the workgroup before the backward solve.
 template <int DT>
 __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, double* Lp, int* sbad) {
     constexpr int EPRE = 4 * (DT + 1);      // prefetched doubles per lane of one block row
@@ -663,13 +664,15 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
         lds_barrier();
     }
     if (c.steps > 0) {
-        // the last factored column: its remaining trailing update and its copy
+        // the last factored column: its remaining trailing update, then its copy
+        // by wavefront 3 while the others return
         const int kp = c.steps - 1, rkp = rk == 0 ? W - 1 : rk - 1;
         rest_tiles(kp, rkp, 0, 4);
         lds_barrier();
         if (w == 3) copy_out(kp, rkp);
+    } else {
+        lds_barrier();
     }
-    lds_barrier();
 }
 
 // Backward solve of the first `steps` block rows of a local system by one
