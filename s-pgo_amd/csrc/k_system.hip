@@ -274,17 +274,22 @@ void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t 
 //            second or third wavefront repeats the diagonal block for its own
 //            three blocks, so no wavefront waits on another inside the column
 //            loop).  The right-hand side rides along as one more column, so
-//            the forward solve is done with the factorisation.  The panel is
-//            written to LDS only.  Meanwhile the other wavefronts apply the
-//            part of the previous column's trailing update that the next
+//            the forward solve is done with the factorisation.  A column's
+//            entries reach the other lanes by a lane read (the one on the
+//            pivot chain) and through LDS broadcasts (the rest).  The panel
+//            is written to LDS only.  Meanwhile the other wavefronts apply
+//            the part of the previous column's trailing update that the next
 //            panel does not need, and wavefront 3 moves the block row it
 //            prefetched into the window, inverts the previous diagonal
-//            factor, copies the previous panel to global memory (both for
-//            the backward solve) and prefetches the next block row.
+//            factor (W_k) and stores W_k and Q_(k,i) = L_(k+i,k) W_k, formed
+//            on MFMA, for the backward solve, and prefetches the next block
+//            row.
 //   phase B  the trailing update of block column k + 1 (the next panel) on
 //            MFMA, one 16x16 tile per wavefront.
-// The backward solve x_k = L_kk^-T (y_k - sum_i L_(k+i),k^T x_(k+i)) runs block
-// by block from the bottom as products with the stored inverses.
+// The backward solve x_k = W_k^T y_k - sum_i Q_(k,i)^T x_(k+i) runs block by
+// block from the bottom, one wavefront, with only the x_(k+1) term on its
+// sequential chain.  The kernels are compiled for profile depths 3, 4 and
+// EDMAX (the depth sizes the prefetch registers and unrolled loops).
 //
 // Long profiles are split in two (a twisted factorisation): a separator S of
 // D block rows in the middle leaves a top part T and a bottom part B that do
