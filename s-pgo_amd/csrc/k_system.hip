@@ -376,15 +376,18 @@ __device__ __forceinline__ void env_invert(double* Lc, const double* dinv, doubl
 #pragma unroll
         for (int r = q + 1; r < EB; ++r) s[r] -= Lc[q * EST + r] * s[q];
     }
-    // no fences: the substitution's reads, these writes and the reads below
-    // may alias, so the compiler keeps their order, and LDS executes one
-    // wavefront's operations in order (fences here cost ~150 registers)
+    // W crosses lanes through LDS: the writes and the reads back are volatile,
+    // so the compiler keeps them in order (and after the substitution's reads,
+    // which may alias them), and LDS executes one wavefront's operations in
+    // order.  Fences here would constrain the whole helper code (~150 more
+    // registers).
+    volatile lds_double* Lv = (lds_double*)Lc;
     if (lane < EB) {
 #pragma unroll
-        for (int r = 0; r < EB; ++r) Lc[r * EST + c] = s[r];
+        for (int r = 0; r < EB; ++r) Lv[r * EST + c] = s[r];
     }
 #pragma unroll
-    for (int t = 0; t < 4; ++t) wv[t] = Lc[(g + 4 * t) * EST + c];
+    for (int t = 0; t < 4; ++t) wv[t] = Lv[(g + 4 * t) * EST + c];
 }
 
 // Backward-solve blocks in global memory are 16 x 16 and lane-major: lane
