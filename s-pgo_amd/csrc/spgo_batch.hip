@@ -71,8 +71,11 @@ struct spgo_batch {
     long long tot_poses = 0, tot_frames = 0, tot_slots = 0, tot_meas = 0, tot_act = 0;
     int tot_groups = 0, max_span = 0;
     Camera cam{};
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    // captured iterations: [0] holds graph_iters[0] iterations and is launched
+    // repeatedly, [1] the remainder of a call
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int graph_iters[2] = {0, 0};
 
     std::vector<IterRecord> last_rec;   // n_probs x last_iters
     std::vector<double> cost0;
@@ -104,11 +107,17 @@ int bupload(spgo_batch* b, const std::vector<V>& v, const V** dst) {
     return SPGO_OK;
 }
 
+void destroy_graph_slot(spgo_batch* b, int k) {
+    if (b->gexec[k]) (void)hipGraphExecDestroy(b->gexec[k]);
+    if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+    b->gexec[k] = nullptr;
+    b->graph[k] = nullptr;
+    b->graph_iters[k] = 0;
+}
+
 void destroy_graph(spgo_batch* b) {
-    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->gexec = nullptr;
-    b->graph = nullptr;
+    destroy_graph_slot(b, 0);
+    destroy_graph_slot(b, 1);
 }
 
 int ensure_device(spgo_batch* b) {
@@ -396,6 +405,28 @@ void launch_linearise(spgo_batch* b, StateBuf<T>* s) {
     launch_accept(s[0].cost, s[1].cost, b->lay, b->ctl, b->rec, 0, 0, 0, 0, 0, 1, b->stream);
 }
 
+// Graph slot k holds `iters` consecutive iterations (captured again when the
+// count differs); iters == 0 leaves the slot empty.
+int capture_iterations(spgo_batch* b, int k, int iters, bool mixed) {
+    if (iters == b->graph_iters[k] && (iters == 0 || b->gexec[k])) return SPGO_OK;
+    destroy_graph_slot(b, k);
+    if (iters == 0) return SPGO_OK;
+    int rc = bcheck(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (rc) return rc;
+    for (int it = 0; it < iters; ++it) {
+        if (mixed)
+            launch_iteration<float>(b, b->bf);
+        else
+            launch_iteration<double>(b, b->bd);
+    }
+    rc = bcheck(b, hipStreamEndCapture(b->stream, &b->graph[k]), "hipStreamEndCapture");
+    if (rc) return rc;
+    rc = bcheck(b, hipGraphInstantiate(&b->gexec[k], b->graph[k], nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (rc) return rc;
+    b->graph_iters[k] = iters;
+    return SPGO_OK;
+}
+
 int ensure_records(spgo_batch* b, int iterations) {
     if (iterations <= b->rec_cap) return SPGO_OK;
     destroy_graph(b);
@@ -439,28 +470,26 @@ int run_step(spgo_batch* b, int iterations) {
     }
     rc = bcheck(b, hipGetLastError(), "kernel launch");
     if (rc) return rc;
-    if (b->opt.use_graph && iterations > 0 && !b->gexec) {
-        rc = bcheck(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        if (rc) return rc;
-        if (mixed)
-            launch_iteration<float>(b, b->bf);
-        else
-            launch_iteration<double>(b, b->bd);
-        rc = bcheck(b, hipStreamEndCapture(b->stream, &b->graph), "hipStreamEndCapture");
-        if (rc) return rc;
-        rc = bcheck(b, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-        if (rc) return rc;
-    }
+    // up to GRAPH_UNROLL iterations per graph launch, the remainder in a second graph
+    const bool graph = b->opt.use_graph && iterations > 0;
+    const int unroll = std::min(iterations, GRAPH_UNROLL);
+    const int n_full = graph ? iterations / unroll : 0;
+    const int n_rem = graph ? iterations % unroll : 0;
+    if (graph && ((rc = capture_iterations(b, 0, unroll, mixed)) || (rc = capture_iterations(b, 1, n_rem, mixed))))
+        return rc;
     rc = bcheck(b, hipEventRecord(b->ev0, b->stream), "hipEventRecord");
     if (rc) return rc;
-    for (int it = 0; it < iterations; ++it) {
-        if (b->opt.use_graph) {
-            rc = bcheck(b, hipGraphLaunch(b->gexec, b->stream), "hipGraphLaunch");
-            if (rc) return rc;
-        } else if (mixed) {
-            launch_iteration<float>(b, b->bf);
-        } else {
-            launch_iteration<double>(b, b->bd);
+    if (graph) {
+        for (int it = 0; it < n_full && !rc; ++it)
+            rc = bcheck(b, hipGraphLaunch(b->gexec[0], b->stream), "hipGraphLaunch");
+        if (!rc && n_rem > 0) rc = bcheck(b, hipGraphLaunch(b->gexec[1], b->stream), "hipGraphLaunch");
+        if (rc) return rc;
+    } else {
+        for (int it = 0; it < iterations; ++it) {
+            if (mixed)
+                launch_iteration<float>(b, b->bf);
+            else
+                launch_iteration<double>(b, b->bd);
         }
     }
     rc = bcheck(b, hipEventRecord(b->ev1, b->stream), "hipEventRecord");

@@ -72,11 +72,17 @@ void project_rotation(double* P) {
     P[8] = R[6], P[9] = R[7], P[10] = R[8];
 }
 
+static void destroy_graph_slot(spgo_solver* h, int k) {
+    if (h->gexec[k]) (void)hipGraphExecDestroy(h->gexec[k]);
+    if (h->graph[k]) (void)hipGraphDestroy(h->graph[k]);
+    h->gexec[k] = nullptr;
+    h->graph[k] = nullptr;
+    h->graph_iters[k] = 0;
+}
+
 void destroy_graph(spgo_solver* h) {
-    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-    if (h->graph) (void)hipGraphDestroy(h->graph);
-    h->gexec = nullptr;
-    h->graph = nullptr;
+    destroy_graph_slot(h, 0);
+    destroy_graph_slot(h, 1);
     h->graph_dirty = true;
 }
 
@@ -569,6 +575,25 @@ void launch_linearise(spgo_solver* h, StateBuf<T>* b) {
     launch_accept(b[0].cost, b[1].cost, h->lay, h->ctl, h->rec, 0, 0, 0, 0, 0, 1, s);
 }
 
+// Graph slot k holds `iters` consecutive iterations (captured again when the
+// count differs); iters == 0 leaves the slot empty.
+static int capture_iterations(spgo_solver* h, int k, int iters, bool mixed) {
+    if (iters == h->graph_iters[k] && (iters == 0 || h->gexec[k])) return SPGO_OK;
+    destroy_graph_slot(h, k);
+    if (iters == 0) return SPGO_OK;
+    int rc = check_hip(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (rc) return rc;
+    for (int it = 0; it < iters && !rc; ++it)
+        rc = mixed ? launch_iteration<float>(h, h->bf) : launch_iteration<double>(h, h->bd);
+    const int rc2 = check_hip(h, hipStreamEndCapture(h->stream, &h->graph[k]), "hipStreamEndCapture");
+    if (rc) return rc;
+    if (rc2) return rc2;
+    rc = check_hip(h, hipGraphInstantiate(&h->gexec[k], h->graph[k], nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (rc) return rc;
+    h->graph_iters[k] = iters;
+    return SPGO_OK;
+}
+
 // Copy the device state into the host mirror if the device is newer.
 int pull_state(spgo_solver* h) {
     if (!h->dev_newer) return SPGO_OK;
@@ -668,35 +693,31 @@ int run_step(spgo_solver* h, int iterations) {
     }
     rc = check_hip(h, hipGetLastError(), "kernel launch");
     if (rc) return rc;
-    // CG decides on the host whether the Cholesky runs: no graph for it
+    // CG decides on the host whether the Cholesky runs: no graph for it.  One
+    // graph holds GRAPH_UNROLL iterations and is launched repeatedly; a second
+    // one holds the remainder of the call.
     const bool graph = h->opt.use_graph && !h->pcg;
-    if (graph && iterations > 0 && h->graph_dirty) {
-        destroy_graph(h);
-        rc = check_hip(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        if (rc) return rc;
-        if (mixed)
-            rc = launch_iteration<float>(h, h->bf);
-        else
-            rc = launch_iteration<double>(h, h->bd);
-        const int rc2 = check_hip(h, hipStreamEndCapture(h->stream, &h->graph), "hipStreamEndCapture");
-        if (rc) return rc;
-        rc = rc2;
-        if (rc) return rc;
-        rc = check_hip(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-        if (rc) return rc;
+    const int unroll = std::min(iterations, GRAPH_UNROLL);
+    const int n_full = graph && iterations > 0 ? iterations / unroll : 0;
+    const int n_rem = graph && iterations > 0 ? iterations % unroll : 0;
+    if (graph && iterations > 0) {
+        if (h->graph_dirty) destroy_graph(h);
+        if ((rc = capture_iterations(h, 0, unroll, mixed)) || (rc = capture_iterations(h, 1, n_rem, mixed)))
+            return rc;
         h->graph_dirty = false;
     }
     rc = check_hip(h, hipEventRecord(h->ev0, h->stream), "hipEventRecord");
     if (rc) return rc;
-    for (int it = 0; it < iterations; ++it) {
-        if (graph) {
-            rc = check_hip(h, hipGraphLaunch(h->gexec, h->stream), "hipGraphLaunch");
-        } else if (mixed) {
-            rc = launch_iteration<float>(h, h->bf);
-        } else {
-            rc = launch_iteration<double>(h, h->bd);
-        }
+    if (graph) {
+        for (int it = 0; it < n_full && !rc; ++it)
+            rc = check_hip(h, hipGraphLaunch(h->gexec[0], h->stream), "hipGraphLaunch");
+        if (!rc && n_rem > 0) rc = check_hip(h, hipGraphLaunch(h->gexec[1], h->stream), "hipGraphLaunch");
         if (rc) return rc;
+    } else {
+        for (int it = 0; it < iterations; ++it) {
+            rc = mixed ? launch_iteration<float>(h, h->bf) : launch_iteration<double>(h, h->bd);
+            if (rc) return rc;
+        }
     }
     rc = check_hip(h, hipEventRecord(h->ev1, h->stream), "hipEventRecord");
     if (rc) return rc;

@@ -225,16 +225,23 @@ def test_singular_system_reported_and_rejected():
 
 
 def test_graph_and_direct_launch_identical():
+    # one graph carries a fixed number of iterations and a second one the
+    # remainder: calls of 5, 4, 37 and 3 iterations capture, reuse and
+    # re-capture both
     p = make_problem(10, 800, 4, seed=8)
     res = []
     for ug in (1, 0):
         s = solver_from_problem(p, make_options(use_graph=ug))
-        s.step(5)
-        res.append((s.get_state(), s.stats()))
-    (X1, a1), (c1, r1) = res[0]
-    (X2, a2), (c2, r2) = res[1]
-    assert np.array_equal(X1, X2) and np.array_equal(a1, a2)
-    assert [r["cost"] for r in r1] == [r["cost"] for r in r2]
+        calls = []
+        for n in (5, 4, 37, 3):
+            s.step(n)
+            calls.append((s.get_state(), s.stats()))
+        res.append(calls)
+    for ((X1, a1), (c1, r1)), ((X2, a2), (c2, r2)), n in zip(res[0], res[1], (5, 4, 37, 3)):
+        assert np.array_equal(X1, X2) and np.array_equal(a1, a2)
+        assert len(r1) == n
+        assert [r["cost"] for r in r1] == [r["cost"] for r in r2]
+        assert [r["lam"] for r in r1] == [r["lam"] for r in r2]
 
 
 def test_tile_shapes_do_not_change_results_beyond_rounding():
