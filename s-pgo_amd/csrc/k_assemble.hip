@@ -26,7 +26,6 @@ namespace {
 
 constexpr int VS = 66;                 // LDS row stride of the V tile (landmarks + padding)
 constexpr int NWAVE = WG / 64;
-constexpr int SCRW = 32;               // pair-sum scratch rows per wavefront
 
 // LDS shapes for groups whose pose span is at most KT
 template <int KT>
@@ -34,12 +33,9 @@ struct AsmShape {
     static constexpr int VROWS = 16 * ((6 * KT + 16) / 16);   // B columns + the gradient row, whole tiles
     static constexpr int NTILE = (VROWS / 16) * (VROWS / 16 + 1) / 2;
     static constexpr int MAXTILE = (NTILE + NWAVE - 1) / NWAVE;   // MFMA tiles per wavefront
-    static constexpr int A0 = VROWS * VS, A1 = NWAVE * SCRW * 27, A2 = (6 * KT) * (6 * KT);
-    static constexpr int VSH = A0 > A1 ? (A0 > A2 ? A0 : A2) : (A1 > A2 ? A1 : A2);
+    static constexpr int A0 = VROWS * VS, A2 = (6 * KT) * (6 * KT + 1);
+    static constexpr int VSH = A0 > A2 ? A0 : A2;
 };
-
-// keep the compiler from reordering one wavefront's LDS accesses
-__device__ __forceinline__ void wave_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
 
 // workgroup barrier that orders LDS only: the kernel never reads back what it
 // stores to global memory, so it does not wait for those stores
@@ -49,6 +45,93 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_s_waitcnt(0xC07F);
     __builtin_amdgcn_s_barrier();
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// ---- sums over the wavefront of the 27 pair-sum entries --------------------
+// A transposing butterfly: at every exchange a lane keeps one half of its
+// entries and adds its partner's copy of that half, 27 -> 14 -> 7 -> 4 -> 2 -> 1
+// entries, all in registers.  Partners differ in lane bit 5 and 4
+// (v_permlane32_swap / v_permlane16_swap), then are mirrored within rows,
+// half rows and quads (lane ^ 15, ^ 7, ^ 3: DPP), the lane with the bit clear
+// keeping the first half; a last quad swap (lane ^ 1) adds the two copies.
+// The masks 32, 16, 15, 7, 3, 1 are independent, so every total covers each
+// of the 64 lanes once.
+
+// Lower half: x + the upper partner's x; upper half: y + the lower partner's y.
+// v_permlane32_swap(x, y) exchanges x of the upper half with y of the lower.
+__device__ __forceinline__ float keep_sum32(float x, float y) {
+    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+__device__ __forceinline__ double keep_sum32(double x, double y) {
+    const unsigned long long bx = (unsigned long long)__double_as_longlong(x);
+    const unsigned long long by = (unsigned long long)__double_as_longlong(y);
+    const auto pl = __builtin_amdgcn_permlane32_swap((unsigned)bx, (unsigned)by, false, false);
+    const auto ph = __builtin_amdgcn_permlane32_swap((unsigned)(bx >> 32), (unsigned)(by >> 32), false, false);
+    const double p0 = __longlong_as_double((long long)(((unsigned long long)ph[0] << 32) | pl[0]));
+    const double p1 = __longlong_as_double((long long)(((unsigned long long)ph[1] << 32) | pl[1]));
+    return p0 + p1;
+}
+// The same between the rows (16 lanes) of each row pair: even rows keep x.
+__device__ __forceinline__ float keep_sum16(float x, float y) {
+    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+__device__ __forceinline__ double keep_sum16(double x, double y) {
+    const unsigned long long bx = (unsigned long long)__double_as_longlong(x);
+    const unsigned long long by = (unsigned long long)__double_as_longlong(y);
+    const auto pl = __builtin_amdgcn_permlane16_swap((unsigned)bx, (unsigned)by, false, false);
+    const auto ph = __builtin_amdgcn_permlane16_swap((unsigned)(bx >> 32), (unsigned)(by >> 32), false, false);
+    const double p0 = __longlong_as_double((long long)(((unsigned long long)ph[0] << 32) | pl[0]));
+    const double p1 = __longlong_as_double((long long)(((unsigned long long)ph[1] << 32) | pl[1]));
+    return p0 + p1;
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// DPP exchange with the partner selected by CTRL, which differs in lane bit BIT
+template <int CTRL, int BIT, typename T>
+__device__ __forceinline__ T keep_sum_dpp(T x, T y, int lane) {
+    const bool up = (lane >> BIT) & 1;
+    return (up ? y : x) + dpp_move<CTRL>(up ? x : y);
+}
+constexpr int DPP_ROW_MIRROR = 0x140, DPP_ROW_HALF_MIRROR = 0x141;
+constexpr int DPP_QUAD_REVERSE = 0x1B, DPP_QUAD_SWAP_PAIRS = 0xB1;   // quad_perm [3,2,1,0], [1,0,3,2]
+
+// Returns the wavefront total of one entry in lanes 2i and 2i + 1 (which one:
+// wave_sum27_entry).  Every lane of the wavefront must be active.
+template <typename T>
+__device__ __forceinline__ T wave_sum27(T (&v)[27], int lane) {
+    T a[14], b[7], c[4], d[2];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) a[i] = keep_sum32(v[i], i + 14 < 27 ? v[i + 14] : (T)0);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) b[i] = keep_sum16(a[i], a[i + 7]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c[i] = keep_sum_dpp<DPP_ROW_MIRROR, 3>(b[i], i + 4 < 7 ? b[i + 4] : (T)0, lane);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) d[i] = keep_sum_dpp<DPP_ROW_HALF_MIRROR, 2>(c[i], c[i + 2], lane);
+    T e = keep_sum_dpp<DPP_QUAD_REVERSE, 1>(d[0], d[1], lane);
+    e += dpp_move<DPP_QUAD_SWAP_PAIRS>(e);
+    return e;
+}
+
+// The entry whose total wave_sum27 leaves in this lane, -1 for padding: the
+// same exchanges applied to the codes q + 1 give 64 (q + 1), exactly.
+template <typename T>
+__device__ __forceinline__ int wave_sum27_entry(int lane) {
+    T code[27];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) code[i] = (T)(i + 1);
+    return (int)wave_sum27(code, lane) / 64 - 1;
 }
 
 constexpr int PRE_SEG = 2;   // slot segments per wavefront whose measurements are loaded up front
@@ -234,6 +317,7 @@ __global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 2
     const T fx0 = (T)A.cam.fx[0], fy0 = (T)A.cam.fy[0], cx0 = (T)A.cam.cx[0], cy0 = (T)A.cam.cy[0];
     const T inv_param = A.rb.param != 0.0 ? (T)(1.0 / A.rb.param) : (T)0;
 
+    const int pair_q = wave_sum27_entry<T>(lane);   // the pair-sum entry this lane receives
     group_setup();
     for (int ci = chunk_begin; ci < chunk_end; ++ci) {
         const bool first = ci == chunk_begin;
@@ -424,28 +508,9 @@ __global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 2
                 }
             }
             if (seg > 0) {
-                // pair sums of the segment: fold the two halves of the wavefront,
-                // then 27 lanes each add 32 partials
-                T* scr = Vsh + wv * SCRW * 27;
-#pragma unroll
-                for (int q = 0; q < 27; ++q) {
-                    // FP32: permlane swap; FP64 keeps the LDS exchange, whose
-                    // fewer temporaries keep the 8-pose shape free of spills
-                    if constexpr (sizeof(T) == 4) pr[q] = half_sum(pr[q]);
-                    else pr[q] += __shfl_xor(pr[q], 32, 64);
-                }
-                if (lane < 32) {
-#pragma unroll
-                    for (int q = 0; q < 27; ++q) scr[lane * 27 + q] = pr[q];
-                }
-                wave_order();
-                if (lane < 27) {
-                    T s4[4] = {(T)0, (T)0, (T)0, (T)0};
-#pragma unroll
-                    for (int jj = 0; jj < SCRW; ++jj) s4[jj & 3] += scr[jj * 27 + lane];
-                    S[seg][lane] += (s4[0] + s4[1]) + (s4[2] + s4[3]);
-                }
-                wave_order();
+                // pair sums of the segment, reduced over the wavefront in registers
+                const T tot = wave_sum27(pr, lane);
+                if (!(lane & 1) && pair_q >= 0) S[seg][pair_q] += tot;
             }
         }
         lds_barrier();
@@ -520,7 +585,10 @@ __global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 2
         Rsuf[(s0 - 1) * 27 + q] = run;
     }
     lds_barrier();
-    T* Dl = Vsh;   // K6 x K6, full
+    // K6 x K6, full, rows K6 + 1 apart: an odd stride puts the 16 rows of a
+    // transposed tile store (one per lane) in different banks
+    T* Dl = Vsh;
+    const int DS = K6 + 1;
 #pragma unroll
     for (int ti = 0; ti < SH::MAXTILE; ++ti) {
         const int tile = wv + ti * NWAVE;
@@ -535,8 +603,8 @@ __global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 2
                 if (row < K6 && col < K6) {
                     const int kb = max(row / 6, col / 6);
                     const T val = Rsuf[kb * 27 + sym6(row % 6, col % 6)] - acc[ti][v];
-                    Dl[row * K6 + col] = val;
-                    Dl[col * K6 + row] = val;
+                    Dl[row * DS + col] = val;
+                    Dl[col * DS + row] = val;
                 } else if (row == K6 && col < K6) {
                     dly[col] = Rsuf[(col / 6) * 27 + 21 + (col % 6)] - acc[ti][v];
                 }
@@ -556,13 +624,13 @@ __global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 2
         int bi = 0;
         while ((bi + 1) * (bi + 2) / 2 <= blk) ++bi;
         const int bj = blk - bi * (bi + 1) / 2;
-        const T* Db = Dl + 6 * bi * K6 + 6 * bj;
+        const T* Db = Dl + 6 * bi * DS + 6 * bj;
         double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int p = 0; p < 6; ++p) {
             const double a = Adk[bi][6 * p + r];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) t[q] += a * (double)Db[p * K6 + q];
+            for (int q = 0; q < 6; ++q) t[q] += a * (double)Db[p * DS + q];
         }
         T* out = Dg + (6 * bi + r) * K6 + 6 * bj;
 #pragma unroll
