@@ -199,6 +199,21 @@ void launch_accept(const double* cost_buf0, const double* cost_buf1, const Layou
                        reference_mode, lam_up, lam_down, lam_min, lam_max, init);
 }
 
+// Start of a step call: restart the iteration records, mark the factorisation
+// usable, optionally reset the damping, and (when the last linearisation is
+// kept) take its cost as the step's initial cost.  One problem's control block.
+__global__ void k_begin_step(Control* ctl, int reset_lambda, double lambda0, int keep_lin) {
+    if (threadIdx.x != 0) return;
+    if (reset_lambda) ctl->lambda = lambda0;
+    ctl->iter = 0;
+    ctl->chol_ok = 1;
+    if (keep_lin) ctl->cost0 = ctl->cost;
+}
+
+void launch_begin_step(Control* ctl, int reset_lambda, double lambda0, int keep_lin, hipStream_t s) {
+    hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(64), 0, s, ctl, reset_lambda, lambda0, keep_lin);
+}
+
 __global__ void k_fill(double* p, double v, long long n) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         p[i] = v;

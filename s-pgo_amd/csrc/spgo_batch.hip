@@ -406,11 +406,10 @@ void launch_linearise(spgo_batch* b, StateBuf<T>* s) {
 }
 
 // Graph slot k holds `iters` consecutive iterations (captured again when the
-// count differs); iters == 0 leaves the slot empty.
+// count differs); iters == 0 keeps what the slot holds for a later call.
 int capture_iterations(spgo_batch* b, int k, int iters, bool mixed) {
-    if (iters == b->graph_iters[k] && (iters == 0 || b->gexec[k])) return SPGO_OK;
+    if (iters == 0 || (iters == b->graph_iters[k] && b->gexec[k])) return SPGO_OK;
     destroy_graph_slot(b, k);
-    if (iters == 0) return SPGO_OK;
     int rc = bcheck(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     if (rc) return rc;
     for (int it = 0; it < iters; ++it) {

@@ -115,6 +115,9 @@ struct spgo_solver {
     const float* net_params = nullptr;   // the network's device parameters for this step
     spgo::LpSolve lps;          // CG buffers
     int* pcg_flag_host = nullptr;   // pinned: CG convergence flag read back each iteration
+    Control* ctl_host = nullptr;    // pinned: the control block and records read back after a step
+    IterRecord* rec_host = nullptr; // rec_host_cap entries
+    int rec_host_cap = 0;
     Control* ctl = nullptr;
     IterRecord* rec = nullptr;
     int rec_cap = 0;

@@ -49,6 +49,7 @@ void launch_accept(const double* cost_buf0, const double* cost_buf1, const Layou
                    IterRecord* rec, int reference_mode, double lam_up, double lam_down, double lam_min, double lam_max,
                    int init, hipStream_t s);
 
+void launch_begin_step(Control* ctl, int reset_lambda, double lambda0, int keep_lin, hipStream_t s);
 void launch_fill(double* p, double v, long long n, hipStream_t s);
 void launch_set_diag(double* H, int Pp, int from, double v, hipStream_t s);
 

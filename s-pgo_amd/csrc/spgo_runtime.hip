@@ -576,11 +576,10 @@ void launch_linearise(spgo_solver* h, StateBuf<T>* b) {
 }
 
 // Graph slot k holds `iters` consecutive iterations (captured again when the
-// count differs); iters == 0 leaves the slot empty.
+// count differs); iters == 0 keeps what the slot holds for a later call.
 static int capture_iterations(spgo_solver* h, int k, int iters, bool mixed) {
-    if (iters == h->graph_iters[k] && (iters == 0 || h->gexec[k])) return SPGO_OK;
+    if (iters == 0 || (iters == h->graph_iters[k] && h->gexec[k])) return SPGO_OK;
     destroy_graph_slot(h, k);
-    if (iters == 0) return SPGO_OK;
     int rc = check_hip(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     if (rc) return rc;
     for (int it = 0; it < iters && !rc; ++it)
@@ -660,17 +659,26 @@ int run_step(spgo_solver* h, int iterations) {
         if (!h->rec) return h->fail(SPGO_E_DEVICE, "device allocation failed");
         h->rec_cap = cap;
     }
-    Control c;
-    rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
-    if (rc) return rc;
-    if (h->lambda_reset) {
-        c.lambda = h->opt.mode == SPGO_MODE_REFERENCE ? 0.0 : h->opt.lambda0;
-        h->lambda_reset = false;
+    // pinned copies of the control block and the records, read back once the
+    // step's work is done (one stream synchronisation for all of it)
+    if (!h->ctl_host && hipHostMalloc((void**)&h->ctl_host, sizeof(Control), hipHostMallocDefault) != hipSuccess) {
+        h->ctl_host = nullptr;
+        return h->fail(SPGO_E_DEVICE, "pinned host allocation failed");
     }
-    c.iter = 0;
-    c.chol_ok = 1;
-    rc = check_hip(h, hipMemcpy(h->ctl, &c, sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");
-    if (rc) return rc;
+    if (h->rec_host_cap < h->rec_cap) {
+        if (h->rec_host) (void)hipHostFree(h->rec_host);
+        h->rec_host_cap = 0;
+        if (hipHostMalloc((void**)&h->rec_host, h->rec_cap * sizeof(IterRecord), hipHostMallocDefault) != hipSuccess) {
+            h->rec_host = nullptr;
+            return h->fail(SPGO_E_DEVICE, "pinned host allocation failed");
+        }
+        h->rec_host_cap = h->rec_cap;
+    }
+    // on the device: restart the records and, with the linearisation kept, the
+    // initial cost (a fresh linearisation sets both itself)
+    launch_begin_step(h->ctl, h->lambda_reset ? 1 : 0, h->opt.mode == SPGO_MODE_REFERENCE ? 0.0 : h->opt.lambda0,
+                      h->lin_valid ? 1 : 0, h->stream);
+    h->lambda_reset = false;
     const bool mixed = h->prec == SPGO_PREC_MIXED;
     h->net_params = nullptr;
     if (h->pcg && h->net && !(h->net_params = lpcg_device_params(h->net)))
@@ -681,15 +689,6 @@ int run_step(spgo_solver* h, int iterations) {
         else
             launch_linearise<double>(h, h->bd);
         h->lin_valid = true;
-    } else {
-        // keep the linearisation; restart the record and the initial cost
-        Control c2;
-        rc = check_hip(h, hipMemcpy(&c2, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
-        if (rc) return rc;
-        c2.cost0 = c2.cost;
-        c2.iter = 0;
-        rc = check_hip(h, hipMemcpy(h->ctl, &c2, sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");
-        if (rc) return rc;
     }
     rc = check_hip(h, hipGetLastError(), "kernel launch");
     if (rc) return rc;
@@ -721,6 +720,13 @@ int run_step(spgo_solver* h, int iterations) {
     }
     rc = check_hip(h, hipEventRecord(h->ev1, h->stream), "hipEventRecord");
     if (rc) return rc;
+    // every iteration writes one record: `iterations` of them, at most
+    rc = check_hip(h, hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(Control), hipMemcpyDeviceToHost, h->stream),
+                   "hipMemcpyAsync");
+    if (!rc && iterations > 0)
+        rc = check_hip(h, hipMemcpyAsync(h->rec_host, h->rec, iterations * sizeof(IterRecord), hipMemcpyDeviceToHost,
+                                         h->stream), "hipMemcpyAsync");
+    if (rc) return rc;
     rc = check_hip(h, hipStreamSynchronize(h->stream), "hipStreamSynchronize");
     if (rc) return rc;
     rc = check_hip(h, hipGetLastError(), "kernel execution");
@@ -728,14 +734,10 @@ int run_step(spgo_solver* h, int iterations) {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
     h->last_ms = ms;
-    rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
-    if (rc) return rc;
-    h->last_rec.resize(c.iter);
-    if (c.iter > 0) {
-        rc = check_hip(h, hipMemcpy(h->last_rec.data(), h->rec, c.iter * sizeof(IterRecord), hipMemcpyDeviceToHost),
-                       "hipMemcpy");
-        if (rc) return rc;
-    }
+    const Control c = *h->ctl_host;
+    if (c.iter < 0 || c.iter > std::max(iterations, 0))
+        return h->fail(SPGO_E_DEVICE, "the device wrote an unexpected number of iteration records");
+    h->last_rec.assign(h->rec_host, h->rec_host + c.iter);
     h->last_cost0 = c.cost0;
     h->dev_newer = true;
     return SPGO_OK;
@@ -829,6 +831,8 @@ int spgo_destroy(spgo_solver* h) {
     h->mem.release();
     h->lps.release();
     if (h->pcg_flag_host) (void)hipHostFree(h->pcg_flag_host);
+    if (h->ctl_host) (void)hipHostFree(h->ctl_host);
+    if (h->rec_host) (void)hipHostFree(h->rec_host);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
