@@ -73,7 +73,7 @@ typedef struct spgo_options {
     double lam_up, lam_down, lam_min, lam_max;
     double alpha_min, alpha_max;  /* inverse-depth clamp (LMutils.h:54-55)     */
     int32_t refine_steps;    /* iterative-refinement steps (MIXED)             */
-    int32_t use_graph;       /* replay each LM iteration as a captured graph   */
+    int32_t use_graph;       /* replay the LM iterations from captured graphs  */
     int32_t chunk_landmarks; /* tile shape: landmarks per LDS chunk (<= 64)    */
     int32_t group_chunks;    /* tile shape: chunks per workgroup               */
     int32_t factorization;   /* SPGO_FACTOR_*                                   */
