@@ -15,8 +15,9 @@
 //     on MFMA (the landmark Schur contraction, Solver_cuda.cu:690-712),
 // and writes D = Q - Y, d = q - y mapped to the incremental-pose parameters
 // with the block-diagonal adjoints Ad(G_a^-1 G_i).  The pose system is the sum
-// of these blocks over the groups that cover each pose pair (k_form_system).  Reductions are deterministic: wavefront-private
-// LDS partials combined in a fixed order, no global atomics.
+// of these blocks over the groups that cover each pose pair (k_form_system).
+// Reductions are deterministic: fixed cross-lane exchange patterns and
+// wavefront-private LDS partials combined in a fixed order, no global atomics.
 #include "spgo_device.h"
 #include "spgo_kernels.h"
 
