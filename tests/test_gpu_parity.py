@@ -226,18 +226,18 @@ def test_singular_system_reported_and_rejected():
 
 def test_graph_and_direct_launch_identical():
     # one graph carries a fixed number of iterations and a second one the
-    # remainder: calls of 5, 4, 37 and 3 iterations capture, reuse and
-    # re-capture both
+    # remainder: calls of 5, 4, 37, 70 and 3 iterations capture, reuse and
+    # re-capture both; 70 records outgrow the initial record buffer
     p = make_problem(10, 800, 4, seed=8)
     res = []
     for ug in (1, 0):
         s = solver_from_problem(p, make_options(use_graph=ug))
         calls = []
-        for n in (5, 4, 37, 3):
+        for n in (5, 4, 37, 70, 3):
             s.step(n)
             calls.append((s.get_state(), s.stats()))
         res.append(calls)
-    for ((X1, a1), (c1, r1)), ((X2, a2), (c2, r2)), n in zip(res[0], res[1], (5, 4, 37, 3)):
+    for ((X1, a1), (c1, r1)), ((X2, a2), (c2, r2)), n in zip(res[0], res[1], (5, 4, 37, 70, 3)):
         assert np.array_equal(X1, X2) and np.array_equal(a1, a2)
         assert len(r1) == n
         assert [r["cost"] for r in r1] == [r["cost"] for r in r2]
