@@ -150,34 +150,6 @@ __device__ __forceinline__ double row_sum4(double v) {
     return mk(pl[0], ph[0]) + mk(pl[1], ph[1]);
 }
 
-// v(lane) + v(lane ^ 32), the same in both lanes, without LDS
-__device__ __forceinline__ float half_sum(float v) {
-    const unsigned u = __float_as_uint(v);
-    const auto q = __builtin_amdgcn_permlane32_swap(u, u, false, false);
-    return __uint_as_float(q[0]) + __uint_as_float(q[1]);
-}
-
-__device__ __forceinline__ double half_sum(double v) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const auto pl = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
-    const auto ph = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
-    const double x = __longlong_as_double((long long)(((unsigned long long)ph[0] << 32) | pl[0]));
-    const double y = __longlong_as_double((long long)(((unsigned long long)ph[1] << 32) | pl[1]));
-    return x + y;
-}
-
-__device__ __forceinline__ int wave_min_int(int v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-__device__ __forceinline__ int wave_max_int(int v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
 // Row of the 16x16 MFMA accumulator held in register v of lane `lane`
 // (column = lane & 15).  Measured on gfx950: the f32 16x16x4 result keeps
 // four consecutive rows per lane group, the f64 16x16x4 result interleaves them.
