@@ -1,12 +1,13 @@
-"""The mixed-precision and batched workloads at their benchmark sizes, through
-properties that do not depend on size: the initial cost against the FP64
-oracle, monotone accepted costs, and the final cost against the FP64 path
-(within 1e-6 relative, the precision target of these modes).
+"""The mixed-precision, batched and learned-CG workloads at their benchmark
+sizes, through properties that do not depend on size: the initial cost
+against the FP64 oracle, monotone accepted costs, and the final cost against
+the FP64 path (within 1e-6 relative, the precision target of these modes).
 
 256 keyframes / 100,000 landmarks / 1,000,000 stereo measurements with 5 %
-outliers (Huber 2 px), and 512 independent windows of 8-32 keyframes and
-200-2,000 landmarks.  The sizes the oracle solves in seconds are covered in
-test_gpu_parity.py and test_batch.py.
+outliers (Huber 2 px); 512 independent windows of 8-32 keyframes and
+200-2,000 landmarks; 1,024 keyframes / 400,000 landmarks with 12 observations
+each.  The sizes the oracle solves in seconds are covered in
+test_gpu_parity.py, test_batch.py and test_lpcg_gpu.py.
 """
 import numpy as np
 import pytest
@@ -67,3 +68,37 @@ def test_batch_full_size_against_lone_fp64_solvers():
         assert rel <= FINAL, (p, rel)
     print(f"[full-c4] 64 of 512 windows: worst final-cost difference to the lone FP64 solver {worst:.3g}",
           flush=True)
+
+
+@pytest.mark.gpu
+def test_learned_cg_full_size_against_the_cholesky_path():
+    # 1,024 keyframes (a 6,144 x 6,144 reduced system), 400,000 landmarks, 12
+    # observations each: the network trained on systems of 128-384 keyframes;
+    # LM with CG preconditioned by it (FP32 linearisation) reaches the
+    # Cholesky path's cost after 10 iterations without falling back
+    from spgo.lpcg import LearnedPCG, solver_systems
+    from spgo.problems import make_long_problem
+
+    K = 12
+    train = [make_long_problem(n, 390 * n, K, seed=5100 + n) for n in (128, 192, 256, 384)]
+    net = LearnedPCG(K - 1, seed=11)
+    net.load(solver_systems(train, K - 1, states_per_problem=3, iters_between=2))
+    net.train(300, lr=1e-3)
+    p = make_long_problem(1024, 400000, K, seed=5005)
+    finals = {}
+    for tag, kw in (("cholesky", {}), ("cg", {"factorization": 4, "precision": "mixed"})):
+        s = solver_from_problem(p, make_options(mode="lm", **kw))
+        if tag == "cg":
+            s.set_preconditioner(net)
+        s.step(10)
+        c0, recs = s.stats()
+        costs = [r["cost"] for r in recs]
+        assert all(b <= a for a, b in zip(costs, costs[1:])), tag
+        if tag == "cg":
+            assert not any(r["pcg_fallback"] for r in recs)
+            print(f"[full-c5] CG iterations {[r['pcg_iters'] for r in recs]}", flush=True)
+        finals[tag] = costs[-1]
+    rel = abs(finals["cg"] - finals["cholesky"]) / finals["cholesky"]
+    print(f"[full-c5] final cost {finals['cholesky']:.12g} (Cholesky), {finals['cg']:.12g} (learned CG); "
+          f"relative difference {rel:.3g}", flush=True)
+    assert rel <= FINAL
