@@ -547,6 +547,19 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
         }
     };
 
+    // the same tiles over all four wavefronts, tile t on wavefront (nf + t) mod 4:
+    // the helpers first, the factoring wavefronts (after their columns) last
+    auto rest_share = [&](int kp, int rkp, int nf) {
+        const int mp = mk[kp];
+        const int nt = (mp - 1) * mp / 2;
+        for (int t = (w - nf + 4) & 3; t < nt; t += 4) {
+            int i = 2;
+            while ((i - 1) * i / 2 <= t) ++i;
+            const int j = 2 + t - (i - 2) * (i - 1) / 2;
+            env_tile(win, wrap(rkp + i), wrap(rkp + j), rkp, lane);
+        }
+    };
+
     int rk = 0;   // k mod W
     for (int k = 0; k < c.steps; ++k, rk = wrap(rk + 1)) {
         const int m = mk[k];
@@ -642,6 +655,9 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
                 xs[EB * (k + bi) + rr] = dg ? yv : b;
             }
             if (bad) *sbad = 1;
+            // the tiles of panel k - 1 that fall to this wavefront (none unless
+            // that panel has more than the helpers take)
+            if (k >= 1) rest_share(k - 1, rk == 0 ? W - 1 : rk - 1, nf);
         } else {
             // Helper wavefronts nf .. 3.  Wavefront 3 moves the prefetched block
             // row k + D + 1 into its slot (row k - 2's, free since the last phase
@@ -663,7 +679,7 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
             if (k >= 1) {
                 const int rkp = rk == 0 ? W - 1 : rk - 1;
                 if (w == nf) copy_out(k - 1, rkp);
-                rest_tiles(k - 1, rkp, nf, 4 - nf);   // after the copy or the loader
+                rest_share(k - 1, rkp, nf);   // after the copy or the loader
             }
         }
         lds_barrier();
