@@ -549,8 +549,7 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
 
     // the same tiles over all four wavefronts, tile t on wavefront (nf + t) mod 4:
     // the helpers first, the factoring wavefronts (after their columns) last
-    auto rest_share = [&](int kp, int rkp, int nf) {
-        const int mp = mk[kp];
+    auto rest_share = [&](int kp, int rkp, int nf, int mp) {
         const int nt = (mp - 1) * mp / 2;
         for (int t = (w - nf + 4) & 3; t < nt; t += 4) {
             int i = 2;
@@ -566,6 +565,9 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
         const int nf = env_factor_waves(m);
         // ---------------- phase A
         if (w < nf) {
+            // panel k - 1's depth, read now so that the check after the columns
+            // costs the column chain nothing
+            const int mprev = k >= 1 ? mk[k - 1] : 0;
             const int bi = (lane < EB) ? 0 : 1 + 3 * w + ((lane - EB) >> 4);
             const int rr = lane & 15;
             const bool valid = bi <= m;
@@ -657,7 +659,8 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
             if (bad) *sbad = 1;
             // the tiles of panel k - 1 that fall to this wavefront (none unless
             // that panel has more than the helpers take)
-            if (k >= 1) rest_share(k - 1, rk == 0 ? W - 1 : rk - 1, nf);
+            if (((w - nf + 4) & 3) < (mprev - 1) * mprev / 2)
+                rest_share(k - 1, rk == 0 ? W - 1 : rk - 1, nf, mprev);
         } else {
             // Helper wavefronts nf .. 3.  Wavefront 3 moves the prefetched block
             // row k + D + 1 into its slot (row k - 2's, free since the last phase
@@ -679,7 +682,7 @@ __device__ void env_forward(const EnvSrc src, const EnvCtx c, double* winv, doub
             if (k >= 1) {
                 const int rkp = rk == 0 ? W - 1 : rk - 1;
                 if (w == nf) copy_out(k - 1, rkp);
-                rest_share(k - 1, rkp, nf);   // after the copy or the loader
+                rest_share(k - 1, rkp, nf, mk[k - 1]);   // after the copy or the loader
             }
         }
         lds_barrier();
