@@ -180,14 +180,24 @@ def make_problem(n_poses: int, n_landmarks: int, obs_per_landmark, seed: int,
 
     Kmat = np.array([[fx, 0, cx], [0, fy, cy], [0, 0, 1.0]])
 
-    def project(Tinv, P):
-        # Tinv (F,4,4), P (n,3) -> pixel (n,F,2), depth (n,F)
-        Pc = np.einsum('fij,nj->nfi', Tinv[:, :3, :3], P) + Tinv[None, :, :3, 3]
-        z = Pc[..., 2]
-        zs = np.where(np.abs(z) < 1e-12, 1e-12, z)
-        u = fx * Pc[..., 0] / zs + cx
-        v = fy * Pc[..., 1] / zs + cy
-        return np.stack([u, v], -1), z
+    def visible(P):
+        # both cameras of every frame see P (n,3): depth above min_depth, pixel
+        # inside the image -> (n, F); in chunks of candidates to stay in cache
+        out = np.empty((P.shape[0], F), dtype=bool)
+        for c0 in range(0, P.shape[0], 8192):
+            Q = P[c0:c0 + 8192]
+            ok = np.ones((Q.shape[0], F), dtype=bool)
+            for Tinv in (Ginv, GRinv):
+                R, t = Tinv[:, :3, :3], Tinv[:, :3, 3]
+                Pc = [Q[:, 0:1] * R[None, :, i, 0] + Q[:, 1:2] * R[None, :, i, 1] + Q[:, 2:3] * R[None, :, i, 2]
+                      + t[None, :, i] for i in range(3)]
+                z = Pc[2]
+                zs = np.where(np.abs(z) < 1e-12, 1e-12, z)
+                u = fx * Pc[0] / zs + cx
+                v = fy * Pc[1] / zs + cy
+                ok &= (z > min_depth) & (u >= 0) & (u < width) & (v >= 0) & (v < height)
+            out[c0:c0 + 8192] = ok
+        return out
 
     def draw(k: int, need: int):
         # `need` landmarks with a run of k consecutive visible frames
@@ -196,11 +206,7 @@ def make_problem(n_poses: int, n_landmarks: int, obs_per_landmark, seed: int,
         while need > 0:
             m = max(4 * need, 1024)
             cand = rng.uniform(-box / 2, box / 2, size=(m, 3))
-            pl, zl = project(Ginv, cand)
-            pr, zr = project(GRinv, cand)
-            ok = ((zl > min_depth) & (zr > min_depth)
-                  & (pl[..., 0] >= 0) & (pl[..., 0] < width) & (pl[..., 1] >= 0) & (pl[..., 1] < height)
-                  & (pr[..., 0] >= 0) & (pr[..., 0] < width) & (pr[..., 1] >= 0) & (pr[..., 1] < height))
+            ok = visible(cand)
             # run of k consecutive visible frames starting at s, s in [0, F-k]
             c = np.concatenate([np.zeros((m, 1), np.int64), np.cumsum(ok, axis=1)], axis=1)
             run = (c[:, k:k + n_start] - c[:, 0:n_start]) == k          # (m, n_start)
@@ -428,3 +434,72 @@ def make_config(name: str, **overrides) -> Problem:
     kw = dict(CONFIGS[name])
     kw.update(overrides)
     return make_problem(name=name, **kw)
+
+
+def map_value_to_index(v: int, x: int, n: int) -> int:
+    """Anchor frame of landmark ``v`` of ``x`` when ``n`` frames host them in
+    equal shares (PythonUtils/Optimizer.py:12-18)."""
+    vc = max(0, min(v, x))
+    return min(int((vc / x) * n), n - 1)
+
+
+def make_manager_problem(n: int, m: int, seed: int, arc: float = 0.6, radius: float = 2.0,
+                         depth_range=(1.0, 4.0), noise_px: float = 1.0, fx: float = 500.0, fy: float = 500.0,
+                         cx: float = 320.0, cy: float = 240.0, width: int = 640, height: int = 480,
+                         baseline: float = 0.12, min_depth: float = 0.2, name: str = "") -> Problem:
+    """A problem enumerated the way the reference's harness feeds its solver
+    (CUDAmanager.py:69-79): ``n`` incremental poses (``n + 1`` frames), ``n m``
+    landmarks, landmark ``l`` hosted by frame ``map_value_to_index(l, n m, n)``
+    and written for every frame from its host to the last in which both
+    cameras see it.  A landmark's frames need not be consecutive and may span
+    the whole trajectory.  Frames sit on an arc of ``arc`` radians of a circle
+    of ``radius`` metres, looking at the origin; each landmark is drawn in its
+    host's view (pixel uniform in the image, depth uniform in ``depth_range``).
+    The initial state is the harness's: identity incremental poses and inverse
+    depths 1 (include/Solver.hpp:36-44)."""
+    rng = np.random.default_rng(seed)
+    N, F, L = int(n), int(n) + 1, int(n) * int(m)
+    theta = arc * (np.arange(F) / max(F - 1, 1) - 0.5)
+    G = np.stack([look_at_origin(np.array([radius * math.cos(a), radius * math.sin(a), 0.0])) for a in theta])
+    T_rl = np.eye(4)
+    T_rl[0, 3] = baseline
+    Ginv = _inv(G)
+    GRinv = _inv(G @ T_rl)
+    host = np.array([map_value_to_index(l, L, N) for l in range(L)], np.int64)
+    u = rng.uniform(0, width, size=L)
+    v = rng.uniform(0, height, size=L)
+    d = rng.uniform(depth_range[0], depth_range[1], size=L)
+    Pc = np.stack([(u - cx) / fx * d, (v - cy) / fy * d, d], -1)
+    P = np.einsum('nij,nj->ni', G[host, :3, :3], Pc) + G[host, :3, 3]
+
+    def proj(Tinv, Q):
+        Qc = np.einsum('ij,nj->ni', Tinv[:3, :3], Q) + Tinv[:3, 3]
+        z = Qc[:, 2]
+        zs = np.where(np.abs(z) < 1e-12, 1e-12, z)
+        return np.stack([fx * Qc[:, 0] / zs + cx, fy * Qc[:, 1] / zs + cy], -1), z
+
+    ma, mt, ml, uvs = [], [], [], []
+    for f in range(F):
+        pl, zl = proj(Ginv[f], P)
+        pr, zr = proj(GRinv[f], P)
+        ok = ((host <= f) & (zl > min_depth) & (zr > min_depth) & (pl[:, 0] >= 0) & (pl[:, 0] < width)
+              & (pl[:, 1] >= 0) & (pl[:, 1] < height) & (pr[:, 0] >= 0) & (pr[:, 0] < width)
+              & (pr[:, 1] >= 0) & (pr[:, 1] < height))
+        ok[host == f] = True        # the host always sees its own landmarks
+        idx = np.nonzero(ok)[0]
+        ma.append(host[idx])
+        mt.append(np.full(idx.size, f, np.int64))
+        ml.append(idx)
+        uvs.append(np.concatenate([pl[idx], pr[idx]], 1))
+    ma, mt, ml, uv = (np.concatenate(x) for x in (ma, mt, ml, uvs))
+    order = np.lexsort((mt, ml))                # landmark-major, frames ascending
+    ma, mt, ml, uv = ma[order], mt[order], ml[order], uv[order]
+    uv = uv + rng.normal(0.0, noise_px, size=uv.shape)
+    z_host = (np.einsum('nij,nj->ni', Ginv[host, :3, :3], P) + Ginv[host, :3, 3])[:, 2]
+    X_true = Ginv[1:] @ G[:-1]
+    intr = np.array([fx, fy, cx, cy, fx, fy, cx, cy], dtype=np.float32)
+    return Problem(n_poses=N, m_per_frame=int(m), intrinsics=intr, T_r_to_l=T_rl.reshape(16).astype(np.float32),
+                   meas_a=ma.astype(np.int32), meas_t=mt.astype(np.int32), meas_l=ml.astype(np.int32),
+                   uv=uv.astype(np.float32), X0=np.tile(np.eye(4, dtype=np.float32), (N, 1, 1)),
+                   alpha0=np.ones(L, np.float32), X_true=X_true, alpha_true=1.0 / z_host, n_landmarks=L,
+                   outlier=np.zeros(ma.size, dtype=bool), name=name)

@@ -225,3 +225,75 @@ def test_observation_table_semantics():
     pa = O.problem_arrays(2, 2, intr, T, a, t, l, uv)
     assert np.array_equal(pa.z[0], [[5, 6], [7, 8]]) and np.array_equal(pa.z[1], [[5, 6], [7, 8]])
     assert np.array_equal(pa.ray, np.zeros((3, 2)))
+
+
+# ---------------------------------------------------------------------------
+# Known answers for three constants of the step that finite differences cannot
+# pin: the near-frame weight, the Huber cost above its kink and the step size.
+# ---------------------------------------------------------------------------
+def _identity_scene(targets, shift_px, robust="cauchy", param=9.0, near_boost=True):
+    """One landmark anchored in frame 0 with identity incremental poses: every
+    frame is the identity, so the predicted left pixel equals the anchor ray's
+    pixel and each target's residual is the given pixel shift exactly."""
+    N = max(targets) + 1
+    ray = np.array([300.0, 200.0])
+    intr = np.array([500.0, 500.0, 320.0, 240.0, 500.0, 500.0, 320.0, 240.0], np.float32)
+    T = np.eye(4, dtype=np.float32).reshape(16)        # right camera at the left one
+    a = [0] + [0] * len(targets)
+    t = [0] + list(targets)
+    uv = [[ray[0], ray[1], ray[0], ray[1]]]
+    for _ in targets:
+        uv.append([ray[0] + shift_px[0], ray[1] + shift_px[1], ray[0], ray[1]])
+    pa = O.problem_arrays(N, 1, intr, T, a, t, [0] * len(a), np.array(uv, np.float32))
+    opts = O.Options(robust=robust, robust_param=param, near_boost=near_boost)
+    X = np.tile(np.eye(4), (N, 1, 1))
+    return pa, X, np.array([0.5]), opts
+
+
+def test_near_frame_weight_known_answer():
+    # Solver_cuda.cu:222-228: w = sqrt(1 / (1 + |e|^2 / 9)), times 4 when |t - a| < 2
+    pa, X, al, opts = _identity_scene([1, 2], (3.0, 4.0))
+    e, w, cost = O.residuals(pa, X, al, opts)
+    assert np.allclose(e[1, 0], [3.0, 4.0], atol=1e-9) and np.allclose(e[2, 0], [3.0, 4.0], atol=1e-9)
+    assert w[1, 0] == pytest.approx(4.0 / np.sqrt(1.0 + 25.0 / 9.0), rel=1e-12)   # |t - a| = 1
+    assert w[2, 0] == pytest.approx(1.0 / np.sqrt(1.0 + 25.0 / 9.0), rel=1e-12)   # |t - a| = 2
+    assert w[1, 1] == pytest.approx(4.0, rel=1e-12) and w[2, 1] == pytest.approx(1.0, rel=1e-12)
+    # cost 1/2 kappa c^2 log(1 + s / c^2) with kappa = 16 and 1
+    assert cost[1, 0] == pytest.approx(0.5 * 16.0 * 9.0 * np.log(34.0 / 9.0), rel=1e-12)
+    assert cost[2, 0] == pytest.approx(0.5 * 9.0 * np.log(34.0 / 9.0), rel=1e-12)
+    # without the boost both weights are the plain Cauchy weight
+    pa, X, al, opts = _identity_scene([1, 2], (3.0, 4.0), near_boost=False)
+    _, w, _ = O.residuals(pa, X, al, opts)
+    assert w[1, 0] == pytest.approx(w[2, 0], rel=1e-15)
+
+
+def test_huber_cost_known_answer_and_continuity():
+    # delta = 2 px: rho(s) = s inside, 2 delta sqrt(s) - delta^2 outside
+    pa, X, al, opts = _identity_scene([2], (3.0, 4.0), robust="huber", param=2.0)
+    _, w, cost = O.residuals(pa, X, al, opts)
+    assert cost[1, 0] == pytest.approx(0.5 * (2.0 * 2.0 * 5.0 - 4.0), rel=1e-12)    # |e| = 5 > delta
+    assert w[1, 0] == pytest.approx(np.sqrt(2.0 / 5.0), rel=1e-12)
+    # continuous at the kink |e| = delta, from both sides
+    for h in (1e-7, -1e-7):
+        pa, X, al, opts = _identity_scene([2], (0.0, 2.0 + h), robust="huber", param=2.0)
+        c = O.residuals(pa, X, al, opts)[2][1, 0]
+        assert c == pytest.approx(0.5 * 4.0, abs=3e-7)
+    s = np.array([4.0 - 1e-12, 4.0, 4.0 + 1e-12])
+    rho, _ = O.robust(s, O.Options(robust="huber", robust_param=2.0))
+    assert np.max(np.abs(rho - 4.0)) < 1e-11
+
+
+def test_step_size_scales_the_pose_update_only():
+    # Solver_cuda.cu:545-551: the alpha update carries no step size (551 overwrites 549)
+    rng = np.random.default_rng(9)
+    X = np.stack([O.exp_se3(x) for x in rng.normal(size=(3, 6)) * 0.1])
+    al = np.array([0.5, 0.25, 1.0])
+    d = rng.normal(size=18) * 0.05
+    dal = np.array([0.01, -0.02, 0.03])
+    X1, a1 = O.retract(X, al, d, dal, O.Options(step_size=1.0))
+    Xh, ah = O.retract(X, al, d, dal, O.Options(step_size=0.25))
+    assert np.array_equal(a1, ah)
+    assert np.allclose(ah, al + dal, rtol=0, atol=1e-15)
+    for k in range(3):
+        assert np.allclose(Xh[k], X[k] @ O.exp_se3(0.25 * d[6 * k:6 * k + 6]), atol=1e-14)
+        assert np.allclose(X1[k], X[k] @ O.exp_se3(d[6 * k:6 * k + 6]), atol=1e-14)
