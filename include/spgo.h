@@ -40,9 +40,9 @@ extern "C" {
 #define SPGO_MODE_REFERENCE 0
 #define SPGO_MODE_LM 1
 
-/* FP64: every phase in double.  MIXED: residual/Jacobian, assembly and the
- * landmark Schur contraction in float, the reduced pose system factored with
- * double diagonal blocks and refined in double. */
+/* FP64: every phase in double.  MIXED: residuals, Jacobians, assembly and the
+ * landmark Schur contraction in float (FP32 MFMA); the reduced pose system is
+ * summed, factored and solved in double. */
 #define SPGO_PREC_FP64 0
 #define SPGO_PREC_MIXED 1
 
@@ -62,6 +62,20 @@ extern "C" {
 #define SPGO_FACTOR_ENVELOPE_SPLIT 3
 #define SPGO_FACTOR_PCG 4
 
+/* Parameters of the reduced pose system.  INCREMENTAL: the increments of the
+ * incremental poses X_k themselves (the reference's system, Solver_cuda.cu:
+ * 636-736); every landmark couples the consecutive poses its anchor..target
+ * range covers, so the system is block-banded.  FRAMES: the exact change of
+ * variables z_k = sum_{i >= k} Ad(G_i) delta_i (z_N = 0), in which a
+ * measurement couples only its anchor and target frames: landmarks observed
+ * far apart (loop revisits) add sparse blocks instead of a dense square.  The
+ * damping lambda ||delta||^2 is carried over exactly, so both give the same
+ * step.  AUTO takes INCREMENTAL when every landmark spans at most 16 poses
+ * (and at most 512 landmark groups reach one pose), FRAMES otherwise. */
+#define SPGO_COORD_AUTO 0
+#define SPGO_COORD_INCREMENTAL 1
+#define SPGO_COORD_FRAMES 2
+
 typedef struct spgo_options {
     int32_t mode;            /* SPGO_MODE_*                                    */
     int32_t robust;          /* SPGO_ROBUST_*                                  */
@@ -78,7 +92,8 @@ typedef struct spgo_options {
     int32_t group_chunks;    /* tile shape: chunks per workgroup               */
     int32_t factorization;   /* SPGO_FACTOR_*                                   */
     int32_t pcg_max_iter;    /* CG iterations before the Cholesky fallback (PCG) */
-    int32_t reserved[6];
+    int32_t coordinates;     /* SPGO_COORD_*                                    */
+    int32_t reserved[5];
     double pcg_tol;          /* CG stops at ||r|| <= pcg_tol ||g|| (PCG)         */
 } spgo_options;
 
@@ -98,7 +113,7 @@ typedef struct spgo_layout_info {
     int32_t factorization;   /* SPGO_FACTOR_DENSE, _ENVELOPE, _ENVELOPE_SPLIT or _PCG: the path spgo_step takes */
     int32_t profile_depth;   /* blocks of 16 below the diagonal in the reduced system's profile */
     int32_t reduced_size;    /* rows of the padded reduced pose system                          */
-    int32_t reserved;
+    int32_t coordinates;     /* SPGO_COORD_INCREMENTAL or _FRAMES: the system spgo_step solves */
 } spgo_layout_info;
 
 typedef struct spgo_solver spgo_solver;

@@ -26,18 +26,29 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
     const double* dlt = delta + pd.vec_off;
     // one pose per thread (N <= WG): its pose in both buffers and its step are
     // requested before the control block that selects between them arrives
+    // In frame coordinates the solution is z and delta_k = Ad(G_k^-1) (z_k - z_{k+1})
+    // (z_N = 0) at the linearisation point.
+    const bool frame = lay.frame_mode != 0;
     double Xa[12], Xb[12], dpre[6];
     const bool one = N <= WG && tid < N;
     if (one) {
         pose_load(B0.X + 12 * (pd.pose_base + tid), Xa);
         pose_load(B1.X + 12 * (pd.pose_base + tid), Xb);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) dpre[q] = init ? 0.0 : dlt[6 * tid + q];
+        for (int q = 0; q < 6; ++q)
+            dpre[q] = init ? 0.0 : dlt[6 * tid + q] - ((frame && tid + 1 < N) ? dlt[6 * (tid + 1) + q] : 0.0);
     }
     const int cur = ctl->cur;
     const StateBuf<T> bin = cur ? B1 : B0;
     const StateBuf<T> bout = init ? bin : (cur ? B0 : B1);
     const bool ok = init ? false : (ctl->chol_ok != 0);
+    if (one && frame && !init) {
+        double Gi[12], x[6];
+        pose_load(bin.GinvL + 12 * (pd.frame_base + tid), Gi);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) x[q] = dpre[q];
+        adjoint_apply(Gi, x, dpre);
+    }
 
     const int chunk = (N + WG - 1) / WG;
     const int nact = (N + chunk - 1) / chunk;   // threads holding poses
@@ -53,9 +64,22 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
             pose_load(bin.X + 12 * (pd.pose_base + k), X);
         }
         if (!init) {
-            double xi[6], E[12];
+            double xi[6], E[12], dk[6];
+            if (one) {
 #pragma unroll
-            for (int q = 0; q < 6; ++q) xi[q] = ok ? step * (one ? dpre[q] : dlt[6 * k + q]) : 0.0;
+                for (int q = 0; q < 6; ++q) dk[q] = dpre[q];
+            } else if (frame) {
+                double Gi[12], x[6];
+                pose_load(bin.GinvL + 12 * (pd.frame_base + k), Gi);
+#pragma unroll
+                for (int q = 0; q < 6; ++q) x[q] = dlt[6 * k + q] - (k + 1 < N ? dlt[6 * (k + 1) + q] : 0.0);
+                adjoint_apply(Gi, x, dk);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) dk[q] = dlt[6 * k + q];
+            }
+#pragma unroll
+            for (int q = 0; q < 6; ++q) xi[q] = ok ? step * dk[q] : 0.0;
             exp_se3(xi, E);
             pose_mul(X, E, tmp);
 #pragma unroll

@@ -116,9 +116,93 @@ __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0,
     }
 }
 
+// The reduced system in frame coordinates (one problem): workgroup r writes the
+// six rows of frame r, every element left of the diagonal (zeros outside the
+// pattern), as the sum of the group-block entries the layout's plan lists for
+// the block (r, c), plus the damping lambda T, T = D^T diag(F_i) D with
+// (D z)_i = z_i - z_{i+1} and F_i = Ad(G_i^-1)^T Ad(G_i^-1): lambda ||delta||^2
+// in these variables (T_rr = F_r + F_{r-1}, T_{r,r-1} = -F_{r-1}).
+template <typename T>
+__global__ __launch_bounds__(256) void k_form_frames(Layout lay, StateBuf<T> B0, StateBuf<T> B1, Control* ctl,
+                                                     double* H, double* gvec) {
+    constexpr int MAXC = 2048;
+    __shared__ int cols[MAXC];
+    __shared__ double F[2][36];
+    const int r = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int Pp = lay.probs[0].Pp;
+    if (r == 0 && tid == 0) {
+        ctl->chol_ok = 1;
+        ctl->pcg_fail = 1;
+        ctl->pcg_iters = 0;
+    }
+    const StateBuf<T> bs = ctl->cur ? B1 : B0;
+    const double lam = ctl->lambda;
+    const int e0 = lay.zp_row[r], ne = lay.zp_row[r + 1] - e0;
+    const bool staged = ne <= MAXC;
+    if (staged)
+        for (int i = tid; i < ne; i += 256) cols[i] = lay.zp_col[e0 + i];
+    if (tid < 72) {
+        const int w = tid / 36, k = tid - 36 * w, f = r - w;
+        double v = 0.0;
+        if (f >= 0) {
+            double Gi[12];
+            pose_load(bs.GinvL + 12 * f, Gi);
+            const int p = k / 6, q = k - 6 * p;
+#pragma unroll
+            for (int x = 0; x < 6; ++x) v += adjoint_entry(Gi, x, p) * adjoint_entry(Gi, x, q);
+        }
+        F[w][k] = v;
+    }
+    __syncthreads();
+    const int items = 36 * r + 21;   // row 6r + e holds columns 0 .. 6r + e
+    for (int t = tid; t < items; t += 256) {
+        int e = 0;
+        while (e < 5 && t >= (e + 1) * (6 * r + 1) + (e + 1) * e / 2) ++e;
+        const int col = t - (e * (6 * r + 1) + e * (e - 1) / 2);
+        const int c = col / 6, j = col - 6 * c;
+        int lo = 0, hi = ne;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const int cm = staged ? cols[mid] : lay.zp_col[e0 + mid];
+            if (cm < c) lo = mid + 1;
+            else hi = mid;
+        }
+        double h = 0.0;
+        if (lo < ne && (staged ? cols[lo] : lay.zp_col[e0 + lo]) == c) {
+            const int k = e0 + lo;
+            const int kb = lay.zp_cbeg[k], ke = lay.zp_cbeg[k + 1];
+            for (int q = kb; q < ke; ++q) {
+                const ZCon z = lay.zp_con[q];
+                const long long idx = (z.doff >= 0 && j > e) ? z.off + (long long)j * z.K6 + e
+                                                             : z.off + (long long)e * z.K6 + j;
+                h += (double)bs.D[idx];
+            }
+        }
+        if (lam != 0.0) {
+            if (c == r) h += lam * (F[0][6 * e + j] + F[1][6 * e + j]);
+            else if (c == r - 1) h -= lam * F[1][6 * e + j];
+        }
+        H[(long long)(6 * r + e) * Pp + col] = h;
+    }
+    if (tid < 6) {
+        // the gradient: the diagonal block's contributions (the row's last entry)
+        double gs = 0.0;
+        if (ne > 0 && lay.zp_col[e0 + ne - 1] == r) {
+            const int k = e0 + ne - 1;
+            for (int q = lay.zp_cbeg[k]; q < lay.zp_cbeg[k + 1]; ++q) gs += (double)bs.d[lay.zp_con[q].doff + tid];
+        }
+        gvec[6 * r + tid] = gs;
+    }
+}
+
 template <typename T>
 void launch_form_system(const Layout& lay, const StateBuf<T>* bufs, Control* ctls, double* H, double* g,
                         hipStream_t s) {
+    if (lay.frame_mode) {
+        hipLaunchKernelGGL((k_form_frames<T>), dim3(lay.N), dim3(256), 0, s, lay, bufs[0], bufs[1], ctls, H, g);
+        return;
+    }
     if (lay.form_rows == 6)
         hipLaunchKernelGGL((k_form_system<T, 6>), dim3(lay.N), dim3(256), 0, s, lay, bufs[0], bufs[1], ctls, H, g);
     else

@@ -55,6 +55,15 @@ struct HostLayout {
     int kmax = 0;
     long long v_total = 0, D_total = 0, d_total = 0;
     int anchors_used = 0;
+    // every layout: first pose each pose row couples to in the reduced system
+    std::vector<int> fpose;
+    int band_need = 0;              // largest row - column distance, in poses
+    // frame coordinates
+    int frame_mode = 0;
+    int kt = 0, cl = 0;             // kernel shape: target frames per group, landmarks per chunk
+    std::vector<int> grp_frames;    // per group KTF + 1
+    std::vector<int> zp_row, zp_col, zp_cbeg;
+    std::vector<ZCon> zp_con;
 };
 
 }  // namespace spgo
@@ -146,6 +155,16 @@ void project_rotation(double* P);
 // given chunk_landmarks / group_chunks (0: default)
 int build_layout(spgo_solver* h, HostLayout& out);
 int build_layout(spgo_solver* h, int chunk_landmarks, int group_chunks, HostLayout& out);
+// the layout in frame coordinates (spgo_frames.hip)
+int build_layout_frames(spgo_solver* h, HostLayout& out);
+// the layout in the coordinates the solver's options ask for
+int build_any_layout(spgo_solver* h, HostLayout& out);
+// global left poses G (N + 1 of them, 3x4) of the incremental poses X (N x 12)
+void global_chain(int N, const double* X, double* G);
+// incremental-pose system H = U^T Z U, g = U^T gz from the frame-coordinate
+// system (dense lower triangle of Z, row stride ld), U the map delta -> z at
+// the global poses G (F x 12); H, g of size 6N
+void frames_to_incremental(int N, const double* Z, int ld, const double* gz, const double* G, double* H, double* g);
 // rows of the padded reduced pose system of n_poses poses
 int reduced_size(int n_poses);
 // block profile of the reduced system: fb[i] first nonzero block column of

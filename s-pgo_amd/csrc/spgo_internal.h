@@ -12,6 +12,9 @@ constexpr int CHUNK_LM = 64;        // landmarks per LDS chunk (upper bound)
 constexpr int CHUNK_OBS = 512;      // measurements per LDS chunk (upper bound)
 constexpr int NB = 32;              // Cholesky block size
 constexpr int WG = 256;             // threads per workgroup of the wide kernels
+// Frame coordinates: most distinct target frames of one landmark group (and so
+// of one landmark); a group's block covers its anchor and these frames.
+constexpr int KTF = 64;
 
 // Everything the linearisation needs to start a landmark group, in one record
 // (one memory round trip): the group's fields and those of its first chunk.
@@ -29,6 +32,16 @@ struct alignas(16) GroupDesc {
 struct FsEntry {
     int anchor, K6, d_off, pad;
     long long D_off;
+};
+
+// Frame coordinates: one contribution to a block (row frame r, column frame c)
+// of the reduced system, an entry of a landmark group's block: element (e, j)
+// of the 6x6 block is at off + e K6 + j (for a diagonal block, e >= j; the
+// upper entries are read transposed).  doff: first of the group's six gradient
+// entries of frame r, -1 off the diagonal.
+struct ZCon {
+    long long off;
+    int K6, doff;
 };
 
 // One problem of a batch (a lone solver is a batch of one).  The per-problem
@@ -95,6 +108,15 @@ struct Layout {
     const GroupDesc* grp_desc;    // per group, for k_assemble
     const int* fs_begin;          // per pose row block (N + 1): range in fs_list
     const FsEntry* fs_list;       // covering groups per pose row block, anchors ascending
+    // frame coordinates (frame_mode = 1): the groups' frames and the plan of the
+    // reduced system's blocks
+    int frame_mode;
+    int soff_stride;              // entries per chunk in chunk_soff
+    const int* grp_frames;        // per group KTF + 1 frames: slot 0 the anchor, then its target frames
+    const int* zp_row;            // per frame row r (N + 1 entries): its block entries zp_row[r] .. zp_row[r+1]
+    const int* zp_col;            // column frame of each block entry (ascending within a row)
+    const int* zp_cbeg;           // per block entry (+1): its contributions zp_cbeg[k] .. zp_cbeg[k+1]
+    const ZCon* zp_con;
     int n_probs;                  // problems (1 for a lone solver)
     const ProbDesc* probs;
     const int* pose_prob;         // problem of every pose (N entries)

@@ -19,6 +19,12 @@ struct AsmArgs {
 template <typename T>
 void launch_assemble(const AsmArgs& a, const StateBuf<T>* bufs, hipStream_t s);
 
+// the linearisation in frame coordinates (layouts with lay.frame_mode set)
+template <typename T>
+void launch_assemble_frames(const AsmArgs& a, const StateBuf<T>* bufs, hipStream_t s);
+// landmarks per group of the frame-coordinate kernel for groups of kt target frames
+int frames_landmarks_per_group(int kt);
+
 template <typename T>
 void launch_form_system(const Layout& lay, const StateBuf<T>* bufs, Control* ctls, double* H, double* g,
                         hipStream_t s);

@@ -42,6 +42,28 @@ int check_hip(spgo_solver* h, hipError_t e, const char* what) {
     return h->fail(SPGO_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// G_0 = I, G_{k+1} = G_k X_k^-1 (3x4 poses, F = N + 1 of them)
+void global_chain(int N, const double* X, double* G) {
+    double P[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    std::memcpy(G, P, sizeof(P));
+    for (int k = 0; k < N; ++k) {
+        const double* A = X + 12 * k;
+        double Ai[12], C[12];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) Ai[4 * r + c] = A[4 * c + r];
+        for (int r = 0; r < 3; ++r)
+            Ai[4 * r + 3] = -(Ai[4 * r] * A[3] + Ai[4 * r + 1] * A[7] + Ai[4 * r + 2] * A[11]);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) {
+                double v = P[4 * r] * Ai[c] + P[4 * r + 1] * Ai[4 + c] + P[4 * r + 2] * Ai[8 + c];
+                if (c == 3) v += P[4 * r + 3];
+                C[4 * r + c] = v;
+            }
+        std::memcpy(P, C, sizeof(P));
+        std::memcpy(G + 12 * (k + 1), P, sizeof(P));
+    }
+}
+
 void mat4_to_3x4(const float* T16, double* X12) {
     for (int i = 0; i < 12; ++i) X12[i] = (double)T16[i];
 }
@@ -257,6 +279,15 @@ int build_layout(spgo_solver* h, int chunk_landmarks, int group_chunks_opt, Host
                     hl.fs_list.push_back(FsEntry{a, 6 * hl.grp_K[g], hl.grp_d_off[g], 0, hl.grp_D_off[g]});
     }
     hl.fs_begin[h->N] = (int)hl.fs_list.size();
+    // pose j couples back to the smallest anchor among the groups covering it
+    hl.fpose.resize(h->N);
+    for (int j = 0; j < h->N; ++j) hl.fpose[j] = j;
+    for (size_t g = 0; g < hl.grp_anchor.size(); ++g)
+        for (int k = 0; k < hl.grp_K[g]; ++k) {
+            const int j = hl.grp_anchor[g] + k;
+            if (j < h->N) hl.fpose[j] = std::min(hl.fpose[j], hl.grp_anchor[g]);
+        }
+    for (int j = 0; j < h->N; ++j) hl.band_need = std::max(hl.band_need, j - hl.fpose[j]);
     out = std::move(hl);
     return SPGO_OK;
 }
@@ -272,13 +303,7 @@ int reduced_size(int n_poses) { return ((6 * n_poses + NB - 1) / NB) * NB; }
 // _ENVELOPE or _ENVELOPE_SPLIT), or -1 if the requested one cannot be used.
 int profile_blocks(int n_poses, const HostLayout& L, std::vector<int>& fb) {
     const int P = 6 * n_poses, Pp = reduced_size(n_poses), BBk = chol_band_block();
-    std::vector<int> fpose(n_poses);
-    for (int j = 0; j < n_poses; ++j) fpose[j] = j;
-    for (size_t g = 0; g < L.grp_anchor.size(); ++g)
-        for (int k = 0; k < L.grp_K[g]; ++k) {
-            const int j = L.grp_anchor[g] + k;
-            if (j < n_poses) fpose[j] = std::min(fpose[j], L.grp_anchor[g]);
-        }
+    const std::vector<int>& fpose = L.fpose;
     const int nbk = Pp / BBk;
     fb.assign(nbk, 0);
     int depth = 0;
@@ -289,6 +314,36 @@ int profile_blocks(int n_poses, const HostLayout& L, std::vector<int>& fb) {
         depth = std::max(depth, i - f);
     }
     return depth;
+}
+
+// The incremental kernels stage, per pose row, the groups whose anchors can
+// reach it (k_form_system): at most 512.
+static bool incremental_reach_ok(const spgo_solver* h, const HostLayout& L, std::string& why) {
+    for (int j = 0; j < h->N; ++j) {
+        const int amin = std::max(0, j - L.kmax + 1);
+        if (L.anchor_grp[j + 1] - L.anchor_grp[amin] > 512) {
+            why = "too many landmark groups reach pose " + std::to_string(j) + "; raise chunk_landmarks or group_chunks";
+            return false;
+        }
+    }
+    return true;
+}
+
+// The layout in the coordinates the options ask for; AUTO takes incremental
+// coordinates when the measurements fit them.
+int build_any_layout(spgo_solver* h, HostLayout& out) {
+    const int c = h->opt.coordinates;
+    if (c != SPGO_COORD_FRAMES) {
+        int rc = build_layout(h, out);
+        std::string why;
+        if (rc == SPGO_OK) {
+            if (incremental_reach_ok(h, out, why)) return SPGO_OK;
+            if (c == SPGO_COORD_INCREMENTAL) return h->fail(SPGO_E_LAYOUT, why);
+        } else if (rc != SPGO_E_LAYOUT || c == SPGO_COORD_INCREMENTAL) {
+            return rc;
+        }
+    }
+    return build_layout_frames(h, out);
 }
 
 static int factor_path(const spgo_solver* h, const HostLayout& L, std::vector<int>& fb, int& depth) {
@@ -365,7 +420,7 @@ int setup_layout(spgo_solver* h) {
     h->rec = nullptr;
     h->rec_cap = 0;
     HostLayout hl;
-    int rc = build_layout(h, hl);
+    int rc = build_any_layout(h, hl);
     if (rc) return rc;
     h->hl = std::move(hl);
     HostLayout& L = h->hl;
@@ -382,6 +437,13 @@ int setup_layout(spgo_solver* h) {
     lay.multi_chunk = 0;
     for (const GroupDesc& d : L.grp_desc) lay.multi_chunk |= d.chunk_end - d.chunk_begin > 1;
     lay.form_rows = 1;   // one problem: one row per workgroup fills the device
+    lay.frame_mode = L.frame_mode;
+    lay.soff_stride = L.frame_mode ? L.kt + 2 : KMAX + 2;
+    if (L.frame_mode &&
+        ((rc = upload(h, L.grp_frames, &lay.grp_frames)) || (rc = upload(h, L.zp_row, &lay.zp_row)) ||
+         (rc = upload(h, L.zp_col, &lay.zp_col)) || (rc = upload(h, L.zp_cbeg, &lay.zp_cbeg)) ||
+         (rc = upload(h, L.zp_con, &lay.zp_con))))
+        return rc;
     if ((rc = upload(h, L.ob_t, &lay.ob_t)) || (rc = upload(h, L.ob_meta, &lay.ob_meta)) ||
         (rc = upload(h, L.ob_uv, &lay.ob_uv)) || (rc = upload(h, L.lm_id, &lay.lm_id)) ||
         (rc = upload(h, L.lm_ray, &lay.lm_ray)) || (rc = upload(h, L.lm_pos, &lay.lm_pos)) ||
@@ -396,13 +458,6 @@ int setup_layout(spgo_solver* h) {
     const int P = 6 * h->N;
     h->Pp = reduced_size(h->N);
     {
-        // k_form_system stages the groups of the anchors that can reach a pose row
-        for (int j = 0; j < h->N; ++j) {
-            const int amin = std::max(0, j - L.kmax + 1);
-            if (L.anchor_grp[j + 1] - L.anchor_grp[amin] > 512)
-                return h->fail(SPGO_E_LAYOUT, "too many landmark groups reach pose " + std::to_string(j) +
-                                                  "; raise chunk_landmarks or group_chunks");
-        }
         std::vector<int> fb;
         int bw = 0;
         h->path = factor_path(h, L, fb, bw);
@@ -413,7 +468,7 @@ int setup_layout(spgo_solver* h) {
         h->lps.release();
         if (h->pcg) {
             // CG over the pose blocks: spans of K poses couple blocks up to K - 1 apart
-            const int need = std::max(1, L.kmax - 1);
+            const int need = std::max(1, L.band_need);
             const int W = h->net ? lpcg_band_blocks(h->net) : std::min(need, LP_WMAX);
             if (need > W)
                 return h->fail(SPGO_E_LAYOUT, "landmark spans need a band of " + std::to_string(need) +
@@ -753,6 +808,7 @@ bool valid_opts(const spgo_options* o, std::string& why) {
     if (o->chunk_landmarks < 0 || o->chunk_landmarks > CHUNK_LM) return why = "chunk_landmarks", false;
     if (o->group_chunks < 0) return why = "group_chunks", false;
     if (o->factorization < 0 || o->factorization > 4) return why = "factorization", false;
+    if (o->coordinates < 0 || o->coordinates > 2) return why = "coordinates", false;
     if (o->pcg_max_iter < 1) return why = "pcg_max_iter", false;
     if (!(o->pcg_tol > 0.0)) return why = "pcg_tol", false;
     return true;
@@ -845,7 +901,8 @@ int spgo_set_options(spgo_solver* h, const spgo_options* o) {
     std::string why;
     if (!valid_opts(o, why)) return h->fail(SPGO_E_ARG, "invalid option: " + why);
     const bool relayout = o->precision != h->opt.precision || o->chunk_landmarks != h->opt.chunk_landmarks ||
-                          o->group_chunks != h->opt.group_chunks || o->factorization != h->opt.factorization;
+                          o->group_chunks != h->opt.group_chunks || o->factorization != h->opt.factorization ||
+                          o->coordinates != h->opt.coordinates;
     const bool relin = relayout || o->robust != h->opt.robust || o->robust_param != h->opt.robust_param ||
                        o->near_boost != h->opt.near_boost || o->eps != h->opt.eps;
     if (o->mode != h->opt.mode || o->lambda0 != h->opt.lambda0) h->lambda_reset = true;
@@ -920,6 +977,15 @@ int spgo_write_observation(spgo_solver* h, int32_t anchor, int32_t target, int32
 int spgo_write_observations_bulk(spgo_solver* h, int32_t count, const int32_t* anchor, const int32_t* target,
                                  const int32_t* feat, const float* uvLR) {
     if (!h || count < 0 || (count > 0 && (!anchor || !target || !feat || !uvLR))) return SPGO_E_ARG;
+    // every entry is checked before any is written, so a rejected call leaves
+    // the measurement table as it was
+    for (int i = 0; i < count; ++i)
+        if (anchor[i] < 0 || anchor[i] >= h->F || target[i] < 0 || target[i] >= h->F || feat[i] < 0 ||
+            feat[i] >= h->L)
+            return h->fail(SPGO_E_ARG, "frame or landmark id out of range at entry " + std::to_string(i) +
+                                           "; nothing was written");
+    if ((long long)h->ma.size() + count > h->max_meas)
+        return h->fail(SPGO_E_CAPACITY, "measurement capacity N(N+1)M exceeded; nothing was written");
     h->ma.reserve(h->ma.size() + count);
     h->mt.reserve(h->mt.size() + count);
     h->ml.reserve(h->ml.size() + count);
@@ -1120,12 +1186,23 @@ int spgo_linearize(spgo_solver* h, double lambda, double* H, double* g, double* 
     std::vector<double> Hp((size_t)h->Pp * h->Pp);
     rc = check_hip(h, hipMemcpy(Hp.data(), h->H, Hp.size() * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
     if (rc) return rc;
-    if (H)
-        for (int i = 0; i < P; ++i)
-            for (int j = 0; j <= i; ++j) H[(size_t)i * P + j] = H[(size_t)j * P + i] = Hp[(size_t)i * h->Pp + j];
-    if (g) {
-        rc = check_hip(h, hipMemcpy(g, h->gvec, P * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
-        if (rc) return rc;
+    std::vector<double> gp(P);
+    rc = check_hip(h, hipMemcpy(gp.data(), h->gvec, P * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (rc) return rc;
+    if (h->lay.frame_mode) {
+        // the frame-coordinate system mapped to the incremental one at the
+        // current global poses
+        if ((rc = pull_state(h))) return rc;
+        std::vector<double> G((size_t)h->F * 12), Hi((size_t)P * P), gi(P);
+        global_chain(h->N, h->X.data(), G.data());
+        frames_to_incremental(h->N, Hp.data(), h->Pp, gp.data(), G.data(), Hi.data(), gi.data());
+        if (H) std::memcpy(H, Hi.data(), Hi.size() * sizeof(double));
+        if (g) std::memcpy(g, gi.data(), P * sizeof(double));
+    } else {
+        if (H)
+            for (int i = 0; i < P; ++i)
+                for (int j = 0; j <= i; ++j) H[(size_t)i * P + j] = H[(size_t)j * P + i] = Hp[(size_t)i * h->Pp + j];
+        if (g) std::memcpy(g, gp.data(), P * sizeof(double));
     }
     rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
     if (rc) return rc;
@@ -1217,7 +1294,7 @@ int spgo_set_preconditioner(spgo_solver* h, spgo_lpcg* net) {
 int spgo_problem_info(spgo_solver* h, spgo_layout_info* out) {
     if (!h || !out) return SPGO_E_ARG;
     HostLayout hl;
-    int rc = build_layout(h, hl);
+    int rc = build_any_layout(h, hl);
     if (rc) return rc;
     out->n_poses = h->N;
     out->n_slots = h->L;
@@ -1225,7 +1302,9 @@ int spgo_problem_info(spgo_solver* h, spgo_layout_info* out) {
     out->n_landmarks = (int)hl.lm_id.size();
     out->n_groups = (int)hl.grp_anchor.size();
     out->n_chunks = (int)hl.chunk.size();
-    out->max_span = hl.grp_K.empty() ? 0 : *std::max_element(hl.grp_K.begin(), hl.grp_K.end());
+    out->max_span = 0;
+    for (size_t m = 0; m < h->ma.size(); ++m) out->max_span = std::max(out->max_span, h->mt[m] - h->ma[m]);
+    out->coordinates = hl.frame_mode ? SPGO_COORD_FRAMES : SPGO_COORD_INCREMENTAL;
     out->n_anchors_used = hl.anchors_used;
     std::vector<int> fb;
     int depth = 0;

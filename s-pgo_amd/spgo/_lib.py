@@ -25,6 +25,9 @@ SPGO_FACTOR_DENSE = 1
 SPGO_FACTOR_ENVELOPE = 2
 SPGO_FACTOR_ENVELOPE_SPLIT = 3
 SPGO_FACTOR_PCG = 4
+SPGO_COORD_AUTO = 0
+SPGO_COORD_INCREMENTAL = 1
+SPGO_COORD_FRAMES = 2
 
 ERRORS = {-1: "invalid argument", -2: "capacity exceeded", -3: "unsupported layout",
           -4: "device error", -5: "invalid state"}
@@ -37,7 +40,8 @@ class Options(ctypes.Structure):
                 ("alpha_min", c_double), ("alpha_max", c_double),
                 ("refine_steps", c_int32), ("use_graph", c_int32),
                 ("chunk_landmarks", c_int32), ("group_chunks", c_int32),
-                ("factorization", c_int32), ("pcg_max_iter", c_int32), ("reserved", c_int32 * 6),
+                ("factorization", c_int32), ("pcg_max_iter", c_int32), ("coordinates", c_int32),
+                ("reserved", c_int32 * 5),
                 ("pcg_tol", c_double)]
 
 
@@ -50,7 +54,7 @@ class LayoutInfo(ctypes.Structure):
     _fields_ = [("n_poses", c_int32), ("n_slots", c_int32), ("n_measurements", c_int32),
                 ("n_landmarks", c_int32), ("n_groups", c_int32), ("n_chunks", c_int32),
                 ("max_span", c_int32), ("n_anchors_used", c_int32), ("factorization", c_int32),
-                ("profile_depth", c_int32), ("reduced_size", c_int32), ("reserved", c_int32)]
+                ("profile_depth", c_int32), ("reduced_size", c_int32), ("coordinates", c_int32)]
 
 
 class BatchInfo(ctypes.Structure):

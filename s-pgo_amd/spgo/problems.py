@@ -503,3 +503,96 @@ def make_manager_problem(n: int, m: int, seed: int, arc: float = 0.6, radius: fl
                    uv=uv.astype(np.float32), X0=np.tile(np.eye(4, dtype=np.float32), (N, 1, 1)),
                    alpha0=np.ones(L, np.float32), X_true=X_true, alpha_true=1.0 / z_host, n_landmarks=L,
                    outlier=np.zeros(ma.size, dtype=bool), name=name)
+
+
+def make_revisit_problem(n_poses: int, n_landmarks: int, obs_per_landmark: int, seed: int,
+                         revisit_frac: float = 0.2, revisit_obs: int = 4, fx: float = 500.0, fy: float = 500.0,
+                         cx: float = 320.0, cy: float = 240.0, width: int = 640, height: int = 480,
+                         baseline: float = 0.12, radius: float = 2.0, depth_range=(1.0, 4.0),
+                         noise_px: float = 1.0, rot_perturb: float = 0.05, trans_perturb: float = 0.1,
+                         alpha_perturb: float = 0.1, min_depth: float = 0.2, name: str = "") -> Problem:
+    """A trajectory with loop revisits: the first ``L = round((1 - revisit_frac)
+    (n_poses + 1))`` frames go once round the circle of ``make_long_problem``;
+    the last ``revisit_frac`` of the frames go round it again (1 cm further out),
+    so frame ``f + L`` revisits the place of frame ``f``.  Each landmark is drawn
+    in its host frame's view and observed by the ``obs_per_landmark``
+    consecutive frames from the host on; a landmark whose host is revisited is
+    also observed by up to ``revisit_obs`` consecutive frames from the host's
+    revisit on (those that see it in both cameras), far after its anchor.
+    Noise and initial perturbations as in ``make_problem``."""
+    rng = np.random.default_rng(seed)
+    N, F, K = int(n_poses), int(n_poses) + 1, int(obs_per_landmark)
+    Lp = int(round((1.0 - revisit_frac) * F))
+    f = np.arange(F)
+    theta = 2.0 * math.pi * (f % Lp) / Lp
+    rad = radius + 0.01 * (f // Lp)
+    G = np.stack([look_at_origin(np.array([r * math.cos(a), r * math.sin(a), 0.0])) for r, a in zip(rad, theta)])
+    T_rl = np.eye(4)
+    T_rl[0, 3] = baseline
+    Ginv = _inv(G)
+    GRinv = _inv(G @ T_rl)
+
+    def visible(fr, P):
+        ok = np.ones(P.shape[0], bool)
+        uv = []
+        for Tinv in (Ginv[fr], GRinv[fr]):
+            Pc = np.einsum('nij,nj->ni', Tinv[:, :3, :3], P) + Tinv[:, :3, 3]
+            z = Pc[:, 2]
+            zs = np.where(np.abs(z) < 1e-12, 1e-12, z)
+            u = fx * Pc[:, 0] / zs + cx
+            v = fy * Pc[:, 1] / zs + cy
+            ok &= (z > min_depth) & (u >= 0) & (u < width) & (v >= 0) & (v < height)
+            uv.append(np.stack([u, v], -1))
+        return ok, np.concatenate(uv, 1)
+
+    pts, hosts = [], []
+    need = int(n_landmarks)
+    while need > 0:
+        m = max(2 * need, 1024)
+        a = rng.integers(0, F - K + 1, size=m)
+        u = rng.uniform(0, width, size=m)
+        v = rng.uniform(0, height, size=m)
+        d = rng.uniform(depth_range[0], depth_range[1], size=m)
+        Pc = np.stack([(u - cx) / fx * d, (v - cy) / fy * d, d], -1)
+        P = np.einsum('nij,nj->ni', G[a, :3, :3], Pc) + G[a, :3, 3]
+        ok = np.ones(m, bool)
+        for k in range(K):
+            ok &= visible(a + k, P)[0]
+        idx = np.nonzero(ok)[0][:need]
+        pts.append(P[idx])
+        hosts.append(a[idx])
+        need -= idx.size
+    P = np.concatenate(pts)
+    host = np.concatenate(hosts).astype(np.int64)
+    order = np.argsort(host, kind="stable")
+    P, host = P[order], host[order]
+    L = P.shape[0]
+    ml, mt = [np.repeat(np.arange(L), K)], [(host[:, None] + np.arange(K)[None, :]).reshape(-1)]
+    for j in range(int(revisit_obs)):
+        b = host + Lp + j
+        cand = np.nonzero(b <= N)[0]
+        ok, _ = visible(b[cand], P[cand])
+        ml.append(cand[ok])
+        mt.append(b[cand[ok]])
+    ml, mt = np.concatenate(ml), np.concatenate(mt)
+    o = np.lexsort((mt, ml))
+    ml, mt = ml[o], mt[o]
+    ma = host[ml]
+    _, uv = visible(mt, P[ml])
+    uv = uv + rng.normal(0.0, noise_px, size=uv.shape)
+    z_host = (np.einsum('nij,nj->ni', Ginv[host, :3, :3], P) + Ginv[host, :3, 3])[:, 2]
+    M = int(math.ceil(L / N))
+    alpha_true = np.ones(N * M)
+    alpha_true[:L] = 1.0 / z_host
+    X_true = Ginv[1:] @ G[:-1]
+    xi = np.concatenate([rng.normal(0.0, rot_perturb / math.sqrt(3.0), size=(N, 3)),
+                         rng.normal(0.0, trans_perturb / math.sqrt(3.0), size=(N, 3))], axis=1)
+    X0 = X_true @ _exp_se3(xi)
+    alpha0 = alpha_true.copy()
+    alpha0[:L] *= 1.0 + rng.uniform(-alpha_perturb, alpha_perturb, size=L)
+    intr = np.array([fx, fy, cx, cy, fx, fy, cx, cy], dtype=np.float32)
+    return Problem(n_poses=N, m_per_frame=M, intrinsics=intr, T_r_to_l=T_rl.reshape(16).astype(np.float32),
+                   meas_a=ma.astype(np.int32), meas_t=mt.astype(np.int32), meas_l=ml.astype(np.int32),
+                   uv=uv.astype(np.float32), X0=X0.astype(np.float32), alpha0=alpha0.astype(np.float32),
+                   X_true=X_true, alpha_true=alpha_true, n_landmarks=L,
+                   outlier=np.zeros(ml.size, dtype=bool), name=name)

@@ -134,17 +134,44 @@ def test_envelope_profile_depth_of_the_64_keyframe_problem():
     assert info["factorization"] == _lib.SPGO_FACTOR_ENVELOPE_SPLIT and info["reduced_size"] == 384
 
 
-def test_layout_rejects_mixed_anchors_and_wide_spans():
+def test_layout_coordinates_for_mixed_anchors_and_wide_spans():
     s = Solver(20, 2)
     s.write_observation(0, 0, 1, [1, 1, 1], [1, 1, 1])
     s.write_observation(1, 2, 1, [1, 1, 1], [1, 1, 1])
     with pytest.raises(RuntimeError, match="different anchor"):
         s.layout_info()
+    # a landmark spanning 19 poses: frame coordinates (AUTO), refused by the
+    # incremental layout when that is asked for
     s2 = Solver(20, 2)
     s2.write_observation(0, 0, 1, [1, 1, 1], [1, 1, 1])
     s2.write_observation(0, 19, 1, [1, 1, 1], [1, 1, 1])
+    info = s2.layout_info()
+    assert info["coordinates"] == _lib.SPGO_COORD_FRAMES and info["max_span"] == 19
+    s2.set_options(make_options(coordinates=_lib.SPGO_COORD_INCREMENTAL))
     with pytest.raises(RuntimeError, match="spans"):
         s2.layout_info()
+    s2.set_options(make_options(coordinates=_lib.SPGO_COORD_FRAMES))
+    assert s2.layout_info()["coordinates"] == _lib.SPGO_COORD_FRAMES
+    # short spans stay incremental under AUTO
+    p = make_problem(10, 200, 4, seed=3)
+    assert solver_from_problem(p).layout_info()["coordinates"] == _lib.SPGO_COORD_INCREMENTAL
+    # one landmark in more frames than the frame-coordinate kernel holds
+    s3 = Solver(70, 1)
+    for t in range(0, 67):
+        s3.write_observation(0, t, 0, [1, 1, 1], [1, 1, 1])
+    with pytest.raises(RuntimeError, match="at most 64"):
+        s3.layout_info()
+
+
+def test_rejected_bulk_write_leaves_the_table_unchanged():
+    s = Solver(4, 2)
+    s.write_observations([0, 0], [0, 1], [0, 0], np.ones((2, 4), np.float32))
+    before = s.layout_info()["n_measurements"]
+    with pytest.raises(RuntimeError, match="nothing was written"):
+        s.write_observations([0, 0, 0], [2, 3, 9], [1, 1, 1], np.full((3, 4), 7, np.float32))
+    assert s.layout_info()["n_measurements"] == before
+    l3, _ = s.get_observation(2, 1)
+    assert np.array_equal(l3, np.zeros(3, np.float32))
 
 
 def test_step_without_gpu_fails_loudly():
