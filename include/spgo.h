@@ -191,6 +191,14 @@ int spgo_set_lambda(spgo_solver* h, double lambda);
  * get_H_and_g_* of include/Solver.hpp:68-69 are never defined). */
 int spgo_linearize(spgo_solver* h, double lambda, double* H, double* g, double* cost);
 
+/* The block band of the reduced system spgo_step factors, in the solver's own
+ * coordinates (spgo_layout_info.coordinates), after linearising the current
+ * state with damping lambda: Hband[i][d] = H_{i,i-d} (6x6 row-major,
+ * d = 0..band_blocks, zero where i - d < 0), g (N x 6) and the robust cost.
+ * Blocks farther from the diagonal (loop revisits) are left out.  These are
+ * the systems spgo_lpcg_load_systems takes.  No reference counterpart. */
+int spgo_reduced_band(spgo_solver* h, double lambda, int32_t band_blocks, double* Hband, double* g, double* cost);
+
 /* Host-side layout summary of the measurements written so far (builds the
  * layout without a GPU). */
 int spgo_problem_info(spgo_solver* h, spgo_layout_info* out);

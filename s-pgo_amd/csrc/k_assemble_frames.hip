@@ -13,12 +13,15 @@
 // reduces in the anchor frame as the incremental kernel does:
 //   * pair sums S(s) = sum J_a^T J_a, s(s) = sum J_a^T r per target slot s,
 //   * per landmark C, g_alpha and u_l(s) = sum J_a^T j_alpha per slot,
-// then maps them to world axes with Ad(G_a^-1) and forms, with P = [1 | -I],
-//   W = P^T diag(S_w) P - P^T (sum_l u_l u_l^T / (C_l + eps)) P,
-//   w = P^T s_w - P^T (sum_l u_l g_l / (C_l + eps)),
-// the second terms as one MFMA product of the rows [sum_s u; -u_1; ..; -u_m; g]
-// scaled by 1 / sqrt(C + eps).  Tiles go from the accumulators straight to the
-// group's (6 (m+1))^2 block; k_form_frames sums the blocks into the system.
+// then maps them to world axes with Ad(G_a^-1) and stores the target-frame
+// block and gradient
+//   B = diag(S_w) - sum_l u_l u_l^T / (C_l + eps),  b = s_w - sum_l u_l g_l / (C_l + eps)
+// (6m x 6m and 6m; the second terms one MFMA product of the rows [u_1; ..; u_m; g]
+// scaled by 1 / sqrt(C + eps)).  With P = [1 | -I] the group's part of the
+// system is P^T B P and its gradient P^T b: the anchor's blocks are sums of the
+// stored entries, R(s) = sum_s' B(s, s') and sum_s R(s), formed here in double
+// from the values as stored, so the anchor rows cancel the target rows exactly
+// in the system whatever the storage precision.  k_form_frames sums them.
 // The back-substitution of the next linearisation needs only
 // z_a - z_{f_s}: delta_alpha = (g_alpha - sum_s u_w(s) . (z_a - z_{f_s})) / (C + eps).
 #include "spgo_asm_common.h"
@@ -34,7 +37,7 @@ template <int KT>
 struct FrShape {
     static constexpr int CL = KT <= 16 ? 64 : (KT <= 32 ? 32 : 8);   // landmarks per group
     static constexpr int VS = CL + 2;                                 // V tile row stride
-    static constexpr int K6 = 6 * (KT + 1);                            // block rows: anchor + targets
+    static constexpr int K6 = 6 * KT;                                  // block rows: the target frames
     static constexpr int VROWS = 16 * ((K6 + 1 + 15) / 16);            // and the gradient row, whole tiles
     static constexpr int US = 6 * KT + 1;                              // U row stride (odd: banks)
 };
@@ -68,7 +71,7 @@ __global__ __launch_bounds__(WG, (sizeof(T) == 4 && KT <= 16) ? 2 : 1) void k_as
     const int m = gdp->K;                      // target slots 1..m
     const int prob = gdp->prob;
     const int glm0 = gdp->lm_begin, d_off = gdp->d_off;
-    const long long voff = gdp->v_off, D_off = gdp->D_off, delta_off = gdp->delta_off;
+    const long long voff = gdp->v_off, D_off = gdp->D_off, delta_off = gdp->delta_off, r_off = gdp->r_off;
     const int4 ch = gdp->chunk0;
     const int lm0 = ch.x, nl = ch.y, ob0 = ch.z;
     const int* frames = lay.grp_frames + (long long)g * (KTF + 1);
@@ -80,8 +83,8 @@ __global__ __launch_bounds__(WG, (sizeof(T) == 4 && KT <= 16) ? 2 : 1) void k_as
     const StateBuf<T> bout = A.init ? bin : (cur ? B0 : B1);  // state being linearised
     const bool update = !A.init;
     const bool chol_ok = update ? (ctl->chol_ok != 0) : true;
-    const int K6 = 6 * (m + 1);
-    const int M6 = 6 * m;
+    const int K6 = 6 * m;                      // rows of the target-frame block B
+    const int M6 = K6;
     const int nT = (K6 + 16) >> 4;             // tile rows: K6 block rows and the gradient row
     const int nTiles = nT * (nT + 1) / 2;
 
@@ -101,6 +104,7 @@ __global__ __launch_bounds__(WG, (sizeof(T) == 4 && KT <= 16) ? 2 : 1) void k_as
     __shared__ int frl[KT + 1];
     __shared__ double alnew[CL];
     __shared__ int lidl[CL];
+    __shared__ double Rsh[KT][36];              // R(s) = sum_s' B(s, s') as stored
 
     double cost = 0.0;
     const T fx0 = (T)A.cam.fx[0], fy0 = (T)A.cam.fy[0], cx0 = (T)A.cam.cx[0], cy0 = (T)A.cam.cy[0];
@@ -358,31 +362,23 @@ __global__ __launch_bounds__(WG, (sizeof(T) == 4 && KT <= 16) ? 2 : 1) void k_as
     lds_barrier();
     if (tid < nl) Vsh[K6 * VS + tid] = gl_grad;   // the gradient row
     // u in world axes, stored for the back-substitution, and the rows of V:
-    // [sum_s u; -u_1; ...; -u_m] / sqrt(C + eps)
+    // [u_1; ...; u_m] / sqrt(C + eps)
     for (int it = tid; it < nl * 6; it += WG) {
         const int j = it / 6, q = it - j * 6;
         const T isc = iscl[j];
         T* vo = bout.v + voff + (long long)j * M6 + q;
         const T* uj = U + j * US;
-        T run = (T)0;
         for (int s = 1; s <= m; ++s) {
             T uw = (T)0;
 #pragma unroll
             for (int p = 0; p < 6; ++p) uw += (T)Ainv[6 * p + q] * uj[6 * (s - 1) + p];
             vo[6 * (s - 1)] = uw;
-            Vsh[(6 * s + q) * VS + j] = -uw * isc;
-            run += uw;
+            Vsh[(6 * (s - 1) + q) * VS + j] = uw * isc;
         }
-        Vsh[q * VS + j] = run * isc;
-    }
-    if (tid < 27) {
-        T sum = (T)0;
-        for (int s = 1; s <= m; ++s) sum += Sw[s][tid];
-        Sw[0][tid] = sum;
     }
     lds_barrier();
 
-    // ---- phase 3: V V^T and V g on MFMA, tiles stored as W = P^T S_w P - Y -----
+    // ---- phase 3: V V^T and V g on MFMA, tiles stored as B = diag(S_w) - Y -----
     const int kend = (nl + 3) & ~3;
     T* Dg = bout.D + D_off;
     for (int tile = wv; tile < nTiles; tile += NWAVE) {
@@ -398,18 +394,40 @@ __global__ __launch_bounds__(WG, (sizeof(T) == 4 && KT <= 16) ? 2 : 1) void k_as
             const int row = r * 16 + mfma16_row((T)0, lane, v);
             const int col = c * 16 + (lane & 15);
             if (row < K6 && col < K6) {
-                const int sr = row / 6, sc = col / 6, k = sym6(row % 6, col % 6);
-                T q = (T)0;
-                if (sr == 0 && sc == 0) q = Sw[0][k];
-                else if (sr == 0 || sc == 0) q = -Sw[sr + sc][k];
-                else if (sr == sc) q = Sw[sr][k];
+                const int sr = row / 6, sc = col / 6;
+                const T q = sr == sc ? Sw[sr + 1][sym6(row % 6, col % 6)] : (T)0;
                 Dg[(long long)row * K6 + col] = q - acc[v];
             } else if (row == K6 && col < K6) {
-                const int sc = col / 6, q6 = col % 6;
-                const T gq = sc == 0 ? Sw[0][21 + q6] : -Sw[sc][21 + q6];
-                bout.d[d_off + col] = gq - acc[v];
+                bout.d[d_off + col] = Sw[col / 6 + 1][21 + col % 6] - acc[v];
             }
         }
+    }
+    // the anchor's blocks from the stored values (the barrier makes this
+    // workgroup's global stores visible to its own loads): R(s) row-block sums
+    // of B, their total, and the gradient total, in double
+    __syncthreads();
+    double* Rg = bout.Rz + r_off;
+    for (int it = tid; it < 36 * m; it += WG) {
+        const int s0 = it / 36, e = (it / 6) % 6, j = it % 6;
+        const int x = 6 * s0 + e;
+        double acc = 0.0;
+        for (int s1 = 0; s1 < m; ++s1) {
+            const int y = 6 * s1 + j;
+            acc += (double)(x >= y ? Dg[(long long)x * K6 + y] : Dg[(long long)y * K6 + x]);
+        }
+        Rsh[s0][6 * e + j] = acc;
+        Rg[36 * (s0 + 1) + 6 * e + j] = acc;
+    }
+    if (tid >= 192 && tid < 198) {
+        double bt = 0.0;
+        for (int s1 = 0; s1 < m; ++s1) bt += (double)bout.d[d_off + 6 * s1 + (tid - 192)];
+        Rg[36 * (m + 1) + (tid - 192)] = bt;
+    }
+    lds_barrier();
+    if (tid < 36) {
+        double tot = 0.0;
+        for (int s1 = 0; s1 < m; ++s1) tot += Rsh[s1][tid];
+        Rg[tid] = tot;
     }
     cost = wave_sum(cost);
     if (lane == 0) red[wv] = cost;

@@ -567,7 +567,8 @@ __global__ void k_lp_band_from_dense(int N, int W, const double* H, int Pp, doub
 __global__ __launch_bounds__(256) void k_lp_pcg_coop(int T, int W, const double* Hb, const double* Gd,
                                                      const double* Go, const double* g, double* x, double* r,
                                                      double* z, double* p, double* q, double* tmp, double* part,
-                                                     double tol, int maxit, Control* ctl, const int* bad) {
+                                                     double tol, int maxit, Control* ctl, const int* bad,
+                                                     const int* far_beg, const int* far_col, const double* far_val) {
     // Two grid barriers per iteration.  After p.q gives alpha, every thread
     // forms the new residual of the block rows it needs (i - 1 .. i + 1) from
     // the old one and q, and with it z = G^T G r locally: no barrier inside the
@@ -649,6 +650,17 @@ __global__ __launch_bounds__(256) void k_lp_pcg_coop(int T, int W, const double*
                     su += bu[k][b] * vu[k][b];
                 }
                 y += wl[k] * sl + wu[k] * su;
+            }
+        }
+        if (far_beg) {
+            // blocks outside the band (both triangles listed)
+            for (int k = far_beg[i]; k < far_beg[i + 1]; ++k) {
+                const double* B = far_val + 36LL * k + 6 * a;
+                const double* V = v + 6LL * far_col[k];
+                double s = 0.0;
+#pragma unroll
+                for (int b = 0; b < 6; ++b) s += B[b] * V[b];
+                y += s;
             }
         }
         return y;
@@ -855,9 +867,32 @@ int lp_coop_max_blocks() {
 
 int lp_launch_pcg_coop(int T, int W, const double* Hb, const double* Gd, const double* Go, const double* g, double* x,
                        double* r, double* z, double* p, double* q, double* tmp, double* part, int blocks, double tol,
-                       int maxit, Control* ctl, const int* bad, hipStream_t st) {
-    void* args[] = {&T, &W, &Hb, &Gd, &Go, &g, &x, &r, &z, &p, &q, &tmp, &part, &tol, &maxit, &ctl, &bad};
+                       int maxit, Control* ctl, const int* bad, const int* far_beg, const int* far_col,
+                       const double* far_val, hipStream_t st) {
+    void* args[] = {&T, &W, &Hb, &Gd, &Go, &g, &x, &r, &z, &p, &q, &tmp, &part, &tol, &maxit, &ctl, &bad,
+                    &far_beg, &far_col, &far_val};
     return (int)hipLaunchCooperativeKernel((const void*)k_lp_pcg_coop, dim3(blocks), dim3(256), args, 0, st);
+}
+
+// far_val[k] = H_{i,j} (i = far_row[k], j = far_col[k]) from the lower triangle
+__global__ void k_lp_far_from_dense(int nfar, const int* far_row, const int* far_col, const double* H, int Pp,
+                                    double* far_val) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nfar * 36) return;
+    const int k = (int)(t / 36), e = (int)(t % 36);
+    int row = 6 * far_row[k] + e / 6, col = 6 * far_col[k] + e % 6;
+    if (col > row) {
+        const int tmp = row;
+        row = col;
+        col = tmp;
+    }
+    far_val[t] = H[(long long)row * Pp + col];
+}
+
+void lp_launch_far_from_dense(int nfar, const int* far_row, const int* far_col, const double* H, int Pp,
+                              double* far_val, hipStream_t st) {
+    hipLaunchKernelGGL(k_lp_far_from_dense, dim3(nblk((long long)nfar * 36)), dim3(256), 0, st, nfar, far_row,
+                       far_col, H, Pp, far_val);
 }
 
 void lp_launch_zero_flag(int* p, hipStream_t st) { hipLaunchKernelGGL(k_lp_zero_flag, dim3(1), dim3(1), 0, st, p); }

@@ -118,8 +118,8 @@ __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0,
 
 // The reduced system in frame coordinates (one problem): workgroup r writes the
 // six rows of frame r, every element left of the diagonal (zeros outside the
-// pattern), as the sum of the group-block entries the layout's plan lists for
-// the block (r, c), plus the damping lambda T, T = D^T diag(F_i) D with
+// pattern), as the sum of the group contributions the layout's plan lists for
+// the block (r, c) (ZCon), plus the damping lambda T, T = D^T diag(F_i) D with
 // (D z)_i = z_i - z_{i+1} and F_i = Ad(G_i^-1)^T Ad(G_i^-1): lambda ||delta||^2
 // in these variables (T_rr = F_r + F_{r-1}, T_{r,r-1} = -F_{r-1}).
 template <typename T>
@@ -174,9 +174,14 @@ __global__ __launch_bounds__(256) void k_form_frames(Layout lay, StateBuf<T> B0,
             const int kb = lay.zp_cbeg[k], ke = lay.zp_cbeg[k + 1];
             for (int q = kb; q < ke; ++q) {
                 const ZCon z = lay.zp_con[q];
-                const long long idx = (z.doff >= 0 && j > e) ? z.off + (long long)j * z.K6 + e
-                                                             : z.off + (long long)e * z.K6 + j;
-                h += (double)bs.D[idx];
+                if (z.K6 > 0) {
+                    const long long idx = (z.doff >= 0 && j > e) ? z.off + (long long)j * z.K6 + e
+                                                                 : z.off + (long long)e * z.K6 + j;
+                    h += (double)bs.D[idx];
+                } else {
+                    const double v = bs.Rz[z.off + 6 * e + j];
+                    h += z.K6 == -2 ? v : -v;
+                }
             }
         }
         if (lam != 0.0) {
@@ -190,7 +195,11 @@ __global__ __launch_bounds__(256) void k_form_frames(Layout lay, StateBuf<T> B0,
         double gs = 0.0;
         if (ne > 0 && lay.zp_col[e0 + ne - 1] == r) {
             const int k = e0 + ne - 1;
-            for (int q = lay.zp_cbeg[k]; q < lay.zp_cbeg[k + 1]; ++q) gs += (double)bs.d[lay.zp_con[q].doff + tid];
+            for (int q = lay.zp_cbeg[k]; q < lay.zp_cbeg[k + 1]; ++q) {
+                const ZCon z = lay.zp_con[q];
+                if (z.K6 == -2) gs += bs.Rz[z.doff + tid];
+                else gs -= (double)bs.d[z.doff + tid];
+            }
         }
         gvec[6 * r + tid] = gs;
     }

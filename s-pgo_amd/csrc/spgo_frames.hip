@@ -8,6 +8,9 @@
 // (Solver_cuda.cu:248-309 accepts any target after the anchor).  A group is
 // the landmarks of one anchor frame whose distinct target frames together
 // number at most kt (16, 32 or 64, the kernel shape), within one LDS chunk.
+// Per group the device keeps the target-frame block B (6m x 6m), its gradient
+// b (6m) and, in double, the anchor's blocks R(s) = sum_s' B(s, s') (row sums
+// of B by target frame), their total and the gradient total.
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -161,9 +164,11 @@ int build_layout_frames(spgo_solver* h, HostLayout& out) {
         hl.grp_v_off.push_back(hl.v_total);
         hl.grp_D_off.push_back(hl.D_total);
         hl.grp_d_off.push_back((int)hl.d_total);
+        hl.grp_r_off.push_back(hl.Rz_total);
         hl.v_total += (long long)nl * 6 * m;
-        hl.D_total += (long long)(6 * (m + 1)) * (6 * (m + 1));
-        hl.d_total += 6 * (m + 1);
+        hl.D_total += (long long)(6 * m) * (6 * m);
+        hl.d_total += 6 * m;
+        hl.Rz_total += 36LL * (m + 1) + 6;
         const size_t fb = hl.grp_frames.size();
         hl.grp_frames.resize(fb + KTF + 1, -1);
         hl.grp_frames[fb] = a;
@@ -188,6 +193,7 @@ int build_layout_frames(spgo_solver* h, HostLayout& out) {
         d.d_off = hl.grp_d_off[g];
         d.v_off = hl.grp_v_off[g];
         d.D_off = hl.grp_D_off[g];
+        d.r_off = hl.grp_r_off[g];
         d.chunk0 = hl.chunk[d.chunk_begin];
         for (int sl = 0; sl < KMAX + 2; ++sl)
             d.soff0[sl] = hl.chunk_soff[(size_t)d.chunk_begin * SO + std::min(sl, SO - 1)];
@@ -225,11 +231,22 @@ int build_layout_frames(spgo_solver* h, HostLayout& out) {
             hl.fpose[r] = std::min(hl.fpose[r], c);
             for (; q < ent.size() && ent[q].r == r && ent[q].c == c; ++q) {
                 const Ent& e = ent[q];
-                const int K6 = 6 * (hl.grp_K[e.g] + 1);
+                const int m = hl.grp_K[e.g], K6 = 6 * m;
+                const long long ro = hl.grp_r_off[e.g];
                 ZCon z;
-                z.off = hl.grp_D_off[e.g] + (long long)(6 * e.sr) * K6 + 6 * e.sc;
-                z.K6 = K6;
-                z.doff = e.sr == e.sc ? hl.grp_d_off[e.g] + 6 * e.sr : -1;
+                if (e.sc > 0) {          // two target frames: a block of B
+                    z.off = hl.grp_D_off[e.g] + (long long)(6 * (e.sr - 1)) * K6 + 6 * (e.sc - 1);
+                    z.K6 = K6;
+                    z.doff = e.sr == e.sc ? hl.grp_d_off[e.g] + 6 * (e.sr - 1) : -1;
+                } else if (e.sr > 0) {   // target frame against the anchor: -R(s)
+                    z.off = ro + 36LL * e.sr;
+                    z.K6 = -1;
+                    z.doff = -1;
+                } else {                 // the anchor itself
+                    z.off = ro;
+                    z.K6 = -2;
+                    z.doff = (int)(ro + 36LL * (m + 1));
+                }
                 hl.zp_con.push_back(z);
             }
         }

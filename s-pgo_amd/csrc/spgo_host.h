@@ -62,6 +62,8 @@ struct HostLayout {
     int frame_mode = 0;
     int kt = 0, cl = 0;             // kernel shape: target frames per group, landmarks per chunk
     std::vector<int> grp_frames;    // per group KTF + 1
+    std::vector<long long> grp_r_off;   // per group: offset of its anchor blocks in Rz
+    long long Rz_total = 0;
     std::vector<int> zp_row, zp_col, zp_cbeg;
     std::vector<ZCon> zp_con;
 };

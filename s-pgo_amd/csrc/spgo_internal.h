@@ -22,7 +22,8 @@ struct alignas(16) GroupDesc {
     int anchor, K, chunk_begin, chunk_end;   // anchor: frame index in the state arrays
     int lm_begin, d_off, prob, a_loc;        // prob: problem of a batch (0 alone); a_loc: anchor inside it
     long long v_off, D_off;
-    long long delta_off, pad2;               // first entry of the problem's reduced-system solution
+    long long delta_off;                     // first entry of the problem's reduced-system solution
+    long long r_off;                         // frame coordinates: the group's anchor blocks in Rz
     int4 chunk0;                  // (first active landmark, n landmarks, first obs, n obs)
     int soff0[KMAX + 2];          // slot offsets of the first chunk
 };
@@ -35,10 +36,13 @@ struct FsEntry {
 };
 
 // Frame coordinates: one contribution to a block (row frame r, column frame c)
-// of the reduced system, an entry of a landmark group's block: element (e, j)
-// of the 6x6 block is at off + e K6 + j (for a diagonal block, e >= j; the
-// upper entries are read transposed).  doff: first of the group's six gradient
-// entries of frame r, -1 off the diagonal.
+// of the reduced system from a landmark group.  K6 > 0: a block of the group's
+// target-frame block B (StateBuf::D), element (e, j) at off + e K6 + j (a
+// diagonal block is read from its lower triangle); K6 = -1: minus the row sum
+// R(s) of the target frame r against the anchor c (StateBuf::Rz, off + 6e + j);
+// K6 = -2: the anchor's own block sum_s R(s).  doff, on the diagonal only: the
+// gradient, minus b(s) at d[doff ..] (K6 > 0) or plus sum_s b(s) at Rz[doff ..]
+// (K6 = -2); -1 off the diagonal.
 struct ZCon {
     long long off;
     int K6, doff;
@@ -136,6 +140,7 @@ struct StateBuf {
     T* D;             // per group (6K)^2 reduced block
     T* d;             // per group 6K reduced gradient
     double* cost;     // per group robust cost
+    double* Rz;       // frame coordinates: per group the anchor's blocks (see ZCon), double
 };
 
 }  // namespace spgo

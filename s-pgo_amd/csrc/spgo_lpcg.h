@@ -65,9 +65,16 @@ void lp_launch_zero_flag(int* p, hipStream_t st);   // a kernel, so a captured g
 // lp_coop_max_blocks(), all resident); part holds 4 x blocks doubles.  Returns
 // the launch status.  Sets ctl->pcg_iters and ctl->pcg_fail.
 int lp_coop_max_blocks();
+// Blocks of H outside the band (loop revisits in frame coordinates) join the
+// product H v: far_beg (T + 1) ranges per block row into far_col / far_val
+// (36 values each, H_{i,j} row-major, both triangles listed); far_beg null: none.
 int lp_launch_pcg_coop(int T, int W, const double* Hb, const double* Gd, const double* Go, const double* g, double* x,
                        double* r, double* z, double* p, double* q, double* tmp, double* part, int blocks, double tol,
-                       int maxit, Control* ctl, const int* bad, hipStream_t st);
+                       int maxit, Control* ctl, const int* bad, const int* far_beg, const int* far_col,
+                       const double* far_val, hipStream_t st);
+// far_val[k] = H_{far_row[k], far_col[k]} from the lower triangle of a dense H
+void lp_launch_far_from_dense(int nfar, const int* far_row, const int* far_col, const double* H, int Pp,
+                              double* far_val, hipStream_t st);
 
 // per-node buffers of one network forward pass
 struct LpNet {
@@ -96,8 +103,15 @@ struct LpSolve {
     int *bad = nullptr, *node_sys = nullptr, *sys_begin = nullptr;
     double* part = nullptr;   // cooperative PCG: 4 x coop_blocks partial sums
     int coop_blocks = 0;
+    // blocks outside the band: per block row (N + 1) ranges, their columns and rows
+    int nfar = 0;
+    int *far_beg = nullptr, *far_col = nullptr, *far_row = nullptr;
+    double* far_val = nullptr;
     std::vector<void*> ptrs;
-    int setup(int n_poses, int band_blocks);   // 0 or a HIP error
+    // far_beg / far_col: host CSR of the blocks outside the band (both
+    // triangles), empty for a banded system
+    int setup(int n_poses, int band_blocks, const std::vector<int>& far_beg_h = {},
+              const std::vector<int>& far_col_h = {});   // 0 or a HIP error
     void release();
 };
 // x = CG solution of the leading 6N x 6N block of the dense H (lower triangle,

@@ -185,7 +185,7 @@ void lp_forward(const LpSet& s, const float* P, const LpNet& b, hipStream_t st) 
                      b.OUT, nullptr, st);
 }
 
-int LpSolve::setup(int n_poses, int band_blocks) {
+int LpSolve::setup(int n_poses, int band_blocks, const std::vector<int>& far_beg_h, const std::vector<int>& far_col_h) {
     release();
     N = n_poses;
     W = band_blocks;
@@ -231,6 +231,21 @@ int LpSolve::setup(int n_poses, int band_blocks) {
     hipError_t e = hipMemcpy(node_sys, ns.data(), T * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(sys_begin, sb, 8, hipMemcpyHostToDevice);
     if (e != hipSuccess) return (int)e;
+    nfar = far_col_h.empty() ? 0 : (int)far_col_h.size();
+    if (nfar > 0) {
+        std::vector<int> rows(nfar);
+        for (int i = 0; i < N; ++i)
+            for (int k = far_beg_h[i]; k < far_beg_h[i + 1]; ++k) rows[k] = i;
+        far_beg = (int*)take((size_t)(N + 1) * 4);
+        far_col = (int*)take((size_t)nfar * 4);
+        far_row = (int*)take((size_t)nfar * 4);
+        far_val = (double*)take((size_t)nfar * 36 * 8);
+        if (!far_beg || !far_col || !far_row || !far_val) return (int)hipErrorOutOfMemory;
+        e = hipMemcpy(far_beg, far_beg_h.data(), (size_t)(N + 1) * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(far_col, far_col_h.data(), (size_t)nfar * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(far_row, rows.data(), (size_t)nfar * 4, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return (int)e;
+    }
     set = LpSet{N, 1, W, Hb, nullptr, node_sys, sys_begin};
     return 0;
 }
@@ -252,8 +267,10 @@ int lp_solve_launch(const LpSolve& b, const double* H, int Pp, const double* g, 
         lp_forward(s, P, b.net, st);
         lp_launch_buildG(s, b.net.OUT, b.C, b.Gd, b.Go, st);
     }
+    if (b.nfar > 0) lp_launch_far_from_dense(b.nfar, b.far_row, b.far_col, H, Pp, b.far_val, st);
     return lp_launch_pcg_coop(b.N, b.W, b.Hb, P ? b.Gd : b.C, P ? b.Go : nullptr, g, x, b.r, b.z, b.p, b.q, b.tmp,
-                              b.part, b.coop_blocks, tol, maxit, ctl, b.bad, st);
+                              b.part, b.coop_blocks, tol, maxit, ctl, b.bad, b.nfar > 0 ? b.far_beg : nullptr,
+                              b.far_col, b.far_val, st);
 }
 
 const float* lpcg_device_params(spgo_lpcg* h) {

@@ -377,8 +377,9 @@ int alloc_state(spgo_solver* h, StateBuf<T>* b, const HostLayout& hl, int n_act)
         b[i].D = h->mem.alloc<T>((size_t)hl.D_total);
         b[i].d = h->mem.alloc<T>((size_t)hl.d_total);
         b[i].cost = h->mem.alloc<double>(hl.grp_anchor.size());
+        b[i].Rz = hl.frame_mode ? h->mem.alloc<double>((size_t)hl.Rz_total) : nullptr;
         if (!b[i].X || !b[i].alpha || !b[i].G || !b[i].GinvL || !b[i].GinvR || !b[i].v || !b[i].Ct || !b[i].g ||
-            !b[i].D || !b[i].d || !b[i].cost)
+            !b[i].D || !b[i].d || !b[i].cost || (hl.frame_mode && !b[i].Rz))
             return h->fail(SPGO_E_DEVICE, "device allocation failed");
     }
     return SPGO_OK;
@@ -470,10 +471,31 @@ int setup_layout(spgo_solver* h) {
             // CG over the pose blocks: spans of K poses couple blocks up to K - 1 apart
             const int need = std::max(1, L.band_need);
             const int W = h->net ? lpcg_band_blocks(h->net) : std::min(need, LP_WMAX);
-            if (need > W)
+            std::vector<int> far_beg, far_col;
+            if (need > W && !L.frame_mode)
                 return h->fail(SPGO_E_LAYOUT, "landmark spans need a band of " + std::to_string(need) +
                                                   " blocks; the preconditioner network covers " + std::to_string(W));
-            if (h->lps.setup(h->N, W) != 0) return h->fail(SPGO_E_DEVICE, "device allocation failed (CG buffers)");
+            if (need > W) {
+                // frame coordinates: the blocks farther than W from the diagonal
+                // (loop revisits) join the CG product, not the preconditioner
+                std::vector<std::vector<int>> rows(h->N);
+                for (int r = 0; r < h->N; ++r)
+                    for (int k = L.zp_row[r]; k < L.zp_row[r + 1]; ++k) {
+                        const int c = L.zp_col[k];
+                        if (r - c > W) {
+                            rows[r].push_back(c);
+                            rows[c].push_back(r);
+                        }
+                    }
+                far_beg.assign(h->N + 1, 0);
+                for (int r = 0; r < h->N; ++r) {
+                    std::sort(rows[r].begin(), rows[r].end());
+                    far_beg[r + 1] = far_beg[r] + (int)rows[r].size();
+                    far_col.insert(far_col.end(), rows[r].begin(), rows[r].end());
+                }
+            }
+            if (h->lps.setup(h->N, W, far_beg, far_col) != 0)
+                return h->fail(SPGO_E_DEVICE, "device allocation failed (CG buffers)");
             if (!h->pcg_flag_host &&
                 hipHostMalloc((void**)&h->pcg_flag_host, sizeof(int), hipHostMallocDefault) != hipSuccess) {
                 h->pcg_flag_host = nullptr;
@@ -1205,6 +1227,38 @@ int spgo_linearize(spgo_solver* h, double lambda, double* H, double* g, double* 
         if (g) std::memcpy(g, gp.data(), P * sizeof(double));
     }
     rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (rc) return rc;
+    if (cost) *cost = c.cost;
+    c.lambda = lam_keep;
+    return check_hip(h, hipMemcpy(h->ctl, &c, sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+int spgo_reduced_band(spgo_solver* h, double lambda, int32_t band_blocks, double* Hband, double* g, double* cost) {
+    if (!h || !(lambda >= 0.0) || band_blocks < 0 || !Hband || !g) return SPGO_E_ARG;
+    int rc = run_step(h, 0);
+    if (rc) return rc;
+    Control c;
+    rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (rc) return rc;
+    const double lam_keep = c.lambda;
+    c.lambda = lambda;
+    rc = check_hip(h, hipMemcpy(h->ctl, &c, sizeof(Control), hipMemcpyHostToDevice), "hipMemcpy");
+    if (rc) return rc;
+    if (h->prec == SPGO_PREC_MIXED)
+        launch_form_system<float>(h->lay, h->bf, h->ctl, h->H, h->gvec, h->stream);
+    else
+        launch_form_system<double>(h->lay, h->bd, h->ctl, h->H, h->gvec, h->stream);
+    const size_t nb = (size_t)h->N * (band_blocks + 1) * 36;
+    double* Hb = nullptr;
+    rc = check_hip(h, hipMalloc(&Hb, nb * sizeof(double)), "hipMalloc");
+    if (rc) return rc;
+    lp_launch_band_from_dense(h->N, band_blocks, h->H, h->Pp, Hb, h->stream);
+    rc = check_hip(h, hipStreamSynchronize(h->stream), "reduced band");
+    if (!rc) rc = check_hip(h, hipMemcpy(Hband, Hb, nb * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+    (void)hipFree(Hb);
+    if (!rc) rc = check_hip(h, hipMemcpy(g, h->gvec, (size_t)6 * h->N * sizeof(double), hipMemcpyDeviceToHost),
+                            "hipMemcpy");
+    if (!rc) rc = check_hip(h, hipMemcpy(&c, h->ctl, sizeof(Control), hipMemcpyDeviceToHost), "hipMemcpy");
     if (rc) return rc;
     if (cost) *cost = c.cost;
     c.lambda = lam_keep;

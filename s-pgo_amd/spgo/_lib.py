@@ -93,6 +93,7 @@ SIGNATURES = {
     "spgo_get_lambda": (c_int32, [c_void_p, F64P]),
     "spgo_set_lambda": (c_int32, [c_void_p, c_double]),
     "spgo_linearize": (c_int32, [c_void_p, c_double, F64P, F64P, F64P]),
+    "spgo_reduced_band": (c_int32, [c_void_p, c_double, c_int32, F64P, F64P, POINTER(c_double)]),
     "spgo_problem_info": (c_int32, [c_void_p, POINTER(LayoutInfo)]),
     "spgo_last_step_ms": (c_int32, [c_void_p, F64P]),
     "spgo_profile_phases": (c_int32, [c_void_p, c_int32, F64P]),

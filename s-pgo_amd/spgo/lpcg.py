@@ -2,10 +2,10 @@
 
 ``LearnedPCG`` owns one ``spgo_lpcg`` handle of libspgo.so: the network
 parameters with their Adam state and a loaded set of block-banded systems.
-Systems come from the solver itself: ``Solver.linearize`` returns the reduced
-system H and right-hand side g at the solver's current state, and
-``band_system`` keeps the blocks H_{i,i-d}, d <= W, that the solver's landmark
-spans can fill.
+Systems come from the solver itself: ``Solver.reduced_band`` returns the
+blocks H_{i,i-d}, d <= W, of the reduced system the step factors and its
+right-hand side g at the solver's current state, in the solver's own
+coordinates (``band_system`` does the same for a dense H given on the host).
 """
 from __future__ import annotations
 
@@ -128,17 +128,34 @@ class LearnedPCG:
 
 
 def solver_systems(problem_list, W: int, states_per_problem: int = 2, lam: float = 1e-4, iters_between: int = 2,
-                   options=None) -> list:
-    """Reduced systems generated by the solver: for each problem, the system at
-    its initial state and after ``iters_between`` LM iterations, repeated
-    ``states_per_problem`` times."""
+                   options=None, restarts: int = 1, perturb=(0.01, 0.05), seed: int = 0) -> list:
+    """Reduced systems generated by the solver, in its own coordinates (the
+    block band of offsets <= W that the CG preconditioner sees, as
+    ``Solver.reduced_band``).  For each problem: ``restarts`` starting states
+    (the first the problem's initial state, the others its true state with
+    every pose perturbed by a twist of RMS size drawn in ``perturb`` rad / m and
+    inverse depths by up to 10 %), from each the system at the start and after
+    every ``iters_between`` further LM iterations, ``states_per_problem``
+    systems per start."""
     from .solver import make_options, solver_from_problem
+    from .problems import _exp_se3
+    rng = np.random.default_rng(seed)
     out = []
     for p in problem_list:
         s = solver_from_problem(p, options if options is not None else make_options(mode="lm"))
-        for k in range(states_per_problem):
-            if k:
-                s.step(iters_between)
-            H, g, _ = s.linearize(lam)
-            out.append(band_system(H, g, W))
+        used = np.unique(p.meas_l)
+        for r in range(restarts):
+            if r:
+                sc = rng.uniform(perturb[0], perturb[1])
+                xi = rng.normal(0.0, sc / np.sqrt(3.0), size=(p.n_poses, 6))
+                X = p.X_true @ _exp_se3(xi)
+                al = p.alpha_true.copy()
+                al[used] *= 1.0 + rng.uniform(-0.1, 0.1, size=used.size)
+                s.set_state(X, al)
+            for k in range(states_per_problem):
+                if k:
+                    s.step(iters_between)
+                Hb, g, _ = s.reduced_band(W, lam)
+                out.append((Hb, g))
+        s.close()
     return out

@@ -203,6 +203,17 @@ class Solver:
                                              ctypes.byref(c)))
         return H, g, c.value
 
+    def reduced_band(self, W: int, lam: float = 0.0):
+        """(Hband (N, W+1, 6, 6), g (N, 6), cost): the block band of the reduced
+        system the step factors, in the solver's own coordinates, at the
+        current state with damping ``lam``."""
+        Hb = np.empty((self.N, W + 1, 6, 6), np.float64)
+        g = np.empty((self.N, 6), np.float64)
+        c = ctypes.c_double()
+        self._check(self._lib.spgo_reduced_band(self._h, float(lam), int(W), Hb.ctypes.data_as(F64P),
+                                                g.ctypes.data_as(F64P), ctypes.byref(c)))
+        return Hb, g, c.value
+
     def layout_info(self) -> dict:
         info = LayoutInfo()
         self._check(self._lib.spgo_problem_info(self._h, ctypes.byref(info)))
