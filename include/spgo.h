@@ -219,6 +219,11 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
 /* Message of the last failed call on h (h == NULL: last failed spgo_create). */
 const char* spgo_last_error(spgo_solver* h);
 
+/* Wait for all work the library queued on the device (every solver, batch and
+ * network) to finish; SPGO_E_DEVICE without a device.  No reference
+ * counterpart (the reference's step is synchronous, Solver_cuda.cu:557-795). */
+int spgo_device_synchronize(void);
+
 struct spgo_lpcg;
 /* The learned preconditioner of SPGO_FACTOR_PCG (NULL: block Jacobi).  The
  * solver reads the network's parameters at every step, so training it later

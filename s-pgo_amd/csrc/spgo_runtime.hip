@@ -905,6 +905,8 @@ int spgo_create(int32_t n_poses, int32_t m_per_frame, const spgo_options* opts, 
 
 int spgo_destroy(spgo_solver* h) {
     if (!h) return SPGO_E_ARG;
+    // the solver's queued work finishes before its memory goes
+    if (h->dev_ready) (void)hipStreamSynchronize(h->stream);
     destroy_graph(h);
     h->mem.release();
     h->lps.release();
@@ -1378,6 +1380,12 @@ int spgo_last_step_ms(spgo_solver* h, double* ms) {
 const char* spgo_last_error(spgo_solver* h) {
     if (!h) return g_create_error.c_str();
     return h->err.c_str();
+}
+
+int spgo_device_synchronize(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SPGO_E_DEVICE;
+    return hipDeviceSynchronize() == hipSuccess ? SPGO_OK : SPGO_E_DEVICE;
 }
 
 }  // extern "C"

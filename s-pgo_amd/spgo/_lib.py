@@ -6,8 +6,10 @@ load, every call raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_uint64, c_void_p
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -94,6 +96,7 @@ SIGNATURES = {
     "spgo_set_lambda": (c_int32, [c_void_p, c_double]),
     "spgo_linearize": (c_int32, [c_void_p, c_double, F64P, F64P, F64P]),
     "spgo_reduced_band": (c_int32, [c_void_p, c_double, c_int32, F64P, F64P, POINTER(c_double)]),
+    "spgo_device_synchronize": (c_int32, []),
     "spgo_problem_info": (c_int32, [c_void_p, POINTER(LayoutInfo)]),
     "spgo_last_step_ms": (c_int32, [c_void_p, F64P]),
     "spgo_profile_phases": (c_int32, [c_void_p, c_int32, F64P]),
@@ -145,6 +148,35 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+# Every live handle owner (Solver, Batch, LearnedPCG), so that what is still
+# open when the interpreter exits is destroyed by an atexit handler -- before
+# module teardown and before the HIP runtime's own exit handlers -- instead of
+# from __del__ during finalisation, when the order of teardown is not defined.
+_live = weakref.WeakSet()
+_CLOSE_ORDER = {"Batch": 0, "Solver": 1, "LearnedPCG": 2}   # a batch uses solvers, a solver a network
+
+
+def register(obj) -> None:
+    _live.add(obj)
+
+
+@atexit.register
+def close_all() -> None:
+    """Destroy every handle still open, batches first, then solvers, then
+    networks, and wait for the device."""
+    objs = sorted(list(_live), key=lambda o: _CLOSE_ORDER.get(type(o).__name__, 1))
+    for o in objs:
+        try:
+            o.close()
+        except Exception:
+            pass
+    if _lib is not None and objs:
+        try:
+            _lib.spgo_device_synchronize()
+        except Exception:
+            pass
 
 
 def check(rc: int, handle=None, batch: bool = False, kind: str = "") -> None:

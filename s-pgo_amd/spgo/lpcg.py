@@ -46,6 +46,7 @@ class LearnedPCG:
 
     def __init__(self, band_blocks: int, seed: int = 0):
         self._lib = _lib.load()
+        _lib.register(self)
         self.W = int(band_blocks)
         h = ctypes.c_void_p()
         _lib.check(self._lib.spgo_lpcg_create(self.W, int(seed), ctypes.byref(h)), None, kind="lpcg")

@@ -52,6 +52,7 @@ class Solver:
 
     def __init__(self, N: int, M: int, options: Optional[Options] = None):
         self._lib = _lib.load()
+        _lib.register(self)
         self.N = int(N)
         self.M = int(M)
         h = ctypes.c_void_p()
@@ -264,6 +265,7 @@ class Batch:
 
     def __init__(self, solvers, options: Optional[Options] = None):
         self._lib = _lib.load()
+        _lib.register(self)
         self.solvers = list(solvers)
         hs = (ctypes.c_void_p * len(self.solvers))(*[s._h.value for s in self.solvers])
         h = ctypes.c_void_p()
