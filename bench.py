@@ -316,6 +316,43 @@ def oracle_sample_rate(p, opts, n_frames: int):
     return n_sub / (per_iter * p.n_obs), per_iter, n_sub
 
 
+def reference_cpu_path_rate(n_timed: int = 5):
+    """The reference's CPU path (Optimizer.step restated, oracle/cpu_path.py)
+    on the 10-keyframe, 500-landmark problem: seconds per call, median of
+    ``n_timed`` after one warm-up with the default BLAS threads, and one call
+    with one BLAS thread."""
+    import contextlib
+    import statistics
+    from oracle import cpu_path as CP
+    from spgo.problems import make_config
+
+    p1 = make_config("c1_small")
+    c = CP.from_problem(p1)
+    c.step()
+    ts = []
+    for _ in range(n_timed):
+        t0 = time.perf_counter()
+        c.step()
+        ts.append(time.perf_counter() - t0)
+    try:
+        from threadpoolctl import threadpool_info, threadpool_limits
+        blas = max([i.get("num_threads", 1) for i in threadpool_info()] + [1])
+        limit = threadpool_limits(1)
+    except Exception:
+        blas, limit = None, contextlib.nullcontext()
+    with limit:
+        t0 = time.perf_counter()
+        c.step()
+        t1 = time.perf_counter() - t0
+    med = statistics.median(ts)
+    return {"value": 1.0 / med, "unit": "LM iterations/s", "cores": blas, "host_cpu_count": os.cpu_count(),
+            "one_thread_value": 1.0 / t1, "kind": "port",
+            "sample": f"the reference CPU path (Optimizer.step, PythonUtils/Optimizer.py:135-288, restated in "
+                      f"float64 numpy with its Python loops and dense Jacobians) on 10 keyframes / 500 landmarks / "
+                      f"{p1.n_obs} stereo measurements: median of {n_timed} calls after one warm-up "
+                      f"({med:.2f} s), BLAS at {blas} threads; one call with one BLAS thread {t1:.2f} s"}
+
+
 def chol_roofline(p, phases, info):
     """Roofline object of the reduced-system factorisation phase."""
     envelope = info["factorization"] in (2, 3)
@@ -396,7 +433,7 @@ def bench_c3(args) -> int:
         if dom != "assemble":
             roof["note"] = f"dominant phase is {dom}; roofline given for the assembly kernel"
 
-    cpu = None
+    cpu = ref_cpu = None
     if not args.no_cpu:
         oo = O.Options(robust="huber", robust_param=2.0)
         rate, per_iter, n_sub = oracle_sample_rate(p, oo, 64)
@@ -406,6 +443,8 @@ def bench_c3(args) -> int:
                          f"iteration of the first 64 keyframes of the same problem ({n_sub} measurements, "
                          f"{per_iter:.2f} s), scaled to the full {p.n_obs} measurements"}
         log(f"cpu baseline: {rate:.4g} LM it/s (sample {per_iter:.2f} s per iteration)")
+        ref_cpu = reference_cpu_path_rate()
+        log(f"reference CPU path at 10 keyframes: {ref_cpu['value']:.3g} LM it/s")
 
     line = {
         "metric": "LM iters/s & batched solves/s vs plain FP64 GPU solver at matched final cost",
@@ -429,6 +468,7 @@ def bench_c3(args) -> int:
         "roofline": roof,
         "roofline_assemble": roof_asm,
         "cpu_baseline": cpu,
+        "reference_cpu_path": ref_cpu,
     }
     print(json.dumps(line))
     return 0
