@@ -86,7 +86,7 @@ typedef struct spgo_options {
     double lambda0;          /* initial LM damping                             */
     double lam_up, lam_down, lam_min, lam_max;
     double alpha_min, alpha_max;  /* inverse-depth clamp (LMutils.h:54-55)     */
-    int32_t refine_steps;    /* iterative-refinement steps (MIXED)             */
+    int32_t reserved0;       /* unused, zero                                    */
     int32_t use_graph;       /* replay the LM iterations from captured graphs  */
     int32_t chunk_landmarks; /* tile shape: landmarks per LDS chunk (<= 64)    */
     int32_t group_chunks;    /* tile shape: chunks per workgroup               */
@@ -198,6 +198,17 @@ int spgo_linearize(spgo_solver* h, double lambda, double* H, double* g, double* 
  * Blocks farther from the diagonal (loop revisits) are left out.  These are
  * the systems spgo_lpcg_load_systems takes.  No reference counterpart. */
 int spgo_reduced_band(spgo_solver* h, double lambda, int32_t band_blocks, double* Hband, double* g, double* cost);
+
+/* spgo_reduced_band without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * graph capture, serving loops with a fixed memory budget).  The band is staged
+ * in a per-device scratch array of 2^20 doubles (8 MiB, held until the process
+ * ends): N (band_blocks + 1) 36 must not exceed that (SPGO_E_ARG otherwise).
+ * The solver must already hold its device state -- stepped or linearised once
+ * since its measurements or options last changed (SPGO_E_STATE otherwise).
+ * Calls that overlap on one GPU share the scratch array and race. */
+int spgo_reduced_band_noalloc(spgo_solver* h, double lambda, int32_t band_blocks, double* Hband, double* g,
+                              double* cost);
 
 /* Host-side layout summary of the measurements written so far (builds the
  * layout without a GPU). */

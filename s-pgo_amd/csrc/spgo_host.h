@@ -159,6 +159,8 @@ int build_layout(spgo_solver* h, HostLayout& out);
 int build_layout(spgo_solver* h, int chunk_landmarks, int group_chunks, HostLayout& out);
 // the layout in frame coordinates (spgo_frames.hip)
 int build_layout_frames(spgo_solver* h, HostLayout& out);
+// layout, state upload and `iterations` LM iterations (0: linearise only)
+int run_step(spgo_solver* h, int iterations);
 // the layout in the coordinates the solver's options ask for
 int build_any_layout(spgo_solver* h, HostLayout& out);
 // global left poses G (N + 1 of them, 3x4) of the incremental poses X (N x 12)

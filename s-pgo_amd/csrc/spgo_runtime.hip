@@ -859,7 +859,6 @@ int spgo_default_options(spgo_options* o) {
     o->lam_max = 1e10;
     o->alpha_min = 1.0 / 30.0;
     o->alpha_max = 1.0 / 0.2;
-    o->refine_steps = 3;
     o->use_graph = 1;
     o->chunk_landmarks = CHUNK_LM;
     o->group_chunks = 1;

@@ -40,7 +40,7 @@ class Options(ctypes.Structure):
                 ("robust_param", c_double), ("eps", c_double), ("lambda0", c_double),
                 ("lam_up", c_double), ("lam_down", c_double), ("lam_min", c_double), ("lam_max", c_double),
                 ("alpha_min", c_double), ("alpha_max", c_double),
-                ("refine_steps", c_int32), ("use_graph", c_int32),
+                ("reserved0", c_int32), ("use_graph", c_int32),
                 ("chunk_landmarks", c_int32), ("group_chunks", c_int32),
                 ("factorization", c_int32), ("pcg_max_iter", c_int32), ("coordinates", c_int32),
                 ("reserved", c_int32 * 5),
@@ -97,6 +97,7 @@ SIGNATURES = {
     "spgo_linearize": (c_int32, [c_void_p, c_double, F64P, F64P, F64P]),
     "spgo_reduced_band": (c_int32, [c_void_p, c_double, c_int32, F64P, F64P, POINTER(c_double)]),
     "spgo_device_synchronize": (c_int32, []),
+    "spgo_reduced_band_noalloc": (c_int32, [c_void_p, c_double, c_int32, F64P, F64P, POINTER(c_double)]),
     "spgo_problem_info": (c_int32, [c_void_p, POINTER(LayoutInfo)]),
     "spgo_last_step_ms": (c_int32, [c_void_p, F64P]),
     "spgo_profile_phases": (c_int32, [c_void_p, c_int32, F64P]),

@@ -215,6 +215,16 @@ class Solver:
                                                 g.ctypes.data_as(F64P), ctypes.byref(c)))
         return Hb, g, c.value
 
+    def reduced_band_noalloc(self, W: int, lam: float = 0.0):
+        """``reduced_band`` without device allocation (the band is staged in the
+        library's scratch array); needs a solver that has stepped once."""
+        Hb = np.empty((self.N, W + 1, 6, 6), np.float64)
+        g = np.empty((self.N, 6), np.float64)
+        c = ctypes.c_double()
+        self._check(self._lib.spgo_reduced_band_noalloc(self._h, float(lam), int(W), Hb.ctypes.data_as(F64P),
+                                                        g.ctypes.data_as(F64P), ctypes.byref(c)))
+        return Hb, g, c.value
+
     def layout_info(self) -> dict:
         info = LayoutInfo()
         self._check(self._lib.spgo_problem_info(self._h, ctypes.byref(info)))
