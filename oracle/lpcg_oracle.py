@@ -14,7 +14,8 @@ build's own design in float64 numpy:
   (C_i H_ii C_i^T = I); normalised off-diagonal blocks N_{i,d} = C_i H_{i,i-d}
   C_{i-d}^T with norms s_{i,d}; node input x_i = [N_{i,1} (36), s_{i,1},
   s_{i+1,1}, sum_j s_ij / W, 1];
-* network: h0 = relu(W_in x + b_in); three message-passing layers
+* network (products with operands rounded to bfloat16 when ``bf`` is set, as
+  the GPU's MFMA layers compute them): h0 = relu(W_in x + b_in); three message-passing layers
   h_{l+1} = relu(W_s h_l + W_m m_l + b_l) with m_i = sum_j s_ij h_j /
   (1 + sum_j s_ij) over the band neighbours j of i; head out = W_o h_3 +
   W_k x + b_o (57 values);
@@ -173,18 +174,32 @@ def aggregate_T(dm: np.ndarray, s: np.ndarray, wsum: np.ndarray) -> np.ndarray:
     return dh
 
 
-def network(P: Params, x: np.ndarray, s: np.ndarray, wsum: np.ndarray):
-    """Forward pass; returns out (n, NOUT) and the cache for the backward pass."""
-    z0 = x @ P.W_in.T + P.b_in
+def bf16(a) -> np.ndarray:
+    """a rounded to bfloat16 (through float32, round to nearest even), as
+    float64: the operand precision of the network's MFMA products on the GPU."""
+    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return (u & 0xFFFFFFFF).astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _ident(a):
+    return a
+
+
+def network(P: Params, x: np.ndarray, s: np.ndarray, wsum: np.ndarray, bf: bool = False):
+    """Forward pass; returns out (n, NOUT) and the cache for the backward pass.
+    bf: the operands of every product rounded to bfloat16, as on the GPU."""
+    r = bf16 if bf else _ident
+    z0 = r(x) @ r(P.W_in).T + P.b_in
     h = [np.maximum(z0, 0.0)]
     zs, ms = [z0], []
     for l in range(NLAYER):
         m = aggregate(h[-1], s, wsum)
-        z = h[-1] @ P.W_s[l].T + m @ P.W_m[l].T + P.b[l]
+        z = r(h[-1]) @ r(P.W_s[l]).T + r(m) @ r(P.W_m[l]).T + P.b[l]
         ms.append(m)
         zs.append(z)
         h.append(np.maximum(z, 0.0))
-    out = h[-1] @ P.W_o.T + x @ P.W_k.T + P.b_o
+    out = r(h[-1]) @ r(P.W_o).T + r(x) @ r(P.W_k).T + P.b_o
     return out, (h, zs, ms)
 
 
@@ -241,10 +256,10 @@ def make_system(H: np.ndarray, g: np.ndarray, W: int) -> System:
     return System(Hb, g.reshape(-1, 6).copy(), x)
 
 
-def system_loss(P: Params, Hb: np.ndarray):
+def system_loss(P: Params, Hb: np.ndarray, bf: bool = False):
     """Kaporin loss of one system and what its gradient needs."""
     C, s, N1, x, wsum = features(Hb)
-    out, cache = network(P, x, s, wsum)
+    out, cache = network(P, x, s, wsum, bf)
     _, _, A, B = g_blocks(out, C)
     B = B.copy()
     B[0] = 0.0
@@ -262,8 +277,11 @@ def forward_loss(P: Params, systems) -> float:
     return float(np.mean([system_loss(P, getattr(sy, "Hb", sy))[0] for sy in systems]))
 
 
-def loss_and_grad(P: Params, systems):
-    """Mean loss over the systems and its gradient (flat, like Params.flat())."""
+def loss_and_grad(P: Params, systems, bf: bool = False):
+    """Mean loss over the systems and its gradient (flat, like Params.flat()).
+    bf: every product's operands rounded to bfloat16 (forward and backward),
+    as the GPU's MFMA layers compute them."""
+    rb = bf16 if bf else _ident
     S = len(systems)
     gW_in = np.zeros_like(P.W_in)
     gb_in = np.zeros_like(P.b_in)
@@ -275,7 +293,7 @@ def loss_and_grad(P: Params, systems):
     gb_o = np.zeros_like(P.b_o)
     total = 0.0
     for sy in systems:
-        L, (x, s, wsum, (h, zs, ms), A, B, N1, T, n6) = system_loss(P, getattr(sy, "Hb", sy))
+        L, (x, s, wsum, (h, zs, ms), A, B, N1, T, n6) = system_loss(P, getattr(sy, "Hb", sy), bf)
         total += L
         n = x.shape[0]
         # dt/dA = 2 A + 2 B N^T, dt/dB = 2 A N + 2 B; the log-det term on diag(A)
@@ -289,19 +307,19 @@ def loss_and_grad(P: Params, systems):
             dout[:, k] = dA[:, r, c]
         dout[:, 21:57] = dB.reshape(n, 36)
         dout /= S
-        gW_o += dout.T @ h[-1]
-        gW_k += dout.T @ x
+        gW_o += rb(dout).T @ rb(h[-1])
+        gW_k += rb(dout).T @ rb(x)
         gb_o += dout.sum(0)
-        dh = dout @ P.W_o
+        dh = rb(dout) @ rb(P.W_o)
         for l in reversed(range(NLAYER)):
             dz = dh * (zs[l + 1] > 0)
-            gW_s[l] += dz.T @ h[l]
-            gW_m[l] += dz.T @ ms[l]
+            gW_s[l] += rb(dz).T @ rb(h[l])
+            gW_m[l] += rb(dz).T @ rb(ms[l])
             gb[l] += dz.sum(0)
-            dm = dz @ P.W_m[l]
-            dh = dz @ P.W_s[l] + aggregate_T(dm, s, wsum)
+            dm = rb(dz) @ rb(P.W_m[l])
+            dh = rb(dz) @ rb(P.W_s[l]) + aggregate_T(dm, s, wsum)
         dz0 = dh * (zs[0] > 0)
-        gW_in += dz0.T @ x
+        gW_in += rb(dz0).T @ rb(x)
         gb_in += dz0.sum(0)
     grad = Params(gW_in, gb_in, gW_s, gW_m, gb, gW_o, gW_k, gb_o).flat()
     return total / S, grad

@@ -142,61 +142,143 @@ __global__ void k_lp_feat2(int T, int W, const double* s, const int* node_sys, c
 }
 
 // ---- network --------------------------------------------------------------
-// z[n][o] = b[o] + W1[o][:] in1[n][:] + W2[o][:] in2[n][:] for o < O (O <= 64);
-// h = relu(z) when h != nullptr.  A block stages W1 and W2 in LDS (padded rows)
-// and handles LP_LIN_NODES nodes, a thread one output of four nodes at a time,
-// each dot product in four independent partial sums (short dependency chains).
-constexpr int LP_LIN_NODES = 8;
-__global__ __launch_bounds__(256) void k_lp_linear(int T, int O, int I1, const float* in1, const float* W1, int I2,
-                                                   const float* in2, const float* W2, const float* b, float* z,
-                                                   float* h) {
-    extern __shared__ float wl[];   // O x (I1 + 1), then O x (I2 + 1)
-    float* w2l = wl + O * (I1 + 1);
-    const int tid = threadIdx.x;
-    for (int k = tid; k < O * I1; k += 256) wl[(k / I1) * (I1 + 1) + k % I1] = W1[k];
-    for (int k = tid; k < O * I2; k += 256) w2l[(k / I2) * (I2 + 1) + k % I2] = W2[k];
+// The dense layers are products over the nodes on MFMA
+// (v_mfma_f32_16x16x16_bf16): activations and weights enter as bfloat16
+// (rounded to nearest even), products accumulate in float32, the master
+// weights, biases and stored activations stay float32.
+typedef short __attribute__((ext_vector_type(4))) bf16x4;
+typedef float __attribute__((ext_vector_type(4))) f32x4;
+
+__device__ __forceinline__ short to_bf16(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+constexpr int MM_NODES = 64;           // nodes per workgroup of k_lp_mm (16 per wavefront)
+constexpr int MM_KP = 128 + 8;         // LDS row stride (bf16) of the staged operands
+constexpr int WG_NODES = 256;          // nodes per workgroup of k_lp_wgrad
+
+// Z[n][c] = bias[c] + sum_k A[n][k] B[c][k], A = [in1 | in2] (T x (I1 + I2)),
+// B = [W1 | W2] (C x (I1 + I2)), or with trans the transpose of W1 (I1 x C:
+// B[c][k] = W1[k][c], no second input).  C <= 64, I1 + I2 <= 128.  With hrelu
+// also max(Z, 0); with accumulate Z is added to.  Wavefront w computes nodes
+// 16w .. 16w + 15 of the workgroup's 64 against all 64 columns (four 16x16 tiles).
+__global__ __launch_bounds__(256) void k_lp_mm(int T, int C, int I1, const float* in1, const float* W1, int I2,
+                                               const float* in2, const float* W2, int trans, const float* bias,
+                                               float* z, float* hrelu, int accumulate) {
+    __shared__ short As[MM_NODES * MM_KP];
+    __shared__ short Bs[64 * MM_KP];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int n0 = blockIdx.x * MM_NODES;
+    const int K = I1 + I2, KP = (K + 15) & ~15;
+    for (int e = tid; e < 64 * KP; e += 256) {
+        const int c = e / KP, k = e - c * KP;
+        float v = 0.f;
+        if (c < C && k < K) v = trans ? W1[(long long)k * C + c] : (k < I1 ? W1[c * I1 + k] : W2[c * I2 + (k - I1)]);
+        Bs[c * MM_KP + k] = to_bf16(v);
+    }
+    for (int e = tid; e < MM_NODES * KP; e += 256) {
+        const int r = e / KP, k = e - r * KP, n = n0 + r;
+        float v = 0.f;
+        if (n < T && k < K) v = k < I1 ? in1[(long long)n * I1 + k] : in2[(long long)n * I2 + (k - I1)];
+        As[r * MM_KP + k] = to_bf16(v);
+    }
     __syncthreads();
-    const int o = tid & 63, q = tid >> 6;
-    if (o >= O) return;
-    const int n0 = blockIdx.x * LP_LIN_NODES;
-    for (int nn = q; nn < LP_LIN_NODES; nn += 4) {
-        const int n = n0 + nn;
-        if (n >= T) break;
-        float acc[4] = {0.f, 0.f, 0.f, 0.f};
-        const float* x1 = in1 + (long long)n * I1;
-        for (int k = 0; k < I1; ++k) acc[k & 3] += wl[o * (I1 + 1) + k] * x1[k];
-        if (I2 > 0) {
-            const float* x2 = in2 + (long long)n * I2;
-            for (int k = 0; k < I2; ++k) acc[k & 3] += w2l[o * (I2 + 1) + k] * x2[k];
+    f32x4 acc[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) acc[ct] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const short* ar = As + (16 * wv + (lane & 15)) * MM_KP + 4 * (lane >> 4);
+    for (int kk = 0; kk < KP; kk += 16) {
+        const bf16x4 a = *(const bf16x4*)(ar + kk);
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            const bf16x4 bv = *(const bf16x4*)(Bs + (16 * ct + (lane & 15)) * MM_KP + 4 * (lane >> 4) + kk);
+            acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[ct], 0, 0, 0);
         }
-        const float a = b[o] + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
-        z[(long long)n * O + o] = a;
-        if (h) h[(long long)n * O + o] = a > 0.f ? a : 0.f;
+    }
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        const int c = 16 * ct + (lane & 15);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int n = n0 + 16 * wv + 4 * (lane >> 4) + v;
+            if (n < T && c < C) {
+                const long long o = (long long)n * C + c;
+                float val = acc[ct][v] + (bias ? bias[c] : 0.f);
+                if (accumulate) val += z[o];
+                z[o] = val;
+                if (hrelu) hrelu[o] = val > 0.f ? val : 0.f;
+            }
+        }
     }
 }
 
-// dx[n][k] (=|+=) sum_o W[o][k] dy[n][o], k < I (I <= 64); W is O x I
-__global__ __launch_bounds__(256) void k_lp_linearT(int T, int O, int I, const float* dy, const float* Wt, float* dx,
-                                                    int accumulate) {
-    extern __shared__ float wl[];   // I x (O + 1): transposed
-    const int tid = threadIdx.x;
-    for (int k = tid; k < O * I; k += 256) {
-        const int o = k / I, c = k % I;
-        wl[c * (O + 1) + o] = Wt[k];
+// Partial weight gradients over WG_NODES nodes: part[blk][o I + i] =
+// sum_n dy[n][o] x[n][i] on MFMA (the node index is the contraction), and
+// part[blk][O I + o] = sum_n dy[n][o].  O, I <= 64.
+__global__ __launch_bounds__(256) void k_lp_wgrad(int T, int O, int I, const float* dy, const float* x, float* part) {
+    __shared__ short Ys[64 * (64 + 8)];   // [o][node]
+    __shared__ short Xs[64 * (64 + 8)];   // [i][node]
+    constexpr int S = 64 + 8;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nb = blockIdx.x * WG_NODES;
+    f32x4 acc[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) acc[ct] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;
+    for (int sub = 0; sub < WG_NODES; sub += 64) {
+        __syncthreads();
+        for (int e = tid; e < 64 * 64; e += 256) {
+            const int k = e >> 6, c = e & 63, n = nb + sub + k;
+            const bool in = n < T;
+            Ys[c * S + k] = to_bf16(in && c < O ? dy[(long long)n * O + c] : 0.f);
+            Xs[c * S + k] = to_bf16(in && c < I ? x[(long long)n * I + c] : 0.f);
+        }
+        if (tid < O)
+            for (int k = 0; k < 64; ++k) {
+                const int n = nb + sub + k;
+                if (n < T) bsum += dy[(long long)n * O + tid];
+            }
+        __syncthreads();
+        const short* ar = Ys + (16 * wv + (lane & 15)) * S + 4 * (lane >> 4);
+#pragma unroll
+        for (int kk = 0; kk < 64; kk += 16) {
+            const bf16x4 a = *(const bf16x4*)(ar + kk);
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const bf16x4 bv = *(const bf16x4*)(Xs + (16 * ct + (lane & 15)) * S + 4 * (lane >> 4) + kk);
+                acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[ct], 0, 0, 0);
+            }
+        }
     }
-    __syncthreads();
-    const int c = tid & 63, q = tid >> 6;
-    if (c >= I) return;
-    const int n0 = blockIdx.x * LP_LIN_NODES;
-    for (int nn = q; nn < LP_LIN_NODES; nn += 4) {
-        const int n = n0 + nn;
-        if (n >= T) break;
-        float a = 0.f;
-        const float* g = dy + (long long)n * O;
-        for (int o = 0; o < O; ++o) a += wl[c * (O + 1) + o] * g[o];
-        if (accumulate) dx[(long long)n * I + c] += a;
-        else dx[(long long)n * I + c] = a;
+    float* out = part + (long long)blockIdx.x * (O * I + O);
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        const int i = 16 * ct + (lane & 15);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int o = 16 * wv + 4 * (lane >> 4) + v;
+            if (o < O && i < I) out[o * I + i] = acc[ct][v];
+        }
     }
+    if (tid < O) out[O * I + tid] = bsum;
+}
+
+// out[t] = sum over blocks of part[blk][t], blocks in order (deterministic)
+__global__ void k_lp_reduce_parts(int nblk, int n, const float* part, float* gW, float* gb, int nW) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float a0 = 0.f, a1 = 0.f;
+    int b = 0;
+    for (; b + 1 < nblk; b += 2) {
+        a0 += part[(long long)b * n + t];
+        a1 += part[(long long)(b + 1) * n + t];
+    }
+    if (b < nblk) a0 += part[(long long)b * n + t];
+    const float s = a0 + a1;
+    if (t < nW) gW[t] = s;
+    else if (gb) gb[t - nW] = s;
 }
 
 // m_i = sum_j s_ij h_j / wsum_i over the band neighbours j of i (same system)
@@ -241,27 +323,6 @@ __global__ void k_lp_aggT(int T, int W, const double* s, const double* wsum, con
 __global__ void k_lp_reluback(int n, const float* dh, const float* z, float* dz) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) dz[t] = z[t] > 0.f ? dh[t] : 0.f;
-}
-
-// gW[o][k] = sum_n dy[n][o] x[n][k]; gb[o] = sum_n dy[n][o] (thread per weight)
-__global__ void k_lp_wgrad(int T, int O, int I, const float* dy, const float* x, float* gW, float* gb) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < O * I) {
-        const int o = t / I, k = t - o * I;
-        float a0 = 0.f, a1 = 0.f;
-        int n = 0;
-        for (; n + 1 < T; n += 2) {
-            a0 += dy[(long long)n * O + o] * x[(long long)n * I + k];
-            a1 += dy[(long long)(n + 1) * O + o] * x[(long long)(n + 1) * I + k];
-        }
-        if (n < T) a0 += dy[(long long)n * O + o] * x[(long long)n * I + k];
-        gW[t] = a0 + a1;
-    } else if (gb && t < O * I + O) {
-        const int o = t - O * I;
-        float a = 0.f;
-        for (int n = 0; n < T; ++n) a += dy[(long long)n * O + o];
-        gb[o] = a;
-    }
 }
 
 // ---- preconditioner blocks and the loss -------------------------------------
@@ -799,14 +860,25 @@ void lp_launch_features(const LpSet& s, double* C, double* sfeat, float* X, doub
 
 void lp_launch_linear(int T, int O, int I1, const float* in1, const float* W1, int I2, const float* in2,
                       const float* W2, const float* b, float* z, float* h, hipStream_t st) {
-    const size_t lds = sizeof(float) * ((size_t)O * (I1 + 1) + (size_t)O * (I2 + 1));
-    hipLaunchKernelGGL(k_lp_linear, dim3(nblk(T, LP_LIN_NODES)), dim3(256), lds, st, T, O, I1, in1, W1, I2, in2, W2,
-                       b, z, h);
+    hipLaunchKernelGGL(k_lp_mm, dim3(nblk(T, MM_NODES)), dim3(256), 0, st, T, O, I1, in1, W1, I2, in2, W2, 0, b, z,
+                       h, 0);
 }
 
 void lp_launch_linearT(int T, int O, int I, const float* dy, const float* Wt, float* dx, int acc, hipStream_t st) {
-    const size_t lds = sizeof(float) * (size_t)I * (O + 1);
-    hipLaunchKernelGGL(k_lp_linearT, dim3(nblk(T, LP_LIN_NODES)), dim3(256), lds, st, T, O, I, dy, Wt, dx, acc);
+    // dx (T x I) = dy (T x O) W (O x I): B[c][k] = W[k][c]
+    hipLaunchKernelGGL(k_lp_mm, dim3(nblk(T, MM_NODES)), dim3(256), 0, st, T, I, O, dy, Wt, 0,
+                       (const float*)nullptr, (const float*)nullptr, 1, (const float*)nullptr, dx, (float*)nullptr,
+                       acc);
+}
+
+int lp_wgrad_blocks(int T) { return (T + WG_NODES - 1) / WG_NODES; }
+
+void lp_launch_wgrad(int T, int O, int I, const float* dy, const float* x, float* gW, float* gb, float* part,
+                     hipStream_t st) {
+    const int nb = lp_wgrad_blocks(T);
+    hipLaunchKernelGGL(k_lp_wgrad, dim3(nb), dim3(256), 0, st, T, O, I, dy, x, part);
+    const int n = O * I + O;
+    hipLaunchKernelGGL(k_lp_reduce_parts, dim3(nblk(n)), dim3(256), 0, st, nb, n, part, gW, gb, O * I);
 }
 
 void lp_launch_agg(const LpSet& s, const double* sfeat, const double* wsum, const float* hin, float* m, int transpose,
@@ -821,10 +893,6 @@ void lp_launch_agg(const LpSet& s, const double* sfeat, const double* wsum, cons
 
 void lp_launch_reluback(int n, const float* dh, const float* z, float* dz, hipStream_t st) {
     hipLaunchKernelGGL(k_lp_reluback, dim3(nblk(n)), dim3(256), 0, st, n, dh, z, dz);
-}
-
-void lp_launch_wgrad(int T, int O, int I, const float* dy, const float* x, float* gW, float* gb, hipStream_t st) {
-    hipLaunchKernelGGL(k_lp_wgrad, dim3(nblk((long long)O * I + O)), dim3(256), 0, st, T, O, I, dy, x, gW, gb);
 }
 
 void lp_launch_buildG(const LpSet& s, const float* out, const double* C, double* Gd, double* Go, hipStream_t st) {

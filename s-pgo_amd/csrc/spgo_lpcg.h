@@ -45,7 +45,11 @@ void lp_launch_linearT(int T, int O, int I, const float* dy, const float* Wt, fl
 void lp_launch_agg(const LpSet& s, const double* sfeat, const double* wsum, const float* hin, float* m, int transpose,
                    hipStream_t st);
 void lp_launch_reluback(int n, const float* dh, const float* z, float* dz, hipStream_t st);
-void lp_launch_wgrad(int T, int O, int I, const float* dy, const float* x, float* gW, float* gb, hipStream_t st);
+// gW = dy^T x (O x I), gb = column sums of dy (optional); part holds
+// lp_wgrad_blocks(T) x (O I + O) floats of partial sums
+void lp_launch_wgrad(int T, int O, int I, const float* dy, const float* x, float* gW, float* gb, float* part,
+                     hipStream_t st);
+int lp_wgrad_blocks(int T);
 void lp_launch_buildG(const LpSet& s, const float* out, const double* C, double* Gd, double* Go, hipStream_t st);
 // Kaporin loss: per-node terms, per-system T and loss; gradient wrt the head output
 void lp_launch_kap(const LpSet& s, const float* out, const double* N1, double* tnode, double* ldnode, double* Tsys,

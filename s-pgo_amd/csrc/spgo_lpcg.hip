@@ -61,6 +61,7 @@ struct spgo_lpcg {
     int *node_sys = nullptr, *sys_begin = nullptr, *bad = nullptr;
     float *X = nullptr, *Z[LP_NLAYER + 1] = {}, *Hh[LP_NLAYER + 1] = {}, *M[LP_NLAYER] = {};
     float *OUT = nullptr, *dOUT = nullptr, *dH = nullptr, *dZ = nullptr, *dM = nullptr;
+    float* part = nullptr;       // partial weight gradients (lp_wgrad_blocks(T) x (64 64 + 64))
     double *px = nullptr, *pr = nullptr, *pz = nullptr, *pp = nullptr, *pq = nullptr, *ptmp = nullptr;
     int* piters = nullptr;
     double* prel = nullptr;
@@ -142,20 +143,20 @@ void backward(spgo_lpcg* h) {
     hipStream_t st = h->stream;
     float* G = h->gP;
     lp_launch_kap_dout(h->set, h->OUT, h->N1, h->Tsys, h->dOUT, st);
-    lp_launch_wgrad(T, LP_NOUT, LP_HID, h->dOUT, h->Hh[LP_NLAYER], G + LP_OFF_WO, G + LP_OFF_BO, st);
-    lp_launch_wgrad(T, LP_NOUT, LP_NIN, h->dOUT, h->X, G + LP_OFF_WK, nullptr, st);
+    lp_launch_wgrad(T, LP_NOUT, LP_HID, h->dOUT, h->Hh[LP_NLAYER], G + LP_OFF_WO, G + LP_OFF_BO, h->part, st);
+    lp_launch_wgrad(T, LP_NOUT, LP_NIN, h->dOUT, h->X, G + LP_OFF_WK, nullptr, h->part, st);
     lp_launch_linearT(T, LP_NOUT, LP_HID, h->dOUT, h->P + LP_OFF_WO, h->dH, 0, st);
     for (int l = LP_NLAYER - 1; l >= 0; --l) {
         float* gl = G + LP_OFF_L0 + l * LP_LAYER_SIZE;
         lp_launch_reluback(T * LP_HID, h->dH, h->Z[l + 1], h->dZ, st);
-        lp_launch_wgrad(T, LP_HID, LP_HID, h->dZ, h->Hh[l], gl, gl + 2 * LP_HID * LP_HID, st);
-        lp_launch_wgrad(T, LP_HID, LP_HID, h->dZ, h->M[l], gl + LP_HID * LP_HID, nullptr, st);
+        lp_launch_wgrad(T, LP_HID, LP_HID, h->dZ, h->Hh[l], gl, gl + 2 * LP_HID * LP_HID, h->part, st);
+        lp_launch_wgrad(T, LP_HID, LP_HID, h->dZ, h->M[l], gl + LP_HID * LP_HID, nullptr, h->part, st);
         lp_launch_linearT(T, LP_HID, LP_HID, h->dZ, layer_wm(h->P, l), h->dM, 0, st);
         lp_launch_linearT(T, LP_HID, LP_HID, h->dZ, layer_ws(h->P, l), h->dH, 0, st);
         lp_launch_agg(h->set, h->sfeat, h->wsum, h->dM, h->dH, 1, st);
     }
     lp_launch_reluback(T * LP_HID, h->dH, h->Z[0], h->dZ, st);
-    lp_launch_wgrad(T, LP_HID, LP_NIN, h->dZ, h->X, G + LP_OFF_WIN, G + LP_OFF_BIN, st);
+    lp_launch_wgrad(T, LP_HID, LP_NIN, h->dZ, h->X, G + LP_OFF_WIN, G + LP_OFF_BIN, h->part, st);
 }
 
 int mean_loss(spgo_lpcg* h, double* loss) {
@@ -389,6 +390,7 @@ int spgo_lpcg_load_systems(spgo_lpcg* h, int32_t n_systems, const int32_t* n_nod
     h->dH = m.alloc<float>((size_t)T * LP_HID);
     h->dZ = m.alloc<float>((size_t)T * LP_HID);
     h->dM = m.alloc<float>((size_t)T * LP_HID);
+    h->part = m.alloc<float>((size_t)lp_wgrad_blocks(T) * (LP_HID * LP_HID + LP_HID));
     h->px = m.alloc<double>((size_t)T * 6);
     h->pr = m.alloc<double>((size_t)T * 6);
     h->pz = m.alloc<double>((size_t)T * 6);
@@ -397,7 +399,7 @@ int spgo_lpcg_load_systems(spgo_lpcg* h, int32_t n_systems, const int32_t* n_nod
     h->ptmp = m.alloc<double>((size_t)T * 6);
     h->piters = m.alloc<int>(n_systems);
     h->prel = m.alloc<double>(n_systems);
-    if (!h->prel || !h->piters || !h->ptmp || !h->pq || !h->pp || !h->pz || !h->pr || !h->px || !h->dM || !h->dZ ||
+    if (!h->prel || !h->piters || !h->ptmp || !h->pq || !h->pp || !h->pz || !h->pr || !h->px || !h->part || !h->dM || !h->dZ ||
         !h->dH || !h->dOUT || !h->OUT || !h->X || !h->bad || !h->sys_begin || !h->node_sys || !h->lsys || !h->Tsys ||
         !h->ldnode || !h->tnode || !h->Go || !h->Gd || !h->wsum || !h->sfeat || !h->N1 || !h->C || !h->g || !h->Hb)
         return h->fail(SPGO_E_DEVICE, "device allocation failed");

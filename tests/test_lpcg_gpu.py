@@ -1,5 +1,6 @@
 """Learned-preconditioner PCG on the GPU, through the C ABI: features, loss and
-gradient against the CPU oracle (FP32 network, FP64 loss), PCG against dense
+gradient against the CPU oracle (bfloat16 MFMA operands and float32
+accumulation in the network, FP64 loss), PCG against dense
 solves, and what training gains over block Jacobi on held-out systems that
 the solver itself generated."""
 import numpy as np
@@ -33,7 +34,8 @@ def test_features_loss_and_gradient_match_the_oracle():
     assert np.allclose(C, Cc, rtol=1e-10, atol=1e-12 * np.abs(Cc).max())
     assert np.allclose(X, Xc, rtol=1e-5, atol=1e-6)
     Lg, gg = lp.loss_grad()
-    Lc, gc = LP.loss_and_grad(LP.Params.from_flat(p.astype(np.float64)), Hbs)
+    # the GPU's layers take bfloat16 operands on MFMA: the oracle rounds the same operands
+    Lc, gc = LP.loss_and_grad(LP.Params.from_flat(p.astype(np.float64)), Hbs, bf=True)
     print(f"[lpcg-grad] loss gpu {Lg:.9g} cpu {Lc:.9g}; |grad| {np.linalg.norm(gc):.4g} "
           f"err {np.linalg.norm(gg - gc):.3g}", flush=True)
     assert abs(Lg - Lc) <= 1e-6 + 1e-4 * abs(Lc)

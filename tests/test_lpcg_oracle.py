@@ -146,3 +146,38 @@ def test_fsai_like_head_beats_block_jacobi():
     _, it_j, _ = LP.pcg(sy.Hb, sy.g, C=C)
     print(f"[fsai-head] block Jacobi {it_j} iterations, first-order factored inverse {it_l}")
     assert it_l < it_j
+
+
+def test_bfloat16_rounding_known_answers():
+    # round to nearest even on the 8-bit significand of bfloat16
+    one = 1.0
+    assert LP.bf16(one) == 1.0
+    assert LP.bf16(1.0 + 2.0 ** -8) == 1.0                    # tie -> even (1.0)
+    assert LP.bf16(1.0 + 3 * 2.0 ** -8) == 1.0 + 2.0 ** -6     # tie -> even (1 + 2^-6)
+    assert LP.bf16(1.0 + 2.0 ** -8 + 2.0 ** -12) == 1.0 + 2.0 ** -7
+    assert LP.bf16(-3.0e-3) == -float(np.float32(LP.bf16(3.0e-3)))
+    x = np.random.default_rng(0).normal(size=1000)
+    assert np.max(np.abs(LP.bf16(x) - x) / np.abs(x)) <= 2.0 ** -8
+
+
+def test_bfloat16_network_stays_close_to_the_exact_one():
+    rng = np.random.default_rng(4)
+    P = LP.init_params(3)
+    P.W_o = rng.normal(size=P.W_o.shape) * 0.05
+    n = 30
+    Hb = np.zeros((n, 3, 6, 6))
+    for i in range(n):
+        M = rng.normal(size=(6, 6))
+        Hb[i, 0] = M @ M.T + 6 * np.eye(6)
+        for d in (1, 2):
+            if i - d >= 0:
+                Hb[i, d] = 0.3 * rng.normal(size=(6, 6))
+    C, s, N1, x, wsum = LP.features(Hb)
+    out, _ = LP.network(P, x, s, wsum)
+    outb, _ = LP.network(P, x, s, wsum, bf=True)
+    rel = np.linalg.norm(outb - out) / np.linalg.norm(out)
+    assert 1e-5 < rel < 2e-2
+    L, g = LP.loss_and_grad(P, [Hb])
+    Lb, gb = LP.loss_and_grad(P, [Hb], bf=True)
+    assert abs(Lb - L) < 1e-2 * abs(L) + 1e-3
+    assert np.linalg.norm(gb - g) < 3e-2 * np.linalg.norm(g)
