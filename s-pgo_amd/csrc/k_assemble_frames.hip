@@ -13,17 +13,19 @@
 // reduces in the anchor frame as the incremental kernel does:
 //   * pair sums S(s) = sum J_a^T J_a, s(s) = sum J_a^T r per target slot s,
 //   * per landmark C, g_alpha and u_l(s) = sum J_a^T j_alpha per slot,
-// then maps them to world axes with Ad(G_a^-1) and stores the target-frame
-// block and gradient
-//   B = diag(S_w) - sum_l u_l u_l^T / (C_l + eps),  b = s_w - sum_l u_l g_l / (C_l + eps)
-// (6m x 6m and 6m; the second terms one MFMA product of the rows [u_1; ..; u_m; g]
-// scaled by 1 / sqrt(C + eps)).  With P = [1 | -I] the group's part of the
-// system is P^T B P and its gradient P^T b: the anchor's blocks are sums of the
-// stored entries, R(s) = sum_s' B(s, s') and sum_s R(s), formed here in double
-// from the values as stored, so the anchor rows cancel the target rows exactly
-// in the system whatever the storage precision.  k_form_frames sums them.
-// The back-substitution of the next linearisation needs only
-// z_a - z_{f_s}: delta_alpha = (g_alpha - sum_s u_w(s) . (z_a - z_{f_s})) / (C + eps).
+// then forms, still in the anchor's axes, the target-frame block and gradient
+//   B_a = diag(S) - sum_l u_l u_l^T / (C_l + eps),  b_a = s - sum_l u_l g_l / (C_l + eps)
+// (6m x 6m and 6m; the second terms one MFMA product of the rows
+// [u_1; ..; u_m; g] scaled by 1 / sqrt(C + eps)), and maps them to world axes
+// in double: B = Ad(G_a^-1)^T B_a Ad(G_a^-1), b = Ad(G_a^-1)^T b_a.  The world
+// axes carry lever arms of the trajectory's size, so the mapped blocks are kept
+// in double whatever the precision of the reduction.  With P = [1 | -I] the
+// group's part of the system is P^T B P and its gradient P^T b: the anchor's
+// blocks are the row sums R(s) = sum_s' B(s, s'), their total and the gradient
+// total, formed here in double from the stored blocks, so the anchor rows
+// cancel the target rows exactly in the system.  k_form_frames sums them.
+// The back-substitution of the next linearisation needs only z_a - z_{f_s}:
+// delta_alpha = (g_alpha - sum_s u_l(s) . Ad(G_a^-1) (z_a - z_{f_s})) / (C + eps).
 #include "spgo_asm_common.h"
 #include "spgo_device.h"
 #include "spgo_kernels.h"
@@ -41,16 +43,6 @@ struct FrShape {
     static constexpr int VROWS = 16 * ((K6 + 1 + 15) / 16);            // and the gradient row, whole tiles
     static constexpr int US = 6 * KT + 1;                              // U row stride (odd: banks)
 };
-
-// (p, q) of entry k < 21 of the row-major upper triangle (inverse of sym6)
-__device__ __forceinline__ void unsym6(int k, int& p, int& q) {
-    p = 0;
-    while (k >= 6 - p) {
-        k -= 6 - p;
-        ++p;
-    }
-    q = p + k;
-}
 
 }  // namespace
 
@@ -89,9 +81,9 @@ __global__ __launch_bounds__(WG, (sizeof(T) == 4 && KT <= 16) ? 2 : 1) void k_as
     const int nTiles = nT * (nT + 1) / 2;
 
     __shared__ T S[KT + 1][27];                 // pair sums per target slot (anchor axes)
-    __shared__ T Sw[KT + 1][27];                // the same in world axes; slot 0: their sum
     __shared__ T Trel[2][KT + 1][12];           // (G^c_f)^-1 G_a for the group's frames
-    __shared__ double Ainv[36];                 // Ad(G_a^-1), row-major
+    __shared__ double Ainv[36];                 // Ad(G_a^-1) of the state being linearised, row-major
+    __shared__ double AinvB[36];                // the same at the linearisation point (back-substitution)
     __shared__ double dd[KT + 1][6];            // z_a - z_{f_s} (back-substitution)
     __shared__ T U[CL * US];                    // per (landmark, slot >= 1): J_a^T j_alpha
     __shared__ T ccp[CHUNK_OBS], ggp[CHUNK_OBS];
@@ -165,20 +157,30 @@ __global__ __launch_bounds__(WG, (sizeof(T) == 4 && KT <= 16) ? 2 : 1) void k_as
 #pragma unroll
         for (int i = 0; i < 12; ++i) Trel[c][s][i] = (T)Tr[i];
     }
-    if (tid >= 64 && tid < 100) {
-        const int e = tid - 64;
+    if (tid >= 64 && tid < 136) {
+        const int e = (tid - 64) % 36;
         double Gai[12];
-        pose_load(bout.GinvL + 12 * a, Gai);
-        Ainv[e] = adjoint_entry(Gai, e / 6, e % 6);
+        pose_load((tid < 100 ? bout.GinvL : bin.GinvL) + 12 * a, Gai);
+        (tid < 100 ? Ainv : AinvB)[e] = adjoint_entry(Gai, e / 6, e % 6);
     }
+    lds_barrier();
     if (update && tid >= 128 && tid < 128 + (m + 1)) {
+        // z_a - z_{f_s} in the anchor's axes at the linearisation point
         const int s = tid - 128;
         const int f = frl[s];
+        double w[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
             const double za = a < NF ? A.delta[delta_off + 6 * a + q] : 0.0;
             const double zf = f < NF ? A.delta[delta_off + 6 * f + q] : 0.0;
-            dd[s][q] = za - zf;
+            w[q] = za - zf;
+        }
+#pragma unroll
+        for (int p = 0; p < 6; ++p) {
+            double t = 0.0;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) t += AinvB[6 * p + q] * w[q];
+            dd[s][p] = t;
         }
     }
     lds_barrier();
@@ -318,7 +320,7 @@ __global__ __launch_bounds__(WG, (sizeof(T) == 4 && KT <= 16) ? 2 : 1) void k_as
     }
     lds_barrier();
 
-    // ---- phase 2: C, g_alpha per landmark; pair sums to world axes --------------
+    // ---- phase 2: C and g_alpha per landmark, the rows of V --------------------
     for (int i = tid; i < SH::VROWS * VS; i += WG) Vsh[i] = (T)0;
     T gl_grad = (T)0;
     if (tid < nl) {
@@ -339,48 +341,21 @@ __global__ __launch_bounds__(WG, (sizeof(T) == 4 && KT <= 16) ? 2 : 1) void k_as
         iscl[tid] = isc;
         gl_grad = G * isc;
     }
-    // S_w(s) = Ainv^T S(s) Ainv, s_w(s) = Ainv^T s(s), slots 1..m
-    for (int it = tid; it < m * 27; it += WG) {
-        const int s = it / 27 + 1, k = it - (s - 1) * 27;
-        double v = 0.0;
-        if (k < 21) {
-            int p, q;
-            unsym6(k, p, q);
-#pragma unroll
-            for (int x = 0; x < 6; ++x) {
-                double r = 0.0;
-#pragma unroll
-                for (int y = 0; y < 6; ++y) r += (double)S[s][sym6(x, y)] * Ainv[6 * y + q];
-                v += Ainv[6 * x + p] * r;
-            }
-        } else {
-#pragma unroll
-            for (int x = 0; x < 6; ++x) v += Ainv[6 * x + (k - 21)] * (double)S[s][21 + x];
-        }
-        Sw[s][k] = (T)v;
-    }
     lds_barrier();
     if (tid < nl) Vsh[K6 * VS + tid] = gl_grad;   // the gradient row
-    // u in world axes, stored for the back-substitution, and the rows of V:
-    // [u_1; ...; u_m] / sqrt(C + eps)
-    for (int it = tid; it < nl * 6; it += WG) {
-        const int j = it / 6, q = it - j * 6;
-        const T isc = iscl[j];
-        T* vo = bout.v + voff + (long long)j * M6 + q;
-        const T* uj = U + j * US;
-        for (int s = 1; s <= m; ++s) {
-            T uw = (T)0;
-#pragma unroll
-            for (int p = 0; p < 6; ++p) uw += (T)Ainv[6 * p + q] * uj[6 * (s - 1) + p];
-            vo[6 * (s - 1)] = uw;
-            Vsh[(6 * (s - 1) + q) * VS + j] = uw * isc;
-        }
+    // u in the anchor's axes, stored for the back-substitution, and the rows of
+    // V: [u_1; ...; u_m] / sqrt(C + eps)
+    for (int it = tid; it < nl * M6; it += WG) {
+        const int j = it / M6, k = it - j * M6;
+        const T u = U[j * US + k];
+        bout.v[voff + it] = u;
+        Vsh[k * VS + j] = u * iscl[j];
     }
     lds_barrier();
 
-    // ---- phase 3: V V^T and V g on MFMA, tiles stored as B = diag(S_w) - Y -----
+    // ---- phase 3: V V^T and V g on MFMA, tiles stored as B_a = diag(S) - Y ----
     const int kend = (nl + 3) & ~3;
-    T* Dg = bout.D + D_off;
+    T* Da = bout.D + D_off;
     for (int tile = wv; tile < nTiles; tile += NWAVE) {
         int r = 0;
         while ((r + 1) * (r + 2) / 2 <= tile) ++r;
@@ -395,32 +370,58 @@ __global__ __launch_bounds__(WG, (sizeof(T) == 4 && KT <= 16) ? 2 : 1) void k_as
             const int col = c * 16 + (lane & 15);
             if (row < K6 && col < K6) {
                 const int sr = row / 6, sc = col / 6;
-                const T q = sr == sc ? Sw[sr + 1][sym6(row % 6, col % 6)] : (T)0;
-                Dg[(long long)row * K6 + col] = q - acc[v];
+                const T q = sr == sc ? S[sr + 1][sym6(row % 6, col % 6)] : (T)0;
+                Da[(long long)row * K6 + col] = q - acc[v];
             } else if (row == K6 && col < K6) {
-                bout.d[d_off + col] = Sw[col / 6 + 1][21 + col % 6] - acc[v];
+                bout.d[d_off + col] = S[col / 6 + 1][21 + col % 6] - acc[v];
             }
         }
     }
-    // the anchor's blocks from the stored values (the barrier makes this
-    // workgroup's global stores visible to its own loads): R(s) row-block sums
-    // of B, their total, and the gradient total, in double
+    // to world axes in double (the barrier makes this workgroup's global stores
+    // visible to its own loads): the lower blocks of B and the gradient b
     __syncthreads();
+    double* Dw = bout.Dw + D_off;
     double* Rg = bout.Rz + r_off;
+    const int nlow = m * (m + 1) / 2;
+    for (int it = tid; it < 36 * nlow; it += WG) {
+        const int blk = it / 36, e = (it / 6) % 6, j = it % 6;
+        int sr = 0;
+        while ((sr + 1) * (sr + 2) / 2 <= blk) ++sr;
+        const int sc = blk - sr * (sr + 1) / 2;
+        double acc = 0.0;
+#pragma unroll
+        for (int p = 0; p < 6; ++p) {
+            double t = 0.0;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const int x = 6 * sr + p, y = 6 * sc + q;
+                t += (double)(x >= y ? Da[(long long)x * K6 + y] : Da[(long long)y * K6 + x]) * Ainv[6 * q + j];
+            }
+            acc += Ainv[6 * p + e] * t;
+        }
+        Dw[(long long)(6 * sr + e) * K6 + 6 * sc + j] = acc;
+    }
+    for (int it = tid; it < M6; it += WG) {
+        const int s0 = it / 6, e = it % 6;
+        double acc = 0.0;
+#pragma unroll
+        for (int p = 0; p < 6; ++p) acc += Ainv[6 * p + e] * (double)bout.d[d_off + 6 * s0 + p];
+        Rg[36 * (m + 1) + 6 + it] = acc;
+    }
+    __syncthreads();
+    // the anchor's blocks: R(s) = sum_s' B(s, s') (upper blocks as transposes),
+    // their total and the gradient total
     for (int it = tid; it < 36 * m; it += WG) {
         const int s0 = it / 36, e = (it / 6) % 6, j = it % 6;
-        const int x = 6 * s0 + e;
         double acc = 0.0;
-        for (int s1 = 0; s1 < m; ++s1) {
-            const int y = 6 * s1 + j;
-            acc += (double)(x >= y ? Dg[(long long)x * K6 + y] : Dg[(long long)y * K6 + x]);
-        }
+        for (int s1 = 0; s1 < m; ++s1)
+            acc += s1 <= s0 ? Dw[(long long)(6 * s0 + e) * K6 + 6 * s1 + j] : Dw[(long long)(6 * s1 + j) * K6 + 6 * s0 + e];
         Rsh[s0][6 * e + j] = acc;
         Rg[36 * (s0 + 1) + 6 * e + j] = acc;
     }
     if (tid >= 192 && tid < 198) {
         double bt = 0.0;
-        for (int s1 = 0; s1 < m; ++s1) bt += (double)bout.d[d_off + 6 * s1 + (tid - 192)];
+        for (int s1 = 0; s1 < m; ++s1) bt += Rg[36 * (m + 1) + 6 + 6 * s1 + (tid - 192)];
         Rg[36 * (m + 1) + (tid - 192)] = bt;
     }
     lds_barrier();

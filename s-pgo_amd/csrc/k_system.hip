@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_form_frames(Layout lay, StateBuf<T> B0,
                 if (z.K6 > 0) {
                     const long long idx = (z.doff >= 0 && j > e) ? z.off + (long long)j * z.K6 + e
                                                                  : z.off + (long long)e * z.K6 + j;
-                    h += (double)bs.D[idx];
+                    h += bs.Dw[idx];
                 } else {
                     const double v = bs.Rz[z.off + 6 * e + j];
                     h += z.K6 == -2 ? v : -v;
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(256) void k_form_frames(Layout lay, StateBuf<T> B0,
             for (int q = lay.zp_cbeg[k]; q < lay.zp_cbeg[k + 1]; ++q) {
                 const ZCon z = lay.zp_con[q];
                 if (z.K6 == -2) gs += bs.Rz[z.doff + tid];
-                else gs -= (double)bs.d[z.doff + tid];
+                else gs -= bs.Rz[z.doff + tid];
             }
         }
         gvec[6 * r + tid] = gs;

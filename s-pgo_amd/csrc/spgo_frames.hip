@@ -8,9 +8,10 @@
 // (Solver_cuda.cu:248-309 accepts any target after the anchor).  A group is
 // the landmarks of one anchor frame whose distinct target frames together
 // number at most kt (16, 32 or 64, the kernel shape), within one LDS chunk.
-// Per group the device keeps the target-frame block B (6m x 6m), its gradient
-// b (6m) and, in double, the anchor's blocks R(s) = sum_s' B(s, s') (row sums
-// of B by target frame), their total and the gradient total.
+// Per group the device keeps, in double and world axes, the target-frame block
+// B (6m x 6m, lower blocks), and in Rz the anchor's blocks R(s) = sum_s' B(s, s')
+// (row sums of B by target frame) after their total, then the gradient total
+// and the gradient b (6m).
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -168,7 +169,7 @@ int build_layout_frames(spgo_solver* h, HostLayout& out) {
         hl.v_total += (long long)nl * 6 * m;
         hl.D_total += (long long)(6 * m) * (6 * m);
         hl.d_total += 6 * m;
-        hl.Rz_total += 36LL * (m + 1) + 6;
+        hl.Rz_total += 36LL * (m + 1) + 6 + 6LL * m;
         const size_t fb = hl.grp_frames.size();
         hl.grp_frames.resize(fb + KTF + 1, -1);
         hl.grp_frames[fb] = a;
@@ -237,7 +238,7 @@ int build_layout_frames(spgo_solver* h, HostLayout& out) {
                 if (e.sc > 0) {          // two target frames: a block of B
                     z.off = hl.grp_D_off[e.g] + (long long)(6 * (e.sr - 1)) * K6 + 6 * (e.sc - 1);
                     z.K6 = K6;
-                    z.doff = e.sr == e.sc ? hl.grp_d_off[e.g] + 6 * (e.sr - 1) : -1;
+                    z.doff = e.sr == e.sc ? (int)(ro + 36LL * (m + 1) + 6 + 6 * (e.sr - 1)) : -1;
                 } else if (e.sr > 0) {   // target frame against the anchor: -R(s)
                     z.off = ro + 36LL * e.sr;
                     z.K6 = -1;

@@ -37,12 +37,12 @@ struct FsEntry {
 
 // Frame coordinates: one contribution to a block (row frame r, column frame c)
 // of the reduced system from a landmark group.  K6 > 0: a block of the group's
-// target-frame block B (StateBuf::D), element (e, j) at off + e K6 + j (a
-// diagonal block is read from its lower triangle); K6 = -1: minus the row sum
-// R(s) of the target frame r against the anchor c (StateBuf::Rz, off + 6e + j);
-// K6 = -2: the anchor's own block sum_s R(s).  doff, on the diagonal only: the
-// gradient, minus b(s) at d[doff ..] (K6 > 0) or plus sum_s b(s) at Rz[doff ..]
-// (K6 = -2); -1 off the diagonal.
+// target-frame block B (StateBuf::Dw, world axes), element (e, j) at
+// off + e K6 + j (a diagonal block is read from its lower triangle); K6 = -1:
+// minus the row sum R(s) of the target frame r against the anchor c
+// (StateBuf::Rz, off + 6e + j); K6 = -2: the anchor's own block sum_s R(s).
+// doff, on the diagonal only, indexes the gradient in Rz: minus b(s) (K6 > 0)
+// or plus sum_s b(s) (K6 = -2); -1 off the diagonal.
 struct ZCon {
     long long off;
     int K6, doff;
@@ -140,7 +140,8 @@ struct StateBuf {
     T* D;             // per group (6K)^2 reduced block
     T* d;             // per group 6K reduced gradient
     double* cost;     // per group robust cost
-    double* Rz;       // frame coordinates: per group the anchor's blocks (see ZCon), double
+    double* Rz;       // frame coordinates: per group the anchor's blocks and the gradient (see ZCon)
+    double* Dw;       // frame coordinates: per group B in world axes (D holds it in the anchor's)
 };
 
 }  // namespace spgo

@@ -378,8 +378,9 @@ int alloc_state(spgo_solver* h, StateBuf<T>* b, const HostLayout& hl, int n_act)
         b[i].d = h->mem.alloc<T>((size_t)hl.d_total);
         b[i].cost = h->mem.alloc<double>(hl.grp_anchor.size());
         b[i].Rz = hl.frame_mode ? h->mem.alloc<double>((size_t)hl.Rz_total) : nullptr;
+        b[i].Dw = hl.frame_mode ? h->mem.alloc<double>((size_t)hl.D_total) : nullptr;
         if (!b[i].X || !b[i].alpha || !b[i].G || !b[i].GinvL || !b[i].GinvR || !b[i].v || !b[i].Ct || !b[i].g ||
-            !b[i].D || !b[i].d || !b[i].cost || (hl.frame_mode && !b[i].Rz))
+            !b[i].D || !b[i].d || !b[i].cost || (hl.frame_mode && (!b[i].Rz || !b[i].Dw)))
             return h->fail(SPGO_E_DEVICE, "device allocation failed");
     }
     return SPGO_OK;
