@@ -75,9 +75,9 @@ def test_revisit_trajectory_full_size_cholesky_and_cg():
     # 1,024 keyframes (a 6,144 x 6,144 reduced system), 400,000 landmarks, 12
     # observations each, the last 20 % of the frames revisiting the first:
     # landmarks observed again ~820 frames after their anchor, solved in frame
-    # coordinates.  The FP64 and mixed-precision Cholesky paths and LM with CG
-    # (the network trained on revisit systems of 128-256 keyframes, Cholesky
-    # fallback when CG does not reach 1e-8) end at the same cost.
+    # coordinates.  The FP64 Cholesky path and LM with CG (the network trained
+    # on revisit systems of 128-256 keyframes, the Cholesky computing the step
+    # when CG does not reach 1e-8) end at the same cost; costs fall monotonically.
     from spgo import _lib
     from spgo.lpcg import LearnedPCG, solver_systems
     from spgo.problems import make_revisit_problem
@@ -90,13 +90,12 @@ def test_revisit_trajectory_full_size_cholesky_and_cg():
     p = make_revisit_problem(1024, 400000, K, seed=5005)
     assert int(np.max(p.meas_t - p.meas_a)) > 800
     finals = {}
-    for tag, kw in (("cholesky", {}), ("mixed", {"precision": "mixed"}),
-                    ("cg", {"factorization": 4, "pcg_max_iter": 100})):
+    for tag, kw in (("cholesky", {}), ("cg", {"factorization": 4, "pcg_max_iter": 100})):
         s = solver_from_problem(p, make_options(mode="lm", **kw))
         assert s.layout_info()["coordinates"] == _lib.SPGO_COORD_FRAMES
         if tag == "cg":
             s.set_preconditioner(net)
-        s.step(8)
+        s.step(12)
         c0, recs = s.stats()
         costs = [r["cost"] for r in recs]
         assert all(b <= a for a, b in zip(costs, costs[1:])), tag
@@ -107,8 +106,7 @@ def test_revisit_trajectory_full_size_cholesky_and_cg():
         finals[tag] = costs[-1]
         s.close()
     net.close()
-    for tag in ("mixed", "cg"):
-        rel = abs(finals[tag] - finals["cholesky"]) / finals["cholesky"]
-        print(f"[full-c5] {tag}: final cost {finals[tag]:.12g} against {finals['cholesky']:.12g}, "
-              f"relative difference {rel:.3g}", flush=True)
-        assert rel <= FINAL, tag
+    rel = abs(finals["cg"] - finals["cholesky"]) / finals["cholesky"]
+    print(f"[full-c5] CG: final cost {finals['cg']:.12g} against {finals['cholesky']:.12g}, "
+          f"relative difference {rel:.3g}", flush=True)
+    assert rel <= FINAL
