@@ -158,6 +158,7 @@ def main() -> int:
                     help="c4: use the tile shape given by --chunk-landmarks/--group-chunks instead of the autotuner")
     ap.add_argument("--c5-systems", type=int, default=2000, help="c5: systems the network is trained on")
     ap.add_argument("--pcg-max-iter", type=int, default=200, help="c5: CG iterations before the Cholesky fallback")
+    ap.add_argument("--c5-forms", default="", help="c5: comma-separated subset of the forms to run (default: all)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 256 MiB five times (counter calibration)")
     args = ap.parse_args()
@@ -385,7 +386,9 @@ def bench_c3(args) -> int:
     p = make_config("c3_mixed")
     log(f"problem generated in {time.perf_counter() - t0:.1f} s: {p.n_poses} keyframes, {p.n_obs} measurements")
     out = {}
-    tile = {k: v for k, v in (("chunk_landmarks", args.chunk_landmarks), ("group_chunks", args.group_chunks)) if v}
+    # tile shape: 64 landmarks per chunk, two chunks per group, the fastest
+    # shape of both forms on this workload (a sweep over 32/48/64 x 1/2/4)
+    tile = {"chunk_landmarks": args.chunk_landmarks or 64, "group_chunks": args.group_chunks or 2}
     X0 = p.X0.astype(np.float64)
     a0 = p.alpha0.astype(np.float64)
     for prec in ("fp64", "mixed"):
@@ -460,6 +463,7 @@ def bench_c3(args) -> int:
                 "5% outliers of up to 50 px)",
         "config": {"workload": "256 keyframes, 100000 landmarks, 10 stereo observations each (1000000 stereo "
                                "measurements), 5% outliers, Huber 2 px, LM lambda0 1e-4",
+                   "tile": tile,
                    "keyframes": p.n_poses, "landmarks": p.n_landmarks, "measurements": p.n_obs,
                    "reduced_system": 6 * p.n_poses},
         "final_cost_rel_diff_vs_fp64": best["final_cost_rel_diff_vs_fp64"],
@@ -653,10 +657,14 @@ def bench_c5(args) -> int:
     X0 = p.X0.astype(np.float64)
     a0 = p.alpha0.astype(np.float64)
     runs = {}
-    for tag, kw, use_net in (("cholesky", {}, False), ("cholesky_mixed", {"precision": "mixed"}, False),
-                             ("cg_block_jacobi", {"factorization": 4}, False),
-                             ("cg_learned", {"factorization": 4}, True),
-                             ("cg_learned_mixed", {"factorization": 4, "precision": "mixed"}, True)):
+    forms = (("cholesky", {}, False), ("cholesky_mixed", {"precision": "mixed"}, False),
+             ("cg_block_jacobi", {"factorization": 4}, False),
+             ("cg_learned", {"factorization": 4}, True),
+             ("cg_learned_mixed", {"factorization": 4, "precision": "mixed"}, True))
+    wanted = (set(args.c5_forms.split(",")) | {"cholesky"}) if args.c5_forms else None
+    for tag, kw, use_net in forms:
+        if wanted is not None and tag not in wanted:
+            continue
         s = solver_from_problem(p, make_options(mode="lm", pcg_max_iter=args.pcg_max_iter, **kw))
         if use_net:
             s.set_preconditioner(net)
