@@ -226,7 +226,11 @@ template void launch_form_system<float>(const Layout&, const StateBuf<float>*, C
 // ---------------------------------------------------------------------------
 // blocked Cholesky, block column k
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_chol_panel(double* H, double* Linv, int Pp, int k, Control* ctl) {
+// Blocks of step k: block 0 factors the diagonal block (and inverts it), block
+// b > 0 the b-th active block row act[act_off[k] + b - 1] (the block rows below
+// k whose profile reaches column block k; the others are zero there).
+__global__ __launch_bounds__(256) void k_chol_panel(double* H, double* Linv, int Pp, int k, const int* act_off,
+                                                    const int* act, Control* ctl) {
     __shared__ double Lkk[NB][NB + 1];
     __shared__ double Li[NB][NB + 1];
     __shared__ double Ab[NB][NB + 1];
@@ -288,7 +292,7 @@ __global__ __launch_bounds__(256) void k_chol_panel(double* H, double* Linv, int
         }
         return;
     }
-    const int b0 = k0 + blockIdx.x * NB;
+    const int b0 = act[act_off[k] + blockIdx.x - 1] * NB;
     for (int e = tid; e < NB * NB; e += 256) {
         const int rr = e / NB, cc = e % NB;
         Ab[rr][cc] = H[(long long)(b0 + rr) * Pp + k0 + cc];
@@ -303,18 +307,20 @@ __global__ __launch_bounds__(256) void k_chol_panel(double* H, double* Linv, int
     }
 }
 
-// Trailing update A_ij -= L_ik L_jk^T for k < j <= i, one NB x NB tile per
-// workgroup, four 16x16 MFMA tiles per wavefront-quarter of the block.
-__global__ __launch_bounds__(256) void k_chol_update(double* H, int Pp, int k, int nblk, const Control* ctl) {
+// Trailing update A_ij -= L_ik L_jk^T for the active block rows j <= i of step
+// k, one NB x NB tile per workgroup, four 16x16 MFMA tiles per
+// wavefront-quarter of the block.
+__global__ __launch_bounds__(256) void k_chol_update(double* H, int Pp, int k, const int* act_off, const int* act,
+                                                     const Control* ctl) {
     if (ctl->pcg_fail == 0) return;   // the CG solve of this iteration converged
-    const int n = nblk - k - 1;
+    const int a0 = act_off[k], n = act_off[k + 1] - a0;
     const int t = blockIdx.x;
     int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
     while (ti * (ti + 1) / 2 > t) --ti;
     const int tj = t - ti * (ti + 1) / 2;
     if (ti >= n) return;
-    const int bi = k + 1 + ti, bj = k + 1 + tj;
+    const int bi = act[a0 + ti], bj = act[a0 + tj];
     __shared__ double Ai[NB][NB + 2];
     __shared__ double Aj[NB][NB + 2];
     const int tid = threadIdx.x;
@@ -342,13 +348,35 @@ __global__ __launch_bounds__(256) void k_chol_update(double* H, int Pp, int k, i
     }
 }
 
-void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t s) {
+void launch_cholesky(double* H, double* Linv, int Pp, const int* act_off, const int* act, const int* nact,
+                     Control* ctl, hipStream_t s) {
     const int nblk = Pp / NB;
     for (int k = 0; k < nblk; ++k) {
-        hipLaunchKernelGGL(k_chol_panel, dim3(nblk - k), dim3(256), 0, s, H, Linv, Pp, k, ctl);
-        const int n = nblk - k - 1;
-        if (n > 0) hipLaunchKernelGGL(k_chol_update, dim3(n * (n + 1) / 2), dim3(256), 0, s, H, Pp, k, nblk, ctl);
+        const int n = nact[k];
+        hipLaunchKernelGGL(k_chol_panel, dim3(1 + n), dim3(256), 0, s, H, Linv, Pp, k, act_off, act, ctl);
+        if (n > 0) hipLaunchKernelGGL(k_chol_update, dim3(n * (n + 1) / 2), dim3(256), 0, s, H, Pp, k, act_off, act, ctl);
     }
+}
+
+void dense_profile(int Pp, const std::vector<int>& fb16, std::vector<int>& first, std::vector<int>& act_off,
+                   std::vector<int>& act, std::vector<int>& nact) {
+    const int nblk = Pp / NB, r = NB / 16;
+    first.assign(nblk, 0);
+    for (int i = 0; i < nblk; ++i) {
+        int f = i;
+        for (int q = 0; q < r && r * i + q < (int)fb16.size(); ++q) f = std::min(f, fb16[r * i + q] / r);
+        first[i] = f;
+    }
+    act_off.assign(nblk + 1, 0);
+    act.clear();
+    nact.assign(nblk, 0);
+    for (int k = 0; k < nblk; ++k) {
+        act_off[k] = (int)act.size();
+        for (int i = k + 1; i < nblk; ++i)
+            if (first[i] <= k) act.push_back(i);
+        nact[k] = (int)act.size() - act_off[k];
+    }
+    act_off[nblk] = (int)act.size();
 }
 
 // ---------------------------------------------------------------------------
@@ -1136,30 +1164,50 @@ int chol_band_block() { return EB; }
 // ---------------------------------------------------------------------------
 // L L^T x = g, one workgroup, right-hand side in LDS
 // ---------------------------------------------------------------------------
+// Both triangular solves in one workgroup, within the block profile: forward,
+// block row k's solution y_k = L_kk^-1 r_k updates the active block rows below
+// it (32 lanes per row read the row's 32 entries of column block k together);
+// backward, x_k = L_kk^-T y_k updates the columns of block row k's profile.
 __global__ __launch_bounds__(1024) void k_chol_solve(const double* H, const double* Linv, const double* g,
-                                                     double* x, int Pp, const Control* ctl) {
+                                                     double* x, int Pp, const int* first, const int* act_off,
+                                                     const int* act, const Control* ctl) {
     extern __shared__ double rhs[];
     __shared__ double y[NB];
     if (ctl->pcg_fail == 0) return;   // the CG solve of this iteration converged
     const int tid = threadIdx.x;
     const int nblk = Pp / NB;
+    const int grp = tid >> 5, m = tid & 31;   // 32 groups of 32 lanes
     for (int i = tid; i < Pp; i += 1024) rhs[i] = g[i];
     __syncthreads();
     for (int k = 0; k < nblk; ++k) {
         const double* Li = Linv + (long long)k * NB * NB;
         if (tid < NB) {
             double s = 0.0;
-            for (int m = 0; m <= tid; ++m) s += Li[tid * NB + m] * rhs[k * NB + m];
+            for (int q = 0; q <= tid; ++q) s += Li[tid * NB + q] * rhs[k * NB + q];
             y[tid] = s;
         }
         __syncthreads();
         if (tid < NB) rhs[k * NB + tid] = y[tid];
-        for (int i = (k + 1) * NB + tid; i < Pp; i += 1024) {
-            const double* row = H + (long long)i * Pp + k * NB;
-            double s = 0.0;
-#pragma unroll 8
-            for (int m = 0; m < NB; ++m) s += row[m] * y[m];
-            rhs[i] -= s;
+        const int a0 = act_off[k], nr = (act_off[k + 1] - a0) * NB;
+        const double ym = y[m];
+        // four rows per group at a time, their loads issued together (rows past
+        // the end repeat the last one and are not written)
+        for (int rr0 = 4 * grp; rr0 < nr; rr0 += 128) {
+            double v[4];
+            int ii[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int rr = min(rr0 + u, nr - 1);
+                ii[u] = act[a0 + rr / NB] * NB + rr % NB;
+                v[u] = H[(long long)ii[u] * Pp + k * NB + m];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                double s = v[u] * ym;
+#pragma unroll
+                for (int off = 16; off > 0; off >>= 1) s += __shfl_xor(s, off, 32);
+                if (m == 0 && rr0 + u < nr) rhs[ii[u]] -= s;
+            }
         }
         __syncthreads();
     }
@@ -1167,16 +1215,22 @@ __global__ __launch_bounds__(1024) void k_chol_solve(const double* H, const doub
         const double* Li = Linv + (long long)k * NB * NB;
         if (tid < NB) {
             double s = 0.0;
-            for (int m = tid; m < NB; ++m) s += Li[m * NB + tid] * rhs[k * NB + m];
+            for (int q = tid; q < NB; ++q) s += Li[q * NB + tid] * rhs[k * NB + q];
             y[tid] = s;
         }
         __syncthreads();
         if (tid < NB) rhs[k * NB + tid] = y[tid];
-        for (int j = tid; j < k * NB; j += 1024) {
-            double s = 0.0;
-#pragma unroll 8
-            for (int m = 0; m < NB; ++m) s += H[(long long)(k * NB + m) * Pp + j] * y[m];
-            rhs[j] -= s;
+        for (int j = first[k] * NB + tid; j < k * NB; j += 1024) {
+            double hv[NB];
+#pragma unroll
+            for (int q = 0; q < NB; ++q) hv[q] = H[(long long)(k * NB + q) * Pp + j];
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int q = 0; q < NB; q += 2) {
+                s0 += hv[q] * y[q];
+                s1 += hv[q + 1] * y[q + 1];
+            }
+            rhs[j] -= s0 + s1;
         }
         __syncthreads();
     }
@@ -1184,8 +1238,9 @@ __global__ __launch_bounds__(1024) void k_chol_solve(const double* H, const doub
 }
 
 void launch_chol_solve(const double* H, const double* Linv, const double* g, double* delta, int Pp,
-                       const Control* ctl, hipStream_t s) {
-    hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), Pp * sizeof(double), s, H, Linv, g, delta, Pp, ctl);
+                       const int* first, const int* act_off, const int* act, const Control* ctl, hipStream_t s) {
+    hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), Pp * sizeof(double), s, H, Linv, g, delta, Pp, first,
+                       act_off, act, ctl);
 }
 
 }  // namespace spgo

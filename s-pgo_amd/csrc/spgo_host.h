@@ -121,6 +121,10 @@ struct spgo_solver {
     const int* firstB = nullptr;   // split: profile of the reversed bottom part
     double *Ssep = nullptr, *Smrg = nullptr, *winvS = nullptr, *LpS = nullptr;   // split: separator scratch
     double* Lpan = nullptr;     // envelope: factored panels for the backward solve
+    // dense path: the factor's block profile (NB-wide blocks) and per step the
+    // block rows it reaches (nact on the host for the launch sizes)
+    const int *dense_first = nullptr, *dense_act_off = nullptr, *dense_act = nullptr;
+    std::vector<int> dense_nact;
     bool pcg = false;           // SPGO_FACTOR_PCG: CG first, the envelope path above as fallback
     spgo_lpcg* net = nullptr;   // its preconditioner network (nullptr: block Jacobi)
     const float* net_params = nullptr;   // the network's device parameters for this step

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "spgo_internal.h"
 
 namespace spgo {
@@ -29,7 +31,15 @@ template <typename T>
 void launch_form_system(const Layout& lay, const StateBuf<T>* bufs, Control* ctls, double* H, double* g,
                         hipStream_t s);
 
-void launch_cholesky(double* H, double* Linv, int Pp, Control* ctl, hipStream_t s);
+// dense blocked Cholesky within a block profile (NB-wide blocks): per step k the
+// active block rows act[act_off[k] ..] below k whose profile reaches column
+// block k, nact[k] of them (host array: launch sizes)
+void launch_cholesky(double* H, double* Linv, int Pp, const int* act_off, const int* act, const int* nact,
+                     Control* ctl, hipStream_t s);
+// the NB-block profile and active lists of the dense Cholesky from the
+// 16-row block profile fb16 (first nonzero 16-column block of each 16-row block)
+void dense_profile(int Pp, const std::vector<int>& fb16, std::vector<int>& first, std::vector<int>& act_off,
+                   std::vector<int>& act, std::vector<int>& nact);
 // envelope Cholesky and both solves, one workgroup per problem of the table;
 // Dmax the largest profile depth, lds the largest chol_band_lds of the problems
 void launch_chol_band(const ProbDesc* probs, int n_probs, int Dmax, size_t lds, double* H, const int* first,
@@ -45,7 +55,7 @@ int prepare_chol_band(int Dmax, size_t lds);
 int chol_band_block();
 size_t chol_band_lds(int Pp, int D);
 void launch_chol_solve(const double* H, const double* Linv, const double* g, double* delta, int Pp,
-                       const Control* ctl, hipStream_t s);
+                       const int* first, const int* act_off, const int* act, const Control* ctl, hipStream_t s);
 
 template <typename T>
 void launch_pose_update(const Layout& lay, const Camera& cam, StateBuf<T>* bufs, const Control* ctl,

@@ -506,6 +506,14 @@ int setup_layout(spgo_solver* h) {
         if (h->band && !(h->Lpan = h->mem.alloc<double>(chol_panel_doubles(h->Pp))))
             return h->fail(SPGO_E_DEVICE, "device allocation failed");
         if ((rc = upload(h, fb, &h->first))) return rc;
+        {
+            // the dense path's block profile: zero blocks of the factor are skipped
+            std::vector<int> f32, aoff, act;
+            dense_profile(h->Pp, fb, f32, aoff, act, h->dense_nact);
+            if ((rc = upload(h, f32, &h->dense_first)) || (rc = upload(h, aoff, &h->dense_act_off)) ||
+                (rc = upload(h, act, &h->dense_act)))
+                return rc;
+        }
         h->band_depth = bw;
         if (h->path == SPGO_FACTOR_ENVELOPE && prepare_chol_band(bw, chol_band_lds(h->Pp, bw)) != 0)
             return h->fail(SPGO_E_DEVICE, "cannot reserve LDS for the envelope Cholesky");
@@ -634,8 +642,9 @@ int launch_iteration(spgo_solver* h, StateBuf<T>* b) {
         if (h->band) {
             launch_chol(h, s);
         } else {
-            launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
-            launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, h->ctl, s);
+            launch_cholesky(h->H, h->Linv, h->Pp, h->dense_act_off, h->dense_act, h->dense_nact.data(), h->ctl, s);
+            launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, h->dense_first, h->dense_act_off,
+                              h->dense_act, h->ctl, s);
         }
     }
     launch_pose_update<T>(h->lay, make_camera(h), b, h->ctl, h->delta, (double)h->step_size, 0, s);
@@ -1305,9 +1314,10 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
             launch_chol(h, s);
             (void)hipEventRecord(ev[2], s);
         } else {
-            launch_cholesky(h->H, h->Linv, h->Pp, h->ctl, s);
+            launch_cholesky(h->H, h->Linv, h->Pp, h->dense_act_off, h->dense_act, h->dense_nact.data(), h->ctl, s);
             (void)hipEventRecord(ev[2], s);
-            launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, h->ctl, s);
+            launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, h->dense_first, h->dense_act_off,
+                              h->dense_act, h->ctl, s);
         }
         (void)hipEventRecord(ev[3], s);
         if (mixed) launch_pose_update<float>(h->lay, make_camera(h), h->bf, h->ctl, h->delta, h->step_size, 0, s);
