@@ -386,9 +386,9 @@ def bench_c3(args) -> int:
     p = make_config("c3_mixed")
     log(f"problem generated in {time.perf_counter() - t0:.1f} s: {p.n_poses} keyframes, {p.n_obs} measurements")
     out = {}
-    # tile shape: 64 landmarks per chunk, two chunks per group, the fastest
-    # shape of both forms on this workload (a sweep over 32/48/64 x 1/2/4)
-    tile = {"chunk_landmarks": args.chunk_landmarks or 64, "group_chunks": args.group_chunks or 2}
+    # tile shape: 64 landmarks per chunk; chunks per group left to the library
+    # (0), which sizes the groups to the assembly's resident workgroups
+    tile = {"chunk_landmarks": args.chunk_landmarks or 64, "group_chunks": args.group_chunks or 0}
     X0 = p.X0.astype(np.float64)
     a0 = p.alpha0.astype(np.float64)
     for prec in ("fp64", "mixed"):

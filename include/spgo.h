@@ -89,7 +89,8 @@ typedef struct spgo_options {
     int32_t reserved0;       /* unused, zero                                    */
     int32_t use_graph;       /* replay the LM iterations from captured graphs  */
     int32_t chunk_landmarks; /* tile shape: landmarks per LDS chunk (<= 64)    */
-    int32_t group_chunks;    /* tile shape: chunks per workgroup               */
+    int32_t group_chunks;    /* tile shape: chunks per workgroup; 0 (default):  */
+                             /* the count that best fills the resident workgroups */
     int32_t factorization;   /* SPGO_FACTOR_*                                   */
     int32_t pcg_max_iter;    /* CG iterations before the Cholesky fallback (PCG) */
     int32_t coordinates;     /* SPGO_COORD_*                                    */

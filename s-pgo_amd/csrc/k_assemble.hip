@@ -18,6 +18,8 @@
 // of these blocks over the groups that cover each pose pair (k_form_system).
 // Reductions are deterministic: fixed cross-lane exchange patterns and
 // wavefront-private LDS partials combined in a fixed order, no global atomics.
+#include <type_traits>
+
 #include "spgo_asm_common.h"
 #include "spgo_device.h"
 #include "spgo_kernels.h"
@@ -44,7 +46,7 @@ struct AsmShape {
 // chunk loop in registers; with one chunk per group they live only from phase 3
 // on, which keeps them out of the register peak of phase 1
 template <typename T, int KT, bool MC>
-__global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 2 : 1)) void k_assemble(AsmArgs A, StateBuf<T> B0, StateBuf<T> B1) {
+__global__ __launch_bounds__(WG, (KT <= 12 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 2 : 1)) void k_assemble(AsmArgs A, StateBuf<T> B0, StateBuf<T> B1) {
     using SH = AsmShape<KT>;
     const Layout& lay = A.lay;
     const int g = blockIdx.x;
@@ -83,9 +85,13 @@ __global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 2
     // per (landmark, slot s >= 1): u = J_a^T j_alpha of its measurements (zero if none);
     // row stride 6 KT + 1 keeps the landmarks of one wavefront in different banks
     __shared__ T U[CHUNK_LM * (6 * KT + 1)];
-    __shared__ T ccp[CHUNK_OBS], ggp[CHUNK_OBS];   // per measurement: j_alpha^2, j_alpha r
     __shared__ T iscl[CHUNK_LM];                // 1 / sqrt(C_l + eps)
-    __shared__ T Vsh[SH::VSH];                  // B columns in / pair scratch / V tile / D
+    __shared__ T Vsh[SH::VSH];                  // B columns in / per-measurement sums / V tile / D
+    // per measurement j_alpha^2 and j_alpha r, written in phase 1 and summed per
+    // landmark in phase 2 before the V tile takes the buffer over
+    static_assert(SH::VSH >= 2 * CHUNK_OBS, "the per-measurement sums fit the V buffer");
+    T* const ccp = Vsh;
+    T* const ggp = Vsh + CHUNK_OBS;
     __shared__ T lmP[CHUNK_LM][6];              // P_a and -P_a/alpha in the anchor frame
     __shared__ int2 lpos[CHUNK_LM];
     __shared__ T dly[6 * KT];
@@ -231,12 +237,101 @@ __global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 2
         // Measurements are in slot-major order, so slot s = t - a (0 for t <= a)
         // is the contiguous segment soff[s] .. soff[s+1]; its pair sums are one
         // reduction over the wavefront (no atomics, fixed order).
+        // One measurement: its residual, robust weight and cost, its share of
+        // the segment's pair sums and of u, and its j_alpha^2, j_alpha r.  A
+        // target before the anchor frame (slot 0 only) takes its relative pose
+        // from the global poses, in double; that path runs in a pass of its own
+        // so that its registers stay out of the main loop's budget.
+        auto measure = [&](const MeasIn& mi, T (&pr)[27], auto before, int seg) __attribute__((always_inline)) {
+            constexpr bool BEFORE = decltype(before)::value;
+            const int t = mi.t;
+            const int pos = (int)(mi.meta & 0xffffu);
+            const int j = (int)(mi.meta >> 16);
+            const float4 uv = mi.uv;
+            const T Pa[3] = {lmP[j][0], lmP[j][1], lmP[j][2]};
+            const T dv[3] = {lmP[j][3], lmP[j][4], lmP[j][5]};
+            const T kappa = (A.rb.near_boost && abs(t - a) < 2) ? (T)16 : (T)1;
+            T u[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
+            T cc = (T)0, gg = (T)0;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                T R[12];
+                if constexpr (!BEFORE) {
+#pragma unroll
+                    for (int i = 0; i < 12; ++i) R[i] = Trel[c][t - a][i];
+                } else {
+                    double Ga[12], Gti[12], Tr[12];
+                    pose_load(bout.G + 12 * a, Ga);
+                    pose_load((c ? bout.GinvR : bout.GinvL) + 12 * t, Gti);
+                    pose_mul(Gti, Ga, Tr);
+#pragma unroll
+                    for (int i = 0; i < 12; ++i) R[i] = (T)Tr[i];
+                }
+                const T fx = c ? (T)A.cam.fx[1] : fx0, fy = c ? (T)A.cam.fy[1] : fy0;
+                const T cx = c ? (T)A.cam.cx[1] : cx0, cy = c ? (T)A.cam.cy[1] : cy0;
+                const T Px = R[0] * Pa[0] + R[1] * Pa[1] + R[2] * Pa[2] + R[3];
+                const T Py = R[4] * Pa[0] + R[5] * Pa[1] + R[6] * Pa[2] + R[7];
+                const T Pz = R[8] * Pa[0] + R[9] * Pa[1] + R[10] * Pa[2] + R[11];
+                const T iz = rcp_refined(Pz);
+                const T ex = (c ? (T)uv.z : (T)uv.x) - (fx * Px * iz + cx);
+                const T ey = (c ? (T)uv.w : (T)uv.y) - (fy * Py * iz + cy);
+                T w;
+                robust_eval(A.rb, inv_param, ex * ex + ey * ey, kappa, w, cost);
+                // rows of M R_{a->t}: M = K_c Pi'(P_t) (LMutils.cu:615-628)
+                const T m0a = fx * iz, m0c = -fx * Px * iz * iz;
+                const T m1b = fy * iz, m1c = -fy * Py * iz * iz;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    T n[3];
+                    if (k == 0) {
+                        n[0] = m0a * R[0] + m0c * R[8];
+                        n[1] = m0a * R[1] + m0c * R[9];
+                        n[2] = m0a * R[2] + m0c * R[10];
+                    } else {
+                        n[0] = m1b * R[4] + m1c * R[8];
+                        n[1] = m1b * R[5] + m1c * R[9];
+                        n[2] = m1b * R[6] + m1c * R[10];
+                    }
+                    // weighted row [P_a x n, n] (the -[P]x of compute_del_tn_del_xi)
+                    T J[6];
+                    J[0] = w * (Pa[1] * n[2] - Pa[2] * n[1]);
+                    J[1] = w * (Pa[2] * n[0] - Pa[0] * n[2]);
+                    J[2] = w * (Pa[0] * n[1] - Pa[1] * n[0]);
+                    J[3] = w * n[0];
+                    J[4] = w * n[1];
+                    J[5] = w * n[2];
+                    const T ja = w * (n[0] * dv[0] + n[1] * dv[1] + n[2] * dv[2]);
+                    const T rk = w * (k == 0 ? ex : ey);
+                    int idx = 0;
+#pragma unroll
+                    for (int p = 0; p < 6; ++p) {
+#pragma unroll
+                        for (int q = p; q < 6; ++q) pr[idx++] += J[p] * J[q];
+                        pr[21 + p] += J[p] * rk;
+                        u[p] += J[p] * ja;
+                    }
+                    cc += ja * ja;
+                    gg += ja * rk;
+                }
+            }
+            if (seg > 0) {
+                // a cell written twice is two measurements of one landmark in one
+                // slot (the reference counts both): add, do not overwrite.  The
+                // slot starts at zero, so one or two contributions give the
+                // same sum in either order.
+#pragma unroll
+                for (int q = 0; q < 6; ++q) atomicAdd(&U[j * (6 * KT + 1) + 6 * (seg - 1) + q], u[q]);
+            }
+            ccp[pos] = cc;
+            ggp[pos] = gg;
+        };
         for (int seg = wv; seg <= K; seg += NWAVE) {
             const int sb = soffL[seg], se = soffL[seg + 1];
             T pr[27];
 #pragma unroll
             for (int q = 0; q < 27; ++q) pr[q] = (T)0;
             const int round = (seg - wv) / NWAVE;
+            bool before = false;
             for (int o = sb + lane; o - lane < se; o += 64) {
                 const bool valid = o < se;
                 MeasIn mi;
@@ -250,86 +345,15 @@ __global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 2
                     mi = meas_load(lay, ob0, o, valid);
                 }
                 if (valid) {
-                    const int t = mi.t;
-                    const int pos = (int)(mi.meta & 0xffffu);
-                    const int j = (int)(mi.meta >> 16);
-                    const float4 uv = mi.uv;
-                    const T Pa[3] = {lmP[j][0], lmP[j][1], lmP[j][2]};
-                    const T dv[3] = {lmP[j][3], lmP[j][4], lmP[j][5]};
-                    const T kappa = (A.rb.near_boost && abs(t - a) < 2) ? (T)16 : (T)1;
-                    T u[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
-                    T cc = (T)0, gg = (T)0;
-#pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-                        T R[12];
-                        if (t >= a) {
-#pragma unroll
-                            for (int i = 0; i < 12; ++i) R[i] = Trel[c][t - a][i];
-                        } else {
-                            double Ga[12], Gti[12], Tr[12];
-                            pose_load(bout.G + 12 * a, Ga);
-                            pose_load((c ? bout.GinvR : bout.GinvL) + 12 * t, Gti);
-                            pose_mul(Gti, Ga, Tr);
-#pragma unroll
-                            for (int i = 0; i < 12; ++i) R[i] = (T)Tr[i];
-                        }
-                        const T fx = c ? (T)A.cam.fx[1] : fx0, fy = c ? (T)A.cam.fy[1] : fy0;
-                        const T cx = c ? (T)A.cam.cx[1] : cx0, cy = c ? (T)A.cam.cy[1] : cy0;
-                        const T Px = R[0] * Pa[0] + R[1] * Pa[1] + R[2] * Pa[2] + R[3];
-                        const T Py = R[4] * Pa[0] + R[5] * Pa[1] + R[6] * Pa[2] + R[7];
-                        const T Pz = R[8] * Pa[0] + R[9] * Pa[1] + R[10] * Pa[2] + R[11];
-                        const T iz = rcp_refined(Pz);
-                        const T ex = (c ? (T)uv.z : (T)uv.x) - (fx * Px * iz + cx);
-                        const T ey = (c ? (T)uv.w : (T)uv.y) - (fy * Py * iz + cy);
-                        T w;
-                        robust_eval(A.rb, inv_param, ex * ex + ey * ey, kappa, w, cost);
-                        // rows of M R_{a->t}: M = K_c Pi'(P_t) (LMutils.cu:615-628)
-                        const T m0a = fx * iz, m0c = -fx * Px * iz * iz;
-                        const T m1b = fy * iz, m1c = -fy * Py * iz * iz;
-#pragma unroll
-                        for (int k = 0; k < 2; ++k) {
-                            T n[3];
-                            if (k == 0) {
-                                n[0] = m0a * R[0] + m0c * R[8];
-                                n[1] = m0a * R[1] + m0c * R[9];
-                                n[2] = m0a * R[2] + m0c * R[10];
-                            } else {
-                                n[0] = m1b * R[4] + m1c * R[8];
-                                n[1] = m1b * R[5] + m1c * R[9];
-                                n[2] = m1b * R[6] + m1c * R[10];
-                            }
-                            // weighted row [P_a x n, n] (the -[P]x of compute_del_tn_del_xi)
-                            T J[6];
-                            J[0] = w * (Pa[1] * n[2] - Pa[2] * n[1]);
-                            J[1] = w * (Pa[2] * n[0] - Pa[0] * n[2]);
-                            J[2] = w * (Pa[0] * n[1] - Pa[1] * n[0]);
-                            J[3] = w * n[0];
-                            J[4] = w * n[1];
-                            J[5] = w * n[2];
-                            const T ja = w * (n[0] * dv[0] + n[1] * dv[1] + n[2] * dv[2]);
-                            const T rk = w * (k == 0 ? ex : ey);
-                            int idx = 0;
-#pragma unroll
-                            for (int p = 0; p < 6; ++p) {
-#pragma unroll
-                                for (int q = p; q < 6; ++q) pr[idx++] += J[p] * J[q];
-                                pr[21 + p] += J[p] * rk;
-                                u[p] += J[p] * ja;
-                            }
-                            cc += ja * ja;
-                            gg += ja * rk;
-                        }
-                    }
-                    if (seg > 0) {
-                        // a cell written twice is two measurements of one landmark in one
-                        // slot (the reference counts both): add, do not overwrite.  The
-                        // slot starts at zero, so one or two contributions give the
-                        // same sum in either order.
-#pragma unroll
-                        for (int q = 0; q < 6; ++q) atomicAdd(&U[j * (6 * KT + 1) + 6 * (seg - 1) + q], u[q]);
-                    }
-                    ccp[pos] = cc;
-                    ggp[pos] = gg;
+                    if (mi.t >= a) measure(mi, pr, std::false_type{}, seg);
+                    else before = true;
+                }
+            }
+            if (seg == 0 && __builtin_amdgcn_ballot_w64(before) != 0) {
+                for (int o = sb + lane; o - lane < se; o += 64) {
+                    const bool valid = o < se;
+                    const MeasIn mi = meas_load(lay, ob0, o, valid);
+                    if (valid && mi.t < a) measure(mi, pr, std::true_type{}, seg);
                 }
             }
             if (seg > 0) {
@@ -341,7 +365,6 @@ __global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 2
         lds_barrier();
 
         // ---- phase 2: per landmark C, g_alpha and its B column v_l ---------------
-        for (int i = tid; i < SH::VROWS * VS; i += WG) Vsh[i] = (T)0;   // scratch -> V tile
         if (tid < nl) {
             // sums over the landmark's measurements, starting at a landmark-dependent
             // position so the lanes of a wavefront read different banks
@@ -362,6 +385,8 @@ __global__ __launch_bounds__(WG, (KT <= 8 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 2
             iscl[tid] = isc;
             gl_grad = G * isc;
         }
+        lds_barrier();   // the sums are read before the V tile overwrites them
+        for (int i = tid; i < SH::VROWS * VS; i += WG) Vsh[i] = (T)0;
         lds_barrier();
         if (tid < nl) Vsh[K6 * VS + tid] = gl_grad;   // the gradient row: y comes out of the same products
         // v_l(k) = sum of u over the slots s > k: a suffix scan per (landmark, component)
@@ -501,6 +526,28 @@ void launch_assemble(const AsmArgs& a, const StateBuf<T>* bufs, hipStream_t s) {
         if (a.lay.multi_chunk) hipLaunchKernelGGL((k_assemble<T, KMAX, true>), grid, block, 0, s, a, bufs[0], bufs[1]);
         else hipLaunchKernelGGL((k_assemble<T, KMAX, false>), grid, block, 0, s, a, bufs[0], bufs[1]);
     }
+}
+
+// Workgroups of the assembly kernel for groups of span kmax resident on the
+// current device at once (per-CU occupancy x CUs); 0 if the device cannot say.
+int assemble_resident_blocks(int kmax, int fp32, int multi_chunk) {
+    const void* f;
+    if (fp32) {
+        if (kmax <= 8) f = multi_chunk ? (const void*)k_assemble<float, 8, true> : (const void*)k_assemble<float, 8, false>;
+        else if (kmax <= 12)
+            f = multi_chunk ? (const void*)k_assemble<float, 12, true> : (const void*)k_assemble<float, 12, false>;
+        else f = multi_chunk ? (const void*)k_assemble<float, KMAX, true> : (const void*)k_assemble<float, KMAX, false>;
+    } else {
+        if (kmax <= 8) f = multi_chunk ? (const void*)k_assemble<double, 8, true> : (const void*)k_assemble<double, 8, false>;
+        else if (kmax <= 12)
+            f = multi_chunk ? (const void*)k_assemble<double, 12, true> : (const void*)k_assemble<double, 12, false>;
+        else f = multi_chunk ? (const void*)k_assemble<double, KMAX, true> : (const void*)k_assemble<double, KMAX, false>;
+    }
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, WG, 0) != hipSuccess) return 0;
+    return per_cu * cus;
 }
 
 template void launch_assemble<double>(const AsmArgs&, const StateBuf<double>*, hipStream_t);

@@ -20,6 +20,7 @@ struct AsmArgs {
 
 template <typename T>
 void launch_assemble(const AsmArgs& a, const StateBuf<T>* bufs, hipStream_t s);
+int assemble_resident_blocks(int kmax, int fp32, int multi_chunk);
 
 // the linearisation in frame coordinates (layouts with lay.frame_mode set)
 template <typename T>

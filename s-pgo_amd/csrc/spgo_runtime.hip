@@ -108,8 +108,43 @@ void destroy_graph(spgo_solver* h) {
     h->graph_dirty = true;
 }
 
+// Chunks per group when the options leave the choice to the library
+// (group_chunks 0).  The assembly runs one workgroup per group, so a launch
+// takes ceil(groups / resident workgroups) rounds, each as long as its largest
+// group: about (chunks + 0.4) chunk times, the 0.4 for the group's set-up and
+// write-back.  The choice minimises that product over 1..8 chunks per group,
+// from the one-chunk layout's chunk count per anchor.
+static int auto_group_chunks(const spgo_solver* h, const HostLayout& one) {
+    const int F = (int)one.anchor_grp.size() - 1;
+    int best = 1;
+    double best_t = 0.0;
+    for (int gc = 1; gc <= 8; ++gc) {
+        long long groups = 0;
+        int largest = 0;
+        for (int a = 0; a < F; ++a) {
+            const int n = one.anchor_grp[a + 1] - one.anchor_grp[a];
+            groups += (n + gc - 1) / gc;
+            largest = std::max(largest, std::min(n, gc));
+        }
+        const int slots = assemble_resident_blocks(one.kmax, h->opt.precision == SPGO_PREC_MIXED, gc > 1 && largest > 1);
+        if (slots <= 0) return 1;   // no device to ask: one chunk per group
+        const double t = (double)((groups + slots - 1) / slots) * (largest + 0.4);
+        if (gc == 1 || t < best_t) {
+            best = gc;
+            best_t = t;
+        }
+        if (largest < gc) break;   // larger groups change nothing
+    }
+    return best;
+}
+
 int build_layout(spgo_solver* h, HostLayout& out) {
-    return build_layout(h, h->opt.chunk_landmarks, h->opt.group_chunks, out);
+    if (h->opt.group_chunks > 0) return build_layout(h, h->opt.chunk_landmarks, h->opt.group_chunks, out);
+    int rc = build_layout(h, h->opt.chunk_landmarks, 1, out);
+    if (rc != SPGO_OK) return rc;
+    const int gc = auto_group_chunks(h, out);
+    if (gc == 1) return SPGO_OK;
+    return build_layout(h, h->opt.chunk_landmarks, gc, out);
 }
 
 // Build the device layout from the measurement list.
@@ -871,7 +906,7 @@ int spgo_default_options(spgo_options* o) {
     o->alpha_max = 1.0 / 0.2;
     o->use_graph = 1;
     o->chunk_landmarks = CHUNK_LM;
-    o->group_chunks = 1;
+    o->group_chunks = 0;   // chosen from the device's resident workgroups
     o->pcg_max_iter = 200;
     o->pcg_tol = 1e-8;
     return SPGO_OK;
