@@ -298,3 +298,38 @@ def test_mixed_precision_reaches_fp64_final_cost():
         print(f"[mixed] {prec}: c0={c0:.10g} final={finals[prec]:.12g}", flush=True)
         assert recs[-1]["cost"] < 0.1 * c0
     assert abs(finals["mixed"] - finals["fp64"]) <= 1e-6 * finals["fp64"]
+
+
+def test_mixed_reduced_system_tracks_fp64_along_the_trajectory():
+    # The mixed path is compared where its iterates cannot drift apart: at the
+    # states of an FP64 run, its reduced system, gradient and cost against the
+    # FP64 path's, and the damped step each system gives.  Bars (FP32
+    # linearisation, FP64 sums of the blocks; about ten times the errors seen):
+    # cost 1e-7 relative; H 2e-6 of its largest entry; g 1e-5 of the largest
+    # entry of the first state's gradient
+    # (near convergence g is a small difference of sums whose FP32 rounding does
+    # not shrink with it); the step 5e-5 of the first state's step in norm (the
+    # same reason).
+    p = make_problem(48, 6000, 10, seed=78, outlier_frac=0.05)
+    o = dict(mode="lm", robust="huber", robust_param=2.0)
+    s64 = solver_from_problem(p, make_options(precision="fp64", **o))
+    s32 = solver_from_problem(p, make_options(precision="mixed", **o))
+    lam = 1e-3
+    gscale = dscale = None
+    for it in range(4):
+        X, al = s64.get_state()
+        s32.set_state(X, al)
+        H64, g64, c64 = s64.linearize(lam)
+        H32, g32, c32 = s32.linearize(lam)
+        eH = np.max(np.abs(H32 - H64)) / np.max(np.abs(H64))
+        gscale = np.max(np.abs(g64)) if gscale is None else gscale
+        eg = np.max(np.abs(g32 - g64)) / gscale
+        d64 = np.linalg.solve(H64, g64)
+        d32 = np.linalg.solve(H32, g32)
+        dscale = np.linalg.norm(d64) if dscale is None else dscale
+        ed = np.linalg.norm(d32 - d64) / dscale
+        print(f"[mixed-track {it}] cost {abs(c32 - c64) / c64:.3g} H {eH:.3g} g {eg:.3g} step {ed:.3g}", flush=True)
+        assert abs(c32 - c64) <= 1e-7 * c64
+        assert eH <= 2e-6 and eg <= 1e-5
+        assert ed <= 5e-5
+        s64.step(3)
