@@ -1162,85 +1162,242 @@ size_t chol_panel_doubles(int Pp) { return (size_t)(Pp / EB) * EDMAX * EB * EB; 
 int chol_band_block() { return EB; }
 
 // ---------------------------------------------------------------------------
-// L L^T x = g, one workgroup, right-hand side in LDS
+// L L^T x = g after the dense blocked factorisation (L in H's lower triangle,
+// the inverse diagonal blocks in Linv): one workgroup per 32-row block row,
+// pipelined across the chip through per-row ready flags.
 // ---------------------------------------------------------------------------
-// Both triangular solves in one workgroup, within the block profile: forward,
-// block row k's solution y_k = L_kk^-1 r_k updates the active block rows below
-// it (32 lanes per row read the row's 32 entries of column block k together);
-// backward, x_k = L_kk^-T y_k updates the columns of block row k's profile.
-__global__ __launch_bounds__(1024) void k_chol_solve(const double* H, const double* Linv, const double* g,
-                                                     double* x, int Pp, const int* first, const int* act_off,
-                                                     const int* act, const Control* ctl) {
-    extern __shared__ double rhs[];
-    __shared__ double y[NB];
+// Forward, L y = g: block row i needs y_j for the block columns first[i] ..
+// i - 1 of its profile.  Backward, L^T x = y: block column i needs x_j for the
+// block rows j > i whose profile reaches it (the column's active list).  A
+// workgroup takes its block row from a ticket counter, so it only waits on
+// rows whose workgroups started before it: the pipeline cannot deadlock,
+// whatever the number of resident workgroups.  The far dependencies (ready
+// first) are summed while the near ones finish; the factor blocks of the eight
+// nearest are loaded into registers before any wait, and those eight are taken
+// together once all their flags are up.  A row publishes its solution, then
+// its flag; a reader waits for the flags (one thread per flag), meets the
+// others at a barrier, and then reads the solutions (the exchange is described
+// at xget).  The waits are bounded: past the bound the row is marked failed
+// (the Cholesky flag) and taken as ready, so every workgroup exits.
+// sync: [0] forward ticket, [1] backward ticket, [2, 2 + nb) forward flags,
+// [2 + nb, 2 + 2 nb) backward flags; cleared by k_solve_reset before each use.
+constexpr int SOLVE_SPIN_MAX = 1 << 20;
+constexpr int SOLVE_NEAR = 8;    // nearest dependencies, preloaded and taken together
+constexpr int SOLVE_FAR = 64;    // far dependencies whose flags are checked together
+constexpr int SOLVE_GROUP = 4;   // far dependency blocks read together
+
+// Everything one workgroup hands to another (the rows' solutions and flags)
+// moves by relaxed atomic loads and stores at device scope.  Those are
+// coherent across the XCDs' L2 caches by themselves, so no acquire or release
+// (which would write back or invalidate a whole L2) is needed: a row waits
+// until its own solution stores have completed before it raises its flag, and
+// a reader loads the solution after it has seen the flag.
+__device__ __forceinline__ double xget(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xput(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Once the Cholesky flag is down (a failed pivot, or a wait past its bound)
+// the solution is discarded, and no row waits any longer.
+__device__ __forceinline__ void solve_wait(const int* flag, Control* ctl) {
+    volatile int* ok = &ctl->chol_ok;
+    int n = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        if (*ok == 0) break;
+        if (++n > SOLVE_SPIN_MAX) {
+            *ok = 0;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// wait for the flags of dependencies [0, n) (n <= 64), one thread each, then
+// let the whole workgroup read what they published
+template <typename F>
+__device__ __forceinline__ void solve_wait_set(F flag_of, int n, int tid, Control* ctl) {
+    if (tid < n) solve_wait(flag_of(tid), ctl);
+    __syncthreads();
+}
+
+// wavefront 0 publishes the row it has stored: once all of its stores have
+// completed (vmcnt 0 counts the whole wavefront's), lane 0 raises the flag
+__device__ __forceinline__ void solve_publish(int* flag, int tid) {
+    if (tid < 64) {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (tid == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_solve_reset(int* sync, int n, const Control* ctl) {
+    if (ctl->pcg_fail == 0) return;
+    for (int i = threadIdx.x; i < n; i += 64) sync[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_solve_fwd(const double* H, const double* Linv, const double* g, double* y,
+                                                   int Pp, const int* first, int* sync, Control* ctl) {
+    __shared__ int srow;
+    __shared__ double rr[NB];
     if (ctl->pcg_fail == 0) return;   // the CG solve of this iteration converged
     const int tid = threadIdx.x;
-    const int nblk = Pp / NB;
-    const int grp = tid >> 5, m = tid & 31;   // 32 groups of 32 lanes
-    for (int i = tid; i < Pp; i += 1024) rhs[i] = g[i];
+    int* flag = sync + 2;
+    if (tid == 0) srow = __hip_atomic_fetch_add(sync, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    for (int k = 0; k < nblk; ++k) {
-        const double* Li = Linv + (long long)k * NB * NB;
-        if (tid < NB) {
-            double s = 0.0;
-            for (int q = 0; q <= tid; ++q) s += Li[tid * NB + q] * rhs[k * NB + q];
-            y[tid] = s;
-        }
-        __syncthreads();
-        if (tid < NB) rhs[k * NB + tid] = y[tid];
-        const int a0 = act_off[k], nr = (act_off[k + 1] - a0) * NB;
-        const double ym = y[m];
-        // four rows per group at a time, their loads issued together (rows past
-        // the end repeat the last one and are not written)
-        for (int rr0 = 4 * grp; rr0 < nr; rr0 += 128) {
-            double v[4];
-            int ii[4];
+    const int i = srow;
+    const int row = tid >> 3, c4 = (tid & 7) * 4;   // row of the block, four of its columns
+    const double* Hr = H + (long long)(i * NB + row) * Pp + c4;
+    const int jlo = first[i];
+    const int nnear = min(SOLVE_NEAR, i - jlo), jn = i - nnear;   // near dependencies: jn .. i - 1
+    double ln[SOLVE_NEAR][4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int rr = min(rr0 + u, nr - 1);
-                ii[u] = act[a0 + rr / NB] * NB + rr % NB;
-                v[u] = H[(long long)ii[u] * Pp + k * NB + m];
-            }
+    for (int u = 0; u < SOLVE_NEAR; ++u) {
+        const int j = min(jn + u, max(i - 1, 0));
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                double s = v[u] * ym;
-#pragma unroll
-                for (int off = 16; off > 0; off >>= 1) s += __shfl_xor(s, off, 32);
-                if (m == 0 && rr0 + u < nr) rhs[ii[u]] -= s;
-            }
-        }
-        __syncthreads();
+        for (int q = 0; q < 4; ++q) ln[u][q] = Hr[j * NB + q];
     }
-    for (int k = nblk - 1; k >= 0; --k) {
-        const double* Li = Linv + (long long)k * NB * NB;
-        if (tid < NB) {
-            double s = 0.0;
-            for (int q = tid; q < NB; ++q) s += Li[q * NB + tid] * rhs[k * NB + q];
-            y[tid] = s;
-        }
-        __syncthreads();
-        if (tid < NB) rhs[k * NB + tid] = y[tid];
-        for (int j = first[k] * NB + tid; j < k * NB; j += 1024) {
-            double hv[NB];
+    double acc = 0.0;
+    for (int j0 = jlo; j0 < jn; j0 += SOLVE_FAR) {
+        const int j1 = min(j0 + SOLVE_FAR, jn);
+        solve_wait_set([&](int t) { return flag + j0 + t; }, j1 - j0, tid, ctl);
+        for (int b0 = j0; b0 < j1; b0 += SOLVE_GROUP) {
+            double lv[SOLVE_GROUP][4], yv[SOLVE_GROUP][4];
 #pragma unroll
-            for (int q = 0; q < NB; ++q) hv[q] = H[(long long)(k * NB + q) * Pp + j];
-            double s0 = 0.0, s1 = 0.0;
+            for (int u = 0; u < SOLVE_GROUP; ++u) {
+                const int j = min(b0 + u, j1 - 1);
 #pragma unroll
-            for (int q = 0; q < NB; q += 2) {
-                s0 += hv[q] * y[q];
-                s1 += hv[q + 1] * y[q + 1];
+                for (int q = 0; q < 4; ++q) {
+                    lv[u][q] = Hr[j * NB + q];
+                    yv[u][q] = xget(y + j * NB + c4 + q);
+                }
             }
-            rhs[j] -= s0 + s1;
+#pragma unroll
+            for (int u = 0; u < SOLVE_GROUP; ++u)
+                if (b0 + u < j1)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc = fma(lv[u][q], yv[u][q], acc);
         }
-        __syncthreads();
     }
-    for (int i = tid; i < Pp; i += 1024) x[i] = rhs[i];
+    solve_wait_set([&](int t) { return flag + jn + t; }, nnear, tid, ctl);
+    {
+        double yv[SOLVE_NEAR][4];
+#pragma unroll
+        for (int u = 0; u < SOLVE_NEAR; ++u) {
+            const int j = min(jn + u, max(i - 1, 0));
+#pragma unroll
+            for (int q = 0; q < 4; ++q) yv[u][q] = xget(y + j * NB + c4 + q);
+        }
+#pragma unroll
+        for (int u = 0; u < SOLVE_NEAR; ++u)
+            if (u < nnear)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc = fma(ln[u][q], yv[u][q], acc);
+    }
+    // the eight lanes of a row are consecutive
+    acc += __shfl_xor(acc, 1, 8);
+    acc += __shfl_xor(acc, 2, 8);
+    acc += __shfl_xor(acc, 4, 8);
+    if ((tid & 7) == 0) rr[row] = g[i * NB + row] - acc;
+    __syncthreads();
+    if (tid < NB) {
+        const double* Li = Linv + (long long)i * NB * NB + tid * NB;
+        double v = 0.0;
+        for (int q = 0; q <= tid; ++q) v = fma(Li[q], rr[q], v);
+        xput(y + i * NB + tid, v);
+    }
+    solve_publish(flag + i, tid);
+}
+
+__global__ __launch_bounds__(256) void k_solve_bwd(const double* H, const double* Linv, const double* y, double* x,
+                                                   int Pp, const int* act_off, const int* act, int* sync,
+                                                   Control* ctl) {
+    __shared__ int srow;
+    __shared__ double part[8][NB];
+    __shared__ double rr[NB];
+    if (ctl->pcg_fail == 0) return;   // the CG solve of this iteration converged
+    const int tid = threadIdx.x, nb = Pp / NB;
+    int* flag = sync + 2 + nb;
+    if (tid == 0) srow = __hip_atomic_fetch_add(sync + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int i = nb - 1 - srow;
+    const int c = tid & 31, q4 = (tid >> 5) * 4;   // column of the block, four of the dependency's rows
+    const double* Hc = H + (long long)q4 * Pp + i * NB + c;
+    // the column's active rows ascend: the first ones are the nearest
+    const int a0 = act_off[i], a1 = act_off[i + 1];
+    const int nnear = min(SOLVE_NEAR, a1 - a0), an = a0 + nnear;   // near dependencies: act[a0 .. an)
+    double ln[SOLVE_NEAR][4];
+#pragma unroll
+    for (int u = 0; u < SOLVE_NEAR; ++u) {
+        const int j = a1 > a0 ? act[min(a0 + u, a1 - 1)] : i;   // the last column has no rows below
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ln[u][q] = Hc[(long long)(j * NB + q) * Pp];
+    }
+    double acc = 0.0;
+    // the far rows, the farthest (ready first) first
+    for (int e1 = a1; e1 > an; e1 -= SOLVE_FAR) {
+        const int e0 = max(e1 - SOLVE_FAR, an);
+        solve_wait_set([&](int t) { return flag + act[e0 + t]; }, e1 - e0, tid, ctl);
+        for (int b1 = e1; b1 > e0; b1 -= SOLVE_GROUP) {
+            double lv[SOLVE_GROUP][4], xv[SOLVE_GROUP][4];
+#pragma unroll
+            for (int u = 0; u < SOLVE_GROUP; ++u) {
+                const int j = act[max(b1 - 1 - u, e0)];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    lv[u][q] = Hc[(long long)(j * NB + q) * Pp];
+                    xv[u][q] = xget(x + j * NB + q4 + q);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SOLVE_GROUP; ++u)
+                if (b1 - 1 - u >= e0)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc = fma(lv[u][q], xv[u][q], acc);
+        }
+    }
+    solve_wait_set([&](int t) { return flag + act[a0 + t]; }, nnear, tid, ctl);
+    {
+        double xv[SOLVE_NEAR][4];
+#pragma unroll
+        for (int u = 0; u < SOLVE_NEAR; ++u) {
+            const int j = a1 > a0 ? act[min(a0 + u, a1 - 1)] : i;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) xv[u][q] = xget(x + j * NB + q4 + q);
+        }
+#pragma unroll
+        for (int u = 0; u < SOLVE_NEAR; ++u)
+            if (u < nnear)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc = fma(ln[u][q], xv[u][q], acc);
+    }
+    part[tid >> 5][c] = acc;
+    __syncthreads();
+    if (tid < NB) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += part[k][tid];
+        rr[tid] = y[i * NB + tid] - s;
+    }
+    __syncthreads();
+    if (tid < NB) {
+        const double* Li = Linv + (long long)i * NB * NB + tid;
+        double v = 0.0;
+        for (int q = tid; q < NB; ++q) v = fma(Li[q * NB], rr[q], v);
+        xput(x + i * NB + tid, v);
+    }
+    solve_publish(flag + i, tid);
 }
 
 void launch_chol_solve(const double* H, const double* Linv, const double* g, double* delta, int Pp,
-                       const int* first, const int* act_off, const int* act, const Control* ctl, hipStream_t s) {
-    hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), Pp * sizeof(double), s, H, Linv, g, delta, Pp, first,
-                       act_off, act, ctl);
+                       const int* first, const int* act_off, const int* act, double* ywork, int* sync,
+                       Control* ctl, hipStream_t s) {
+    const int nb = Pp / NB;
+    hipLaunchKernelGGL(k_solve_reset, dim3(1), dim3(64), 0, s, sync, 2 + 2 * nb, ctl);
+    hipLaunchKernelGGL(k_solve_fwd, dim3(nb), dim3(256), 0, s, H, Linv, g, ywork, Pp, first, sync, ctl);
+    hipLaunchKernelGGL(k_solve_bwd, dim3(nb), dim3(256), 0, s, H, Linv, ywork, delta, Pp, act_off, act, sync, ctl);
 }
 
 }  // namespace spgo

@@ -120,6 +120,8 @@ struct spgo_solver {
     int s0 = 0, s1 = 0;         // split: separator block rows [s0, s1)
     const int* firstB = nullptr;   // split: profile of the reversed bottom part
     double *Ssep = nullptr, *Smrg = nullptr, *winvS = nullptr, *LpS = nullptr;   // split: separator scratch
+    double* solve_y = nullptr;  // dense path: forward solution of the pipelined solves (Pp)
+    int* solve_sync = nullptr;  // dense path: their tickets and ready flags (2 + 2 Pp / NB)
     double* Lpan = nullptr;     // envelope: factored panels for the backward solve
     // dense path: the factor's block profile (NB-wide blocks) and per step the
     // block rows it reaches (nact on the host for the launch sizes)

@@ -56,7 +56,8 @@ int prepare_chol_band(int Dmax, size_t lds);
 int chol_band_block();
 size_t chol_band_lds(int Pp, int D);
 void launch_chol_solve(const double* H, const double* Linv, const double* g, double* delta, int Pp,
-                       const int* first, const int* act_off, const int* act, const Control* ctl, hipStream_t s);
+                       const int* first, const int* act_off, const int* act, double* ywork, int* sync, Control* ctl,
+                       hipStream_t s);
 
 template <typename T>
 void launch_pose_update(const Layout& lay, const Camera& cam, StateBuf<T>* bufs, const Control* ctl,

@@ -548,6 +548,10 @@ int setup_layout(spgo_solver* h) {
             if ((rc = upload(h, f32, &h->dense_first)) || (rc = upload(h, aoff, &h->dense_act_off)) ||
                 (rc = upload(h, act, &h->dense_act)))
                 return rc;
+            // the pipelined solves' forward solution, and their tickets and ready flags
+            h->solve_y = h->mem.alloc<double>(h->Pp);
+            h->solve_sync = h->mem.alloc<int>(2 + 2 * (h->Pp / NB));
+            if (!h->solve_y || !h->solve_sync) return h->fail(SPGO_E_DEVICE, "device allocation failed");
         }
         h->band_depth = bw;
         if (h->path == SPGO_FACTOR_ENVELOPE && prepare_chol_band(bw, chol_band_lds(h->Pp, bw)) != 0)
@@ -679,7 +683,7 @@ int launch_iteration(spgo_solver* h, StateBuf<T>* b) {
         } else {
             launch_cholesky(h->H, h->Linv, h->Pp, h->dense_act_off, h->dense_act, h->dense_nact.data(), h->ctl, s);
             launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, h->dense_first, h->dense_act_off,
-                              h->dense_act, h->ctl, s);
+                              h->dense_act, h->solve_y, h->solve_sync, h->ctl, s);
         }
     }
     launch_pose_update<T>(h->lay, make_camera(h), b, h->ctl, h->delta, (double)h->step_size, 0, s);
@@ -1352,7 +1356,7 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
             launch_cholesky(h->H, h->Linv, h->Pp, h->dense_act_off, h->dense_act, h->dense_nact.data(), h->ctl, s);
             (void)hipEventRecord(ev[2], s);
             launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, h->dense_first, h->dense_act_off,
-                              h->dense_act, h->ctl, s);
+                              h->dense_act, h->solve_y, h->solve_sync, h->ctl, s);
         }
         (void)hipEventRecord(ev[3], s);
         if (mixed) launch_pose_update<float>(h->lay, make_camera(h), h->bf, h->ctl, h->delta, h->step_size, 0, s);
