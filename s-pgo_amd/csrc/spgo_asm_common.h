@@ -130,7 +130,9 @@ __device__ __forceinline__ double rcp_refined(double x) {
     e = fma(-x, y, 1.0);
     return fma(y, e, y);
 }
-__device__ __forceinline__ float rcp_refined(float x) { return 1.0f / x; }
+// FP32: the hardware reciprocal and reciprocal square root (1 ulp), where an
+// IEEE division takes about ten instructions
+__device__ __forceinline__ float rcp_refined(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // 1/sqrt(x): the hardware estimate refined by one Halley step (FP64)
 __device__ __forceinline__ double rsqrt_refined(double x) {
@@ -138,7 +140,13 @@ __device__ __forceinline__ double rsqrt_refined(double x) {
     const double e = fma(-x * y, y, 1.0);
     return fma(y * e, fma(0.375, e, 0.5), y);
 }
-__device__ __forceinline__ float rsqrt_refined(float x) { return 1.0f / sqrtf(x); }
+__device__ __forceinline__ float rsqrt_refined(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// square root and quotient: the hardware's 1-ulp forms in FP32, IEEE in FP64
+__device__ __forceinline__ float root(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double root(double x) { return sqrt(x); }
+__device__ __forceinline__ float quot(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double quot(double a, double b) { return a / b; }
 
 // Robust weight w = sqrt(kappa * rho'(s)) and cost 1/2 kappa rho(s) of the
 // squared residual norm s.  inv_param = 1 / param.  Cauchy:
@@ -155,19 +163,19 @@ __device__ __forceinline__ void robust_eval(const Robust& rb, T inv_param, T s, 
         return;
     } else if (rb.kind == 1) {
         const T d = (T)rb.param;
-        const T rs = sqrt(s);
+        const T rs = root(s);
         if (rs <= d) {
             rho = s;
             drho = (T)1;
         } else {
             rho = (T)2 * d * rs - d * d;
-            drho = d / rs;
+            drho = quot(d, rs);
         }
     } else {
         rho = s;
         drho = (T)1;
     }
-    w = sqrt(kappa * drho);
+    w = root(kappa * drho);
     cost += 0.5 * (double)kappa * (double)rho;
 }
 
