@@ -25,6 +25,18 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+
+// 1/sqrt(x) in double: the hardware estimate refined by one Halley step
+// (cubic convergence), a few FMAs instead of the library's long sequence.
+__device__ __forceinline__ double rsqrt_fast(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(0.375, e, 0.5), y);
+}
+
+// keep the compiler from reordering the LDS accesses of one wavefront (the
+// hardware executes a wavefront's LDS operations in order)
+__device__ __forceinline__ void wave_lds_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -228,60 +240,133 @@ template void launch_form_system<float>(const Layout&, const StateBuf<float>*, C
 // ---------------------------------------------------------------------------
 // Blocks of step k: block 0 factors the diagonal block (and inverts it), block
 // b > 0 the b-th active block row act[act_off[k] + b - 1] (the block rows below
-// k whose profile reaches column block k; the others are zero there).
+// k whose profile reaches column block k; the others are zero there).  Every
+// workgroup factors the diagonal block itself (one wavefront), so the panel
+// blocks need no second launch; its other wavefronts load the panel block
+// meanwhile.
+//   factor: one row per lane, right-looking over the 32 columns.  Column j's
+//     entry of row j + 1 reaches the lanes by a lane read; the rest of the
+//     column goes through LDS, read back as broadcasts and applied at the start
+//     of column j + 1.  Pivot j + 1 is t - (u_(j+1) / L_jj)^2, with t formed from
+//     the diagonal lane's own registers, and 1/L_jj comes from the refined rsqrt.
+//   inverse W = L^-1 by 16x16 blocks: both diagonal blocks by forward
+//     substitution at once (lanes 0-15 and 16-31, one column of W each), then
+//     W21 = -W22 (L21 W11) as two MFMA products.
+//   panel: L_bk = A_bk W^T, one 16x16 MFMA tile per wavefront.
 __global__ __launch_bounds__(256) void k_chol_panel(double* H, double* Linv, int Pp, int k, const int* act_off,
                                                     const int* act, Control* ctl) {
-    __shared__ double Lkk[NB][NB + 1];
-    __shared__ double Li[NB][NB + 1];
-    __shared__ double Ab[NB][NB + 1];
-    if (ctl->pcg_fail == 0) return;   // the CG solve of this iteration converged
-    const int tid = threadIdx.x;
+    __shared__ double Lkk[NB][NB + 1];   // the factor, row-major, zeros above the diagonal
+    __shared__ double Li[NB][NB + 1];    // its inverse, row-major
+    __shared__ double Ab[NB][NB + 1];    // the panel block of this workgroup
+    __shared__ double colb[NB * NB];     // column broadcasts of the factor loop
+    __shared__ double Tm[16][17];        // L21 W11
+    __shared__ double dinv[NB];          // 1 / L_jj
+    if (ctl->pcg_fail == 0) return;      // the CG solve of this iteration converged
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int k0 = k * NB;
-    if (tid < 64) {
-        // one wavefront: lane i holds row i of the diagonal block
-        const int i = tid & (NB - 1);
+    const int b0 = blockIdx.x > 0 ? act[act_off[k] + blockIdx.x - 1] * NB : 0;
+    if (w == 0) {
+        const int r = lane & (NB - 1);   // lanes 32-63 repeat lanes 0-31
         double a[NB];
 #pragma unroll
-        for (int j = 0; j < NB; ++j) a[j] = H[(long long)(k0 + i) * Pp + k0 + j];
+        for (int j = 0; j < NB; ++j) a[j] = H[(long long)(k0 + r) * Pp + k0 + j];
+        double up[NB];
+        double scp = 0.0, uprev = 0.0, iv = 0.0;
+        double piv = readlane_d(a[0], 0);
         bool bad = false;
-#pragma clang loop unroll(full)
+#pragma unroll
         for (int j = 0; j < NB; ++j) {
-            double piv = readlane_d(a[j], j);
-            if (!(piv > 0.0) || !isfinite(piv)) {
-                bad = true;
-                piv = 1.0;
-            }
-            const double ljj = sqrt(piv);
-            const double inv = 1.0 / ljj;
-            a[j] = (i == j) ? ljj : a[j] * inv;
-#pragma clang loop unroll(full)
-            for (int q = j + 1; q < NB; ++q) {
-                const double lq = readlane_d(a[j], q);
-                a[q] -= a[j] * lq;
-            }
-        }
-        if (tid < NB) {
+            double t = 0.0;
+            if (j + 1 < NB) t = readlane_d(j >= 1 ? fma(-scp, uprev, a[j + 1]) : a[j + 1], j + 1);
+            if (j >= 1) {
 #pragma unroll
-            for (int j = 0; j < NB; ++j) Lkk[i][j] = (j <= i) ? a[j] : 0.0;
+                for (int q = j + 1; q < NB; ++q) {
+                    a[q] = fma(-scp, up[q], a[q]);
+                    asm volatile("" : "+v"(a[q]));
+                }
+            }
+            // u = A(:, j) after the updates of columns < j; for lane j, u * inv is
+            // L_jj and sc is 1, so the update only touches entries above the
+            // diagonal of rows <= j, which are not kept
+            const double u = a[j];
+            if (j + 2 < NB) {
+                if (lane < NB) colb[NB * j + lane] = u;
+                wave_lds_order();
+#pragma unroll
+                for (int q = j + 2; q < NB; ++q) up[q] = colb[NB * j + q];
+                wave_lds_order();
+            }
+            double uq = 0.0;
+            if (j + 1 < NB) uq = readlane_d(u, j + 1);
+            // a pivot outside (0, inf) fails the factorisation; 1 stands in for it
+            const bool ok = piv > 0.0 && piv < __builtin_inf();
+            bad = bad || !ok;
+            const double inv = rsqrt_fast(ok ? piv : 1.0);
+            const double l = u * inv;
+            const double sc = l * inv;
+            if (r == j) iv = inv;
+            a[j] = l;
+            if (j + 1 < NB) {
+                a[j + 1] = fma(-sc, uq, a[j + 1]);
+                const double x = uq * inv;
+                piv = fma(-x, x, t);
+            }
+            scp = sc;
+            uprev = u;
         }
-        if (bad && blockIdx.x == 0 && tid == 0) ctl->chol_ok = 0;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (tid < NB) {
-            // column tid of L^-1 by forward substitution
-            const int cidx = tid;
-            double x[NB];
-#pragma clang loop unroll(full)
-            for (int rr = 0; rr < NB; ++rr) {
-                double s = (rr == cidx) ? 1.0 : 0.0;
-#pragma clang loop unroll(full)
-                for (int m = 0; m < rr; ++m) s -= Lkk[rr][m] * x[m];
-                x[rr] = s / Lkk[rr][rr];
+        if (lane < NB) {
+#pragma unroll
+            for (int j = 0; j < NB; ++j) Lkk[r][j] = (j <= r) ? a[j] : 0.0;
+            dinv[r] = iv;
+        }
+        if (bad && blockIdx.x == 0 && lane == 0) ctl->chol_ok = 0;
+    } else if (blockIdx.x > 0) {
+        for (int e = tid - 64; e < NB * NB; e += 192) {
+            const int rr = e / NB, cc = e % NB;
+            Ab[rr][cc] = H[(long long)(b0 + rr) * Pp + k0 + cc];
+        }
+    }
+    __syncthreads();
+    if (w == 0) {
+        if (lane < NB) {
+            // column c of the inverse of diagonal block h (rows and columns 16h ..)
+            const int h = lane >> 4, c = lane & 15, o = 16 * h;
+            double x[16];
+#pragma unroll
+            for (int rr = 0; rr < 16; ++rr) {
+                double v = (rr == c) ? 1.0 : 0.0;
+#pragma unroll
+                for (int m = 0; m < rr; ++m) v = fma(-Lkk[o + rr][o + m], x[m], v);
+                x[rr] = v * dinv[o + rr];
             }
 #pragma unroll
-            for (int rr = 0; rr < NB; ++rr) Li[rr][cidx] = x[rr];
+            for (int rr = 0; rr < 16; ++rr) {
+                Li[o + rr][o + c] = x[rr];
+                Li[o + rr][16 - o + c] = 0.0;   // W12 = 0; W21 is overwritten below
+            }
         }
+    }
+    __syncthreads();
+    if (w == 0) {
+        // T = L21 W11: A(m, kk) = L(16 + m, kk), B(kk, n) = W(kk, n)
+        d4 acc = 0.0;
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Lkk[16 + (lane & 15)][4 * s4 + (lane >> 4)],
+                                                       Li[4 * s4 + (lane >> 4)][lane & 15], acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Tm[mfma16_row(0.0, lane, v)][lane & 15] = acc[v];
+    }
+    __syncthreads();
+    if (w == 0) {
+        // W21 = -W22 T
+        d4 acc = 0.0;
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Li[16 + (lane & 15)][16 + 4 * s4 + (lane >> 4)],
+                                                       Tm[4 * s4 + (lane >> 4)][lane & 15], acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Li[16 + mfma16_row(0.0, lane, v)][lane & 15] = -acc[v];
     }
     __syncthreads();
     if (blockIdx.x == 0) {
@@ -292,18 +377,18 @@ __global__ __launch_bounds__(256) void k_chol_panel(double* H, double* Linv, int
         }
         return;
     }
-    const int b0 = act[act_off[k] + blockIdx.x - 1] * NB;
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int rr = e / NB, cc = e % NB;
-        Ab[rr][cc] = H[(long long)(b0 + rr) * Pp + k0 + cc];
-    }
-    __syncthreads();
-    // L_bk = A_bk L_kk^-T
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int rr = e / NB, cc = e % NB;
-        double s = 0.0;
-        for (int m = 0; m <= cc; ++m) s += Ab[rr][m] * Li[cc][m];
-        H[(long long)(b0 + rr) * Pp + k0 + cc] = s;
+    // L_bk = A_bk W^T: wavefront w forms the 16x16 tile (w / 2, w % 2);
+    // A(m, kk) = A_bk(16 rt + m, kk), B(kk, n) = W(16 ct + n, kk)
+    {
+        const int rt = w >> 1, ct = w & 1;
+        d4 acc = 0.0;
+#pragma unroll
+        for (int s8 = 0; s8 < 8; ++s8)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ab[16 * rt + (lane & 15)][4 * s8 + (lane >> 4)],
+                                                       Li[16 * ct + (lane & 15)][4 * s8 + (lane >> 4)], acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            H[(long long)(b0 + 16 * rt + mfma16_row(0.0, lane, v)) * Pp + k0 + 16 * ct + (lane & 15)] = acc[v];
     }
 }
 
@@ -427,17 +512,7 @@ constexpr int EDMAX = 7;          // deepest profile: blocks below the diagonal
 constexpr int EST = EB + 1;       // LDS row stride of a window block
 constexpr int EBLK = EB * EST;    // doubles per window block
 
-// 1/sqrt(x) in double: the hardware estimate refined by one Halley step
-// (cubic convergence), a few FMAs instead of the library's long sequence.
-__device__ __forceinline__ double rsqrt_fast(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    const double e = fma(-x * y, y, 1.0);
-    return fma(y * e, fma(0.375, e, 0.5), y);
-}
 
-// keep the compiler from reordering the LDS accesses of one wavefront (the
-// hardware executes a wavefront's LDS operations in order)
-__device__ __forceinline__ void wave_lds_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
 
 // workgroup barrier that orders LDS only: it does not wait for outstanding
 // global loads or stores (s_waitcnt lgkmcnt(0), vmcnt/expcnt left at their maximum)
