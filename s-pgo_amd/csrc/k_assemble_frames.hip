@@ -35,9 +35,10 @@ namespace spgo {
 namespace {
 
 // LDS shapes for groups of at most KT target frames
-template <int KT>
+template <typename T, int KT>
 struct FrShape {
-    static constexpr int CL = KT <= 16 ? 64 : (KT <= 32 ? 32 : 8);   // landmarks per group
+    // landmarks per group: 32 in FP64 at 16 frames, so that two workgroups fit a CU
+    static constexpr int CL = KT <= 16 ? (sizeof(T) == 8 ? 32 : 64) : (KT <= 32 ? 32 : 8);
     static constexpr int VS = CL + 2;                                 // V tile row stride
     static constexpr int K6 = 6 * KT;                                  // block rows: the target frames
     static constexpr int VROWS = 16 * ((K6 + 1 + 15) / 16);            // and the gradient row, whole tiles
@@ -47,8 +48,8 @@ struct FrShape {
 }  // namespace
 
 template <typename T, int KT>
-__global__ __launch_bounds__(WG, (sizeof(T) == 4 && KT <= 16) ? 2 : 1) void k_assemble_frames(AsmArgs A, StateBuf<T> B0, StateBuf<T> B1) {
-    using SH = FrShape<KT>;
+__global__ __launch_bounds__(WG, KT <= 16 ? 2 : 1) void k_assemble_frames(AsmArgs A, StateBuf<T> B0, StateBuf<T> B1) {
+    using SH = FrShape<T, KT>;
     constexpr int CL = SH::CL, VS = SH::VS, US = SH::US;
     const Layout& lay = A.lay;
     const int g = blockIdx.x;
@@ -86,9 +87,13 @@ __global__ __launch_bounds__(WG, (sizeof(T) == 4 && KT <= 16) ? 2 : 1) void k_as
     __shared__ double AinvB[36];                // the same at the linearisation point (back-substitution)
     __shared__ double dd[KT + 1][6];            // z_a - z_{f_s} (back-substitution)
     __shared__ T U[CL * US];                    // per (landmark, slot >= 1): J_a^T j_alpha
-    __shared__ T ccp[CHUNK_OBS], ggp[CHUNK_OBS];
     __shared__ T iscl[CL];
-    __shared__ T Vsh[SH::VROWS * VS];           // B columns in / V tile
+    __shared__ T Vsh[SH::VROWS * VS];           // B columns in / per-measurement sums / V tile
+    // per measurement j_alpha^2 and j_alpha r, written in phase 1 and summed per
+    // landmark in phase 2 before the V tile takes the buffer over
+    static_assert(SH::VROWS * VS >= 2 * CHUNK_OBS, "the per-measurement sums fit the V buffer");
+    T* const ccp = Vsh;
+    T* const ggp = Vsh + CHUNK_OBS;
     __shared__ T lmP[CL][6];
     __shared__ int2 lpos[CL];
     __shared__ double red[NWAVE];
@@ -321,7 +326,6 @@ __global__ __launch_bounds__(WG, (sizeof(T) == 4 && KT <= 16) ? 2 : 1) void k_as
     lds_barrier();
 
     // ---- phase 2: C and g_alpha per landmark, the rows of V --------------------
-    for (int i = tid; i < SH::VROWS * VS; i += WG) Vsh[i] = (T)0;
     T gl_grad = (T)0;
     if (tid < nl) {
         const int2 lp = lpos[tid];
@@ -341,6 +345,8 @@ __global__ __launch_bounds__(WG, (sizeof(T) == 4 && KT <= 16) ? 2 : 1) void k_as
         iscl[tid] = isc;
         gl_grad = G * isc;
     }
+    lds_barrier();   // the sums are read before the V tile overwrites them
+    for (int i = tid; i < SH::VROWS * VS; i += WG) Vsh[i] = (T)0;
     lds_barrier();
     if (tid < nl) Vsh[K6 * VS + tid] = gl_grad;   // the gradient row
     // u in the anchor's axes, stored for the back-substitution, and the rows of
@@ -453,6 +459,8 @@ void launch_assemble_frames(const AsmArgs& a, const StateBuf<T>* bufs, hipStream
 template void launch_assemble_frames<double>(const AsmArgs&, const StateBuf<double>*, hipStream_t);
 template void launch_assemble_frames<float>(const AsmArgs&, const StateBuf<float>*, hipStream_t);
 
-int frames_landmarks_per_group(int kt) { return kt <= 16 ? 64 : (kt <= 32 ? 32 : 8); }
+int frames_landmarks_per_group(int kt, int fp64) {
+    return kt <= 16 ? (fp64 ? FrShape<double, 16>::CL : FrShape<float, 16>::CL) : (kt <= 32 ? 32 : 8);
+}
 
 }  // namespace spgo

@@ -66,7 +66,7 @@ int build_layout_frames(spgo_solver* h, HostLayout& out) {
         i = j;
     }
     const int kt = mmax <= 16 ? 16 : (mmax <= 32 ? 32 : KTF);
-    int cl = frames_landmarks_per_group(kt);
+    int cl = frames_landmarks_per_group(kt, h->opt.precision != SPGO_PREC_MIXED);
     if (h->opt.chunk_landmarks > 0) cl = std::min(cl, h->opt.chunk_landmarks);
     const int SO = kt + 2;
     hl.kt = kt;

@@ -26,7 +26,7 @@ int assemble_resident_blocks(int kmax, int fp32, int multi_chunk);
 template <typename T>
 void launch_assemble_frames(const AsmArgs& a, const StateBuf<T>* bufs, hipStream_t s);
 // landmarks per group of the frame-coordinate kernel for groups of kt target frames
-int frames_landmarks_per_group(int kt);
+int frames_landmarks_per_group(int kt, int fp64);
 
 template <typename T>
 void launch_form_system(const Layout& lay, const StateBuf<T>* bufs, Control* ctls, double* H, double* g,
