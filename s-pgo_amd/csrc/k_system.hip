@@ -128,6 +128,71 @@ __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0,
     }
 }
 
+// Entry (e, j) of one block of the frame-coordinate system: the sum over the
+// block's plan entries (ZCon), a block of a group's B (K6 > 0; a diagonal
+// target block is stored lower, doff >= 0, so its upper entries are read
+// transposed) or an anchor block R (K6 = -2: +R, K6 = -1: -R).  The plan
+// entries and values are loaded in batches of 8 before their adds, so a list
+// of hundreds of entries (a frame seen by many groups) costs a few dozen round
+// trips rather than two per entry; past the list the last entry is loaded
+// again and weighted by zero.  Each value enters as sign * value with the sign
+// (+1, -1 or 0) chosen by selects: a value kept or dropped by a condition
+// would let the compiler move its load under a branch and wait for each in
+// turn.
+__device__ __forceinline__ double frame_block_sum(const ZCon* __restrict__ con, int kb, int ke,
+                                                  const double* __restrict__ Dw, const double* __restrict__ Rz,
+                                                  int e, int j) {
+    double h0 = 0.0, h1 = 0.0;
+    for (int qb = kb; qb < ke; qb += 8) {
+        double v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            // the whole entry in one 16-byte load, the address by selects
+            const int4 z = reinterpret_cast<const int4*>(con)[min(qb + i, ke - 1)];
+            const long long off = (long long)(((unsigned long long)(unsigned)z.y << 32) | (unsigned)z.x);
+            const bool dblk = z.z > 0;
+            const bool tr = dblk && z.w >= 0 && j > e;
+            const int ld = dblk ? z.z : 6;
+            const long long idx = off + (long long)(tr ? j : e) * ld + (tr ? e : j);
+            const uintptr_t base = dblk ? (uintptr_t)Dw : (uintptr_t)Rz;
+            const double x = *reinterpret_cast<const __attribute__((address_space(1))) double*>(
+                base + 8 * (uintptr_t)idx);
+            const double w = qb + i < ke ? (z.z == -1 ? -1.0 : 1.0) : 0.0;
+            v[i] = w * x;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i += 2) {
+            h0 += v[i];
+            h1 += v[i + 1];
+        }
+    }
+    return h0 + h1;
+}
+
+// Element x of a frame's gradient: the gradient parts (doff) of its diagonal
+// block's plan entries, + for the anchor's own (K6 = -2), - for a target's,
+// batched as frame_block_sum.
+__device__ __forceinline__ double frame_grad_sum(const ZCon* __restrict__ con, int kb, int ke,
+                                                 const double* __restrict__ Rz, int x) {
+    double g0 = 0.0, g1 = 0.0;
+    for (int qb = kb; qb < ke; qb += 8) {
+        double v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int4 z = reinterpret_cast<const int4*>(con)[min(qb + i, ke - 1)];
+            const double y = Rz[z.w + x];
+            const double w = qb + i < ke ? (z.z == -2 ? 1.0 : -1.0) : 0.0;
+            v[i] = w * y;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i += 2) {
+            g0 += v[i];
+            g1 += v[i + 1];
+        }
+    }
+    return g0 + g1;
+}
+
 // The reduced system in frame coordinates (one problem): workgroup r writes the
 // six rows of frame r, every element left of the diagonal (zeros outside the
 // pattern), as the sum of the group contributions the layout's plan lists for
@@ -137,8 +202,8 @@ __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0,
 template <typename T>
 __global__ __launch_bounds__(256) void k_form_frames(Layout lay, StateBuf<T> B0, StateBuf<T> B1, Control* ctl,
                                                      double* H, double* gvec) {
-    constexpr int MAXC = 2048;
-    __shared__ int cols[MAXC];
+    constexpr int MAXN = 4096;   // frames up to which the row's block map is kept in LDS
+    __shared__ int blk[MAXN];    // blk[c]: the plan index of block (r, c) in this row, -1 if none
     __shared__ double F[2][36];
     const int r = blockIdx.x;
     const int tid = threadIdx.x;
@@ -151,9 +216,12 @@ __global__ __launch_bounds__(256) void k_form_frames(Layout lay, StateBuf<T> B0,
     const StateBuf<T> bs = ctl->cur ? B1 : B0;
     const double lam = ctl->lambda;
     const int e0 = lay.zp_row[r], ne = lay.zp_row[r + 1] - e0;
-    const bool staged = ne <= MAXC;
-    if (staged)
-        for (int i = tid; i < ne; i += 256) cols[i] = lay.zp_col[e0 + i];
+    const bool mapped = r < MAXN;   // the same for the whole workgroup
+    if (mapped) {
+        for (int i = tid; i <= r; i += 256) blk[i] = -1;
+        __syncthreads();
+        for (int i = tid; i < ne; i += 256) blk[lay.zp_col[e0 + i]] = i;
+    }
     if (tid < 72) {
         const int w = tid / 36, k = tid - 36 * w, f = r - w;
         double v = 0.0;
@@ -173,28 +241,23 @@ __global__ __launch_bounds__(256) void k_form_frames(Layout lay, StateBuf<T> B0,
         while (e < 5 && t >= (e + 1) * (6 * r + 1) + (e + 1) * e / 2) ++e;
         const int col = t - (e * (6 * r + 1) + e * (e - 1) / 2);
         const int c = col / 6, j = col - 6 * c;
-        int lo = 0, hi = ne;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            const int cm = staged ? cols[mid] : lay.zp_col[e0 + mid];
-            if (cm < c) lo = mid + 1;
-            else hi = mid;
+        int lo;
+        if (mapped) {
+            lo = blk[c];
+        } else {
+            int hi = ne;
+            lo = 0;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (lay.zp_col[e0 + mid] < c) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo >= ne || lay.zp_col[e0 + lo] != c) lo = -1;
         }
         double h = 0.0;
-        if (lo < ne && (staged ? cols[lo] : lay.zp_col[e0 + lo]) == c) {
+        if (lo >= 0) {
             const int k = e0 + lo;
-            const int kb = lay.zp_cbeg[k], ke = lay.zp_cbeg[k + 1];
-            for (int q = kb; q < ke; ++q) {
-                const ZCon z = lay.zp_con[q];
-                if (z.K6 > 0) {
-                    const long long idx = (z.doff >= 0 && j > e) ? z.off + (long long)j * z.K6 + e
-                                                                 : z.off + (long long)e * z.K6 + j;
-                    h += bs.Dw[idx];
-                } else {
-                    const double v = bs.Rz[z.off + 6 * e + j];
-                    h += z.K6 == -2 ? v : -v;
-                }
-            }
+            h = frame_block_sum(lay.zp_con, lay.zp_cbeg[k], lay.zp_cbeg[k + 1], bs.Dw, bs.Rz, e, j);
         }
         if (lam != 0.0) {
             if (c == r) h += lam * (F[0][6 * e + j] + F[1][6 * e + j]);
@@ -207,11 +270,7 @@ __global__ __launch_bounds__(256) void k_form_frames(Layout lay, StateBuf<T> B0,
         double gs = 0.0;
         if (ne > 0 && lay.zp_col[e0 + ne - 1] == r) {
             const int k = e0 + ne - 1;
-            for (int q = lay.zp_cbeg[k]; q < lay.zp_cbeg[k + 1]; ++q) {
-                const ZCon z = lay.zp_con[q];
-                if (z.K6 == -2) gs += bs.Rz[z.doff + tid];
-                else gs -= bs.Rz[z.doff + tid];
-            }
+            gs = frame_grad_sum(lay.zp_con, lay.zp_cbeg[k], lay.zp_cbeg[k + 1], bs.Rz, tid);
         }
         gvec[6 * r + tid] = gs;
     }

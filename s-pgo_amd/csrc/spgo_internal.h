@@ -47,6 +47,8 @@ struct ZCon {
     long long off;
     int K6, doff;
 };
+// k_form_frames reads an entry as one int4: off in .x/.y, K6 in .z, doff in .w
+static_assert(sizeof(ZCon) == 16 && offsetof(ZCon, K6) == 8 && offsetof(ZCon, doff) == 12, "ZCon layout");
 
 // One problem of a batch (a lone solver is a batch of one).  The per-problem
 // kernels run one workgroup per problem.
