@@ -49,7 +49,10 @@ extern "C" {
 /* Reduced-system factorisation.  ENVELOPE: the profile Cholesky in one
  * workgroup; ENVELOPE_SPLIT: the same, split at a separator in the middle of
  * the profile into a top and a bottom part factored concurrently; DENSE: the
- * multi-kernel blocked Cholesky.  AUTO picks ENVELOPE_SPLIT for long narrow
+ * multi-kernel blocked Cholesky; CR: block cyclic reduction for profiles at
+ * most 4 blocks deep (the band is block tridiagonal in super-rows of 16 D
+ * rows, eliminated every other one per level, log2 of their number levels:
+ * the Cholesky factorisation of H with its super-rows reordered).  AUTO picks ENVELOPE_SPLIT for long narrow
  * profiles (at least 2 D + 6 blocks of 16 rows, D the profile depth),
  * ENVELOPE for short narrow ones and DENSE for wide ones.  PCG: conjugate
  * gradients preconditioned by the network set with spgo_set_preconditioner
@@ -61,6 +64,7 @@ extern "C" {
 #define SPGO_FACTOR_ENVELOPE 2
 #define SPGO_FACTOR_ENVELOPE_SPLIT 3
 #define SPGO_FACTOR_PCG 4
+#define SPGO_FACTOR_CR 5
 
 /* Parameters of the reduced pose system.  INCREMENTAL: the increments of the
  * incremental poses X_k themselves (the reference's system, Solver_cuda.cu:
@@ -111,7 +115,7 @@ typedef struct spgo_iter_stats {
 typedef struct spgo_layout_info {
     int32_t n_poses, n_slots, n_measurements, n_landmarks;
     int32_t n_groups, n_chunks, max_span, n_anchors_used;
-    int32_t factorization;   /* SPGO_FACTOR_DENSE, _ENVELOPE, _ENVELOPE_SPLIT or _PCG: the path spgo_step takes */
+    int32_t factorization;   /* SPGO_FACTOR_DENSE, _ENVELOPE, _ENVELOPE_SPLIT, _CR or _PCG: the path spgo_step takes */
     int32_t profile_depth;   /* blocks of 16 below the diagonal in the reduced system's profile */
     int32_t reduced_size;    /* rows of the padded reduced pose system                          */
     int32_t coordinates;     /* SPGO_COORD_INCREMENTAL or _FRAMES: the system spgo_step solves */

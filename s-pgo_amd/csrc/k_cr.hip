@@ -1,0 +1,761 @@
+// Block cyclic reduction of the banded reduced pose system: the factorisation
+// and both solves of H delta = g (the reference's dense torch Cholesky and
+// cholesky_solve, linear_solvers.cu:195-218), for profiles at most four
+// 16-row blocks deep.
+//
+// A profile D blocks of 16 deep makes H block tridiagonal in super-rows of
+// S = 16 D rows: super-row I couples only to I - 1 and I + 1.  Level l
+// eliminates, all at once, the super-rows i = 2^l (mod 2^(l+1)) from the
+// system of the super-rows that are multiples of 2^l.  With the diagonal block
+// D_i = L L^T, W = L^-1, its neighbours p = i - 2^l and q = i + 2^l, E_i the
+// coupling of i to p and E_q that of q to i:
+//     U_i = W E_i,  V_i = W E_q^T,  y_i = W b_i
+//     D_p -= U_i^T U_i,  D_q -= V_i^T V_i,  b_p -= U_i^T y_i,  b_q -= V_i^T y_i
+//     new coupling of q to p:  -V_i^T U_i
+// This is the Cholesky factorisation of H with its super-rows reordered
+// (odd-even), so it needs H positive definite and nothing more; every pivot is
+// checked.  After the last level super-row 0 is alone; the solution then comes
+// back level by level, x_i = W^T (y_i - U_i x_p - V_i x_q).
+//
+// The sequential part is one S x S factorisation per level (log2 of the
+// number of super-rows levels) instead of a column chain over all of H.
+// Per level three launches: k_cr_factor (one workgroup per eliminated
+// super-row: D_i = L L^T and W = L^-1), k_cr_uv (U, V and y as 16 x 16 FP64
+// MFMA tiles), k_cr_update (the survivors' new D, coupling and b as MFMA
+// tiles); then one k_cr_factor for super-row 0 with its solution, and one
+// k_cr_back per level.  Level 0 reads H and g in place (the lower triangle);
+// later levels read the workspace.  Rows past the padded size are identity.
+//
+// k_cr_factor factors D right-looking over 16-wide block columns.  The
+// diagonal block is factored by one wavefront, one row per lane (the four
+// 16-lane rows of the wavefront repeat it): the column entries reach the
+// other lanes by DPP row broadcasts (row_newbcast), which ride on a register
+// move, so a column costs no LDS round trip or lane read.  The same loop
+// inverts the unit lower factor (forward substitution of the identity, rows
+// broadcast the same way); W_kk = diag(1/L_jj) (unit L)^-1.  The panel blocks
+// below are A_bk W_kk^T and the trailing update L_bk L_ck^T, both on MFMA;
+// W's blocks below its diagonal, W_bc = -W_bb sum_m L_bm W_mc, are formed on
+// MFMA by the other wavefronts while the next diagonal block is factored.
+#include <utility>
+
+#include "spgo_device.h"
+#include "spgo_kernels.h"
+
+namespace spgo {
+
+namespace {
+typedef double __attribute__((ext_vector_type(4))) d4;
+
+// lane J of each 16-lane row, to every lane of that row
+// (row_newbcast writes every lane, so the value the instruction would keep
+// in unwritten lanes is v itself and needs no register of its own)
+template <int J>
+__device__ __forceinline__ double row_bcast(double v) {
+    return __builtin_amdgcn_update_dpp(v, v, 0x150 + J, 0xf, 0xf, true);
+}
+
+// The diagonal-block factorisation is a fixed instruction schedule, one
+// inline-asm block per column (ALU only), generated at compile time.  A
+// column's pivot chain (broadcast, rsqrt, one Newton-Halley step, scaling) has
+// the previous column's updates of the inverse interleaved with it, and the
+// update that feeds the next pivot is issued first.  The hazards the compiler
+// cannot see inside the block are handled in it: a DPP read needs two wait
+// states after a VALU write of its source (each block starts with s_nop 1, as
+// the compiler may place a register copy right before it, and inside a block
+// no DPP source is written fewer than two instructions earlier), and a VALU
+// read of a transcendental (v_rsq_f64) result needs one.
+struct AsmText {
+    char buf[6144] = {};
+    unsigned long n = 0;
+    constexpr void put(const char* p) {
+        while (*p) buf[n++] = *p++;
+    }
+    constexpr void num(unsigned v) {
+        char t[12] = {};
+        int k = 0;
+        do {
+            t[k++] = char('0' + v % 10);
+            v /= 10;
+        } while (v);
+        while (k) buf[n++] = t[--k];
+    }
+    constexpr void hex(unsigned v) {
+        put("0x");
+        for (int sh = 28; sh >= 0; sh -= 4) buf[n++] = "0123456789abcdef"[(v >> sh) & 15];
+    }
+    constexpr void op(int i) {
+        buf[n++] = '%';
+        num((unsigned)i);
+    }
+    constexpr const char* data() const { return buf; }
+    constexpr unsigned long size() const { return n; }
+};
+
+// Operands of a column block: 0-15 the row (av), 16-19 this 16-lane row's four
+// columns of the inverse (yv), 20 m, 21 d, 22 y0, 23 t, 24 e, 25 inv, 26 mi,
+// 27 ivr; inputs 28 m of the previous column, 29 the constant 0.375, 30 the
+// lane's factor (r > J), 31 the lane's factor (r == J).
+enum { OP_Y = 16, OP_M = 20, OP_D, OP_Y0, OP_T, OP_E, OP_INV, OP_MI, OP_IVR, OP_MPREV, OP_C375, OP_GT, OP_EQ };
+
+constexpr void asm_op3(AsmText& a, const char* ins, int o0, const char* s1, int o1, int o2) {
+    a.put(ins);
+    a.op(o0);
+    a.put(s1);
+    a.op(o1);
+    a.put(", ");
+    a.op(o2);
+}
+// The work column J defers into its chain's gaps: the updates of column J - 1
+// that no pivot before column J + 1 needs, item k of the list
+//   k < 4:   yv[k] -= (yv[k] of lane J - 1) m_(J-1)        (the inverse)
+//   k >= 4:  av[q] -= L_q,J-1 L_r,J-1, q = J + k - 3        (the trailing update)
+constexpr int column_pool(int J) { return J >= 1 ? 4 + (15 - J) : 0; }
+constexpr void column_pool_item(AsmText& a, int J, int k) {
+    if (k < 4) {
+        asm_op3(a, "v_fmac_f64_dpp ", OP_Y + k, ", -", OP_Y + k, OP_MPREV);
+        a.put(" row_newbcast:");
+        a.num((unsigned)(J - 1));
+    } else {
+        const int q = J + k - 3;
+        asm_op3(a, "v_fmac_f64_dpp ", q, ", -", J - 1, J - 1);
+        a.put(" row_newbcast:");
+        a.num((unsigned)q);
+    }
+    a.put(" row_mask:0xf bank_mask:0xf\n");
+}
+// pool items [k0, k1) of column J
+constexpr void column_pool_range(AsmText& a, int J, int k0, int k1) {
+    const int n = column_pool(J);
+    for (int k = k0; k < k1 && k < n; ++k) column_pool_item(a, J, k);
+}
+
+// Column J: pivot d = a_JJ (lane J), y0 = rsqrt(d), one Halley step to inv,
+// L_rJ = a_rJ inv (into av[J]), m = L_rJ inv on lanes r > J (0 elsewhere),
+// ivr += inv on lane J, and the update of av[J + 1] (the next pivot's entry);
+// column J - 1's other updates fill the gaps of this chain.  VALU only: a
+// VALU result read by a scalar instruction (a pivot test) or a scalar write of
+// EXEC stalls the wavefront's issue.
+template <int J>
+constexpr AsmText column_asm() {
+    AsmText a;
+    a.put("s_nop 1\nv_mov_b64_dpp ");
+    a.op(OP_D);
+    a.put(", ");
+    a.op(J);
+    a.put(" row_newbcast:");
+    a.num(J);
+    a.put(" row_mask:0xf bank_mask:0xf\nv_rsq_f64 ");
+    a.op(OP_Y0);
+    a.put(", ");
+    a.op(OP_D);
+    a.put("\n");
+    // a VALU read of a transcendental result needs one wait state: the pool
+    // items give it, column 0 (no pool) an s_nop
+    if (column_pool(J) == 0) a.put("s_nop 0\n");
+    column_pool_range(a, J, 0, 4);
+    asm_op3(a, "v_mul_f64 ", OP_T, ", ", OP_D, OP_Y0);   // t = d y0
+    a.put("\n");
+    column_pool_range(a, J, 4, 6);
+    asm_op3(a, "v_fma_f64 ", OP_E, ", -", OP_T, OP_Y0);   // e = 1 - t y0
+    a.put(", 1.0\n");
+    column_pool_range(a, J, 6, 8);
+    asm_op3(a, "v_mul_f64 ", OP_T, ", ", OP_Y0, OP_E);   // t = y0 e
+    a.put("\n");
+    asm_op3(a, "v_fma_f64 ", OP_E, ", ", OP_E, OP_C375);   // e = 0.375 e + 0.5
+    a.put(", 0.5\n");
+    column_pool_range(a, J, 8, 10);
+    asm_op3(a, "v_fma_f64 ", OP_INV, ", ", OP_T, OP_E);   // inv = t e + y0
+    a.put(", ");
+    a.op(OP_Y0);
+    a.put("\n");
+    column_pool_range(a, J, 10, 12);
+    asm_op3(a, "v_mul_f64 ", J, ", ", J, OP_INV);   // l = a_rJ inv
+    a.put("\n");
+    asm_op3(a, "v_mul_f64 ", OP_MI, ", ", OP_INV, OP_GT);
+    a.put("\n");
+    asm_op3(a, "v_fma_f64 ", OP_IVR, ", ", OP_INV, OP_EQ);
+    a.put(", ");
+    a.op(OP_IVR);
+    a.put("\n");
+    asm_op3(a, "v_mul_f64 ", OP_M, ", ", J, OP_MI);
+    a.put("\n");
+    column_pool_range(a, J, 12, 32);
+    if (J + 1 < 16) {
+        // a_r,J+1 -= L_J+1,J L_rJ: the next pivot's entry
+        asm_op3(a, "v_fmac_f64_dpp ", J + 1, ", -", J, J);
+        a.put(" row_newbcast:");
+        a.num((unsigned)(J + 1));
+        a.put(" row_mask:0xf bank_mask:0xf\n");
+    }
+    return a;
+}
+// after column 15 nothing is left: it has no trailing columns, and its
+// inverse update has no rows below it (m = 0 on every lane)
+template <int J> struct ColumnAsm {
+    static constexpr AsmText text = column_asm<J>();
+};
+
+// keep the compiler from reordering the LDS accesses of one wavefront (the
+// hardware executes a wavefront's LDS operations in order)
+__device__ __forceinline__ void wave_lds_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+
+template <class F, int... I>
+__device__ __forceinline__ void static_for_seq(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
+
+// 1/sqrt(x): the hardware estimate refined by one Halley step
+__device__ __forceinline__ double rsqrt_refined(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(0.375, e, 0.5), y);
+}
+
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+struct CrArgs {
+    const double* H;   // reduced system, Pp x Pp, lower triangle
+    const double* g;   // right-hand side (Pp)
+    double* x;         // solution (Pp)
+    double* D;         // per super-row: current diagonal block (S x S, lower tiles)
+    double* E;         // per super-row: current coupling to its left neighbour (S x S)
+    double* W;         // per eliminated super-row: L^-1 (S x S)
+    double* U;         // per eliminated super-row: W E_i
+    double* V;         // per eliminated super-row: W E_q^T
+    double* b;         // per super-row: current right-hand side (S)
+    double* y;         // per eliminated super-row: W b_i (S)
+    Control* ctl;
+    int Pp, ns;
+};
+
+// Element (r, c) of super-block (I, J) of H, identity past the padded size.
+// Branch-free: the address is clamped into H and the value selected after the
+// load, so a thread's loads are all issued before the first one is waited for
+// (a load under a per-lane branch is waited for before the branch closes).
+template <int S>
+__device__ __forceinline__ double h_ld(const CrArgs& a, int I, int J, int r, int c) {
+    const int gr = S * I + r, gc = S * J + c;
+    const bool in = gr < a.Pp && gc < a.Pp;
+    const double v = a.H[(long long)min(gr, a.Pp - 1) * a.Pp + min(gc, a.Pp - 1)];
+    return in ? v : (gr == gc ? 1.0 : 0.0);
+}
+
+// g[S I + t], zero past the padded size (branch-free as h_ld)
+template <int S>
+__device__ __forceinline__ double g_ld(const CrArgs& a, int I, int t) {
+    const int gr = S * I + t;
+    const double v = a.g[min(gr, a.Pp - 1)];
+    return gr < a.Pp ? v : 0.0;
+}
+}  // namespace
+
+// One workgroup per eliminated super-row i of this level (final != 0: super-row
+// 0 after the last level, which also solves for x_0 = W^T W b_0).
+template <int P>
+__global__ __launch_bounds__(256) void k_cr_factor(CrArgs a, int level, int final_) {
+    constexpr int S = 16 * P, SP = S + 1, Q = S / 4;   // Q: a quarter row
+    __shared__ double A[S][SP];    // D, then its factor L (lower blocks)
+    __shared__ double Wl[S][SP];   // W = L^-1
+    __shared__ double vb[S], vy[S];
+    if (a.ctl->pcg_fail == 0) return;   // this iteration's CG solve converged
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int i = final_ ? 0 : ((2 * blockIdx.x + 1) << level);
+    const bool fromH = level == 0;
+    const long long SS = (long long)S * S;
+    {
+        // all loads of the thread first, then the LDS stores
+        constexpr int NE = S * S / 256;
+        double ld[NE];
+        if (fromH) {
+#pragma unroll
+            for (int t = 0; t < NE; ++t) {
+                const int e = tid + 256 * t, r = e / S, c = e - (e / S) * S;
+                ld[t] = h_ld<S>(a, i, i, max(r, c), min(r, c));
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < NE; ++t) ld[t] = a.D[i * SS + tid + 256 * t];
+        }
+        // the lower blocks, the diagonal ones whole: the diagonal factor's lanes
+        // also carry the entries above the diagonal, which the elimination
+        // reduces to rounding noise only if they start symmetric
+#pragma unroll
+        for (int t = 0; t < NE; ++t) {
+            const int e = tid + 256 * t, r = e / S, c = e - (e / S) * S;
+            A[r][c] = (c >> 4) <= (r >> 4) ? ld[t] : 0.0;
+            Wl[r][c] = 0.0;
+        }
+    }
+    __syncthreads();
+    bool bad = false;
+    // Step k: wavefront 0 factors the diagonal block k while the others form W's
+    // block row k - 1 and the trailing update of step k - 1 (all but the next
+    // diagonal tile, which wavefront 0 updated itself); after a barrier the
+    // panel blocks of column k, wavefront 0 taking block row k + 1 and then the
+    // tile (k + 1, k + 1), so it can start the next diagonal block at once.
+    for (int k = 0; k < P; ++k) {
+        if (w == 0) {
+            const int r = lane & 15, k0 = 16 * k;
+            double av[16], yv[4];
+            const int R = lane >> 4;   // this 16-lane row keeps columns 4R .. 4R + 3 of the inverse
+            static_for<16>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                av[j] = A[k0 + r][k0 + j];
+            });
+            static_for<4>([&](auto I) {
+                constexpr int c = decltype(I)::value;
+                yv[c] = (4 * R + c == r) ? 1.0 : 0.0;
+            });
+            // column j: pivot d = a_jj, 1/sqrt(d) refined by one Halley step,
+            // L_rj = a_rj / sqrt(d) (into av[j]), then a_rq -= L_rj L_qj for
+            // q > j (q = j + 1 at once, the others during column j + 1).  The
+            // unit factor's inverse: rows r > j take yv -= (L_rj / L_jj) yv_j
+            // (m), applied during column j + 1.
+            double ivr = 0.0, mprev = 0.0;
+            const double c375 = 0.375;
+            static_for<16>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                const double gtj = r > j ? 1.0 : 0.0, eqj = r == j ? 1.0 : 0.0;
+                double m, d, y0, t, e, inv, mi;
+                asm volatile((ColumnAsm<j>::text)
+                             : "+v"(av[0]), "+v"(av[1]), "+v"(av[2]), "+v"(av[3]), "+v"(av[4]), "+v"(av[5]),
+                               "+v"(av[6]), "+v"(av[7]), "+v"(av[8]), "+v"(av[9]), "+v"(av[10]), "+v"(av[11]),
+                               "+v"(av[12]), "+v"(av[13]), "+v"(av[14]), "+v"(av[15]), "+v"(yv[0]), "+v"(yv[1]),
+                               "+v"(yv[2]), "+v"(yv[3]), "=&v"(m), "=&v"(d), "=&v"(y0), "=&v"(t), "=&v"(e),
+                               "=&v"(inv), "=&v"(mi), "+v"(ivr)
+                             : "v"(mprev), "v"(c375), "v"(gtj), "v"(eqj));
+                mprev = m;
+            });
+            // every pivot positive and finite: 1/sqrt of it is too (ivr of lane r
+            // is that of pivot r; the four 16-lane rows agree)
+            bad = bad || __builtin_amdgcn_ballot_w64(!(ivr > 0.0 && ivr < __builtin_inf())) != 0;
+            static_for<4>([&](auto I) {
+                constexpr int j = decltype(I)::value;
+                const int c = 4 * R + j;
+                Wl[k0 + r][k0 + c] = (c <= r) ? yv[j] * ivr : 0.0;
+            });
+            if (lane < 16) {
+                static_for<16>([&](auto J) {
+                    constexpr int j = decltype(J)::value;
+                    A[k0 + r][k0 + j] = (j <= r) ? av[j] : 0.0;
+                });
+            }
+        } else {
+            if (w < k) {
+                // W block row k - 1, column block c = w - 1 < k - 1:
+                // W_bc = -W_bb sum_{m=c}^{b-1} L_bm W_mc, b = k - 1
+                const int bb = k - 1, c = w - 1;
+                d4 t = 0.0;
+                for (int m = c; m < bb; ++m) {
+#pragma unroll
+                    for (int s = 0; s < 4; ++s)
+                        t = mfma(A[16 * bb + (lane & 15)][16 * m + 4 * s + (lane >> 4)],
+                                 Wl[16 * m + 4 * s + (lane >> 4)][16 * c + (lane & 15)], t);
+                }
+                d4 u = 0.0;
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    u = mfma(Wl[16 * bb + (lane & 15)][16 * bb + 4 * s + (lane >> 4)], t[s], u);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Wl[16 * bb + (lane >> 4) + 4 * v][16 * c + (lane & 15)] = -u[v];
+            }
+            if (k >= 1) {
+                // trailing update of step k - 1, A_bc -= L_b,k-1 L_c,k-1^T for
+                // k <= c <= b < P, except (k, k): tiles 1 .. n(n+1)/2 - 1 of the
+                // lower triangle of the n x n trailing blocks
+                const int n = P - k, T = n * (n + 1) / 2, k0 = 16 * (k - 1);
+                for (int t = w; t < T; t += 3) {
+                    int bq = 0;
+                    while ((bq + 1) * (bq + 2) / 2 <= t) ++bq;
+                    const int bb = k + bq, cc = k + (t - bq * (bq + 1) / 2);
+                    d4 acc = 0.0;
+#pragma unroll
+                    for (int s = 0; s < 4; ++s)
+                        acc = mfma(A[16 * bb + (lane & 15)][k0 + 4 * s + (lane >> 4)],
+                                   A[16 * cc + (lane & 15)][k0 + 4 * s + (lane >> 4)], acc);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) A[16 * bb + (lane >> 4) + 4 * v][16 * cc + (lane & 15)] -= acc[v];
+                }
+            }
+        }
+        __syncthreads();
+        if (k + 1 < P) {
+            // panel: L_bk = A_bk W_kk^T, one block row per wavefront
+            const int k0 = 16 * k;
+            if (w < P - 1 - k) {
+                const int bb = k + 1 + w;
+                d4 acc = 0.0;
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    acc = mfma(A[16 * bb + (lane & 15)][k0 + 4 * s + (lane >> 4)],
+                               Wl[k0 + (lane & 15)][k0 + 4 * s + (lane >> 4)], acc);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) A[16 * bb + (lane >> 4) + 4 * v][k0 + (lane & 15)] = acc[v];
+            }
+            if (w == 0) {
+                // the next diagonal tile from this wavefront's own panel block
+                wave_lds_order();
+                const int b1 = 16 * (k + 1);
+                d4 acc = 0.0;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const double x = A[b1 + (lane & 15)][k0 + 4 * s + (lane >> 4)];
+                    acc = mfma(x, x, acc);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) A[b1 + (lane >> 4) + 4 * v][b1 + (lane & 15)] -= acc[v];
+            }
+            __syncthreads();
+        }
+    }
+    if (P > 1) {
+        // the last block row of W
+        if (w < P - 1) {
+            const int bb = P - 1, c = w;
+            d4 t = 0.0;
+            for (int m = c; m < bb; ++m) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    t = mfma(A[16 * bb + (lane & 15)][16 * m + 4 * s + (lane >> 4)],
+                             Wl[16 * m + 4 * s + (lane >> 4)][16 * c + (lane & 15)], t);
+            }
+            d4 u = 0.0;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) u = mfma(Wl[16 * bb + (lane & 15)][16 * bb + 4 * s + (lane >> 4)], t[s], u);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Wl[16 * bb + (lane >> 4) + 4 * v][16 * c + (lane & 15)] = -u[v];
+        }
+    }
+    if (bad && tid == 0) a.ctl->chol_ok = 0;
+    __syncthreads();
+    if (!final_) {
+        double* Wo = a.W + i * SS;
+        for (int e = tid; e < S * S; e += 256) {
+            const int r = e / S, c = e - (e / S) * S;
+            Wo[e] = Wl[r][c];
+        }
+        return;
+    }
+    // super-row 0 alone: x_0 = W^T (W b_0)
+    for (int t = tid; t < S; t += 256) vb[t] = fromH ? g_ld<S>(a, 0, t) : a.b[t];
+    __syncthreads();
+    const int row = tid >> 2, part = tid & 3;
+    if (row < S) {
+        double s = 0.0;
+        for (int c = Q * part; c < Q * part + Q && c <= row; ++c) s = fma(Wl[row][c], vb[c], s);
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        if (part == 0) vy[row] = s;
+    }
+    __syncthreads();
+    if (row < S) {
+        double s = 0.0;
+        for (int r = max(Q * part, row); r < Q * part + Q; ++r) s = fma(Wl[r][row], vy[r], s);
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        if (part == 0 && row < a.Pp) a.x[row] = s;
+    }
+}
+
+// U_i = W E_i, V_i = W E_q^T (16 x 16 tiles: blockIdx.y < P a block row of U,
+// < 2P one of V, one tile column per wavefront) and y_i = W b_i (blockIdx.y = 2P).
+// W is lower triangular (zeros above), so every tile sums over all of W's row.
+template <int P>
+__global__ __launch_bounds__(256) void k_cr_uv(CrArgs a, int level) {
+    constexpr int S = 16 * P, Q = S / 4;
+    if (a.ctl->pcg_fail == 0) return;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int st = 1 << level, i = (2 * blockIdx.x + 1) << level, q = i + st;
+    const bool fromH = level == 0;
+    const long long SS = (long long)S * S;
+    const double* Wi = a.W + i * SS;
+    const int part = blockIdx.y;
+    if (part < 2 * P) {
+        const bool isV = part >= P;
+        const int r = isV ? part - P : part, c = w;
+        if ((isV && q >= a.ns) || c >= P) return;
+        const int n = 16 * c + (lane & 15);
+        double wa[P][4], eb[P][4];
+#pragma unroll
+        for (int m = 0; m < P; ++m)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) wa[m][s] = Wi[(16 * r + (lane & 15)) * S + 16 * m + 4 * s + (lane >> 4)];
+        if (fromH) {
+            // U: E_i(kk, n) = H(i, i - 1); V: E_q^T(kk, n) = H(q, i)(n, kk)
+            const int I = isV ? q : i, J = isV ? i : i - 1;
+#pragma unroll
+            for (int m = 0; m < P; ++m)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int kk = 16 * m + 4 * s + (lane >> 4);
+                    eb[m][s] = h_ld<S>(a, I, J, isV ? n : kk, isV ? kk : n);
+                }
+        } else {
+            const double* Eb = a.E + (isV ? q : i) * SS;
+#pragma unroll
+            for (int m = 0; m < P; ++m)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int kk = 16 * m + 4 * s + (lane >> 4);
+                    eb[m][s] = Eb[isV ? n * S + kk : kk * S + n];
+                }
+        }
+        d4 acc = 0.0;
+#pragma unroll
+        for (int m = 0; m < P; ++m)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) acc = mfma(wa[m][s], eb[m][s], acc);
+        double* out = (isV ? a.V : a.U) + i * SS;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out[(16 * r + (lane >> 4) + 4 * v) * S + n] = acc[v];
+        return;
+    }
+    // y: four threads per row, a quarter of the row each
+    const int row = tid >> 2, pt = tid & 3;
+    if (row < S) {
+        double wv[Q], bv[Q];
+#pragma unroll
+        for (int j = 0; j < Q; ++j) wv[j] = Wi[row * S + Q * pt + j];
+        if (fromH) {
+#pragma unroll
+            for (int j = 0; j < Q; ++j) bv[j] = g_ld<S>(a, i, Q * pt + j);
+        } else {
+#pragma unroll
+            for (int j = 0; j < Q; ++j) bv[j] = a.b[(long long)i * S + Q * pt + j];
+        }
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < Q; ++j) s = fma(wv[j], bv[j], s);
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        if (pt == 0) a.y[(long long)i * S + row] = s;
+    }
+}
+
+// Survivor q of this level: D_q -= V_im^T V_im + U_ip^T U_ip (blockIdx.y < P: a
+// block row of lower tiles), its new coupling -V_im^T U_im (< 2P: a block row),
+// and b_q -= V_im^T y_im + U_ip^T y_ip (= 2P); im = q - 2^l, ip = q + 2^l.
+template <int P>
+__global__ __launch_bounds__(256) void k_cr_update(CrArgs a, int level) {
+    constexpr int S = 16 * P, K4 = S / 4;
+    if (a.ctl->pcg_fail == 0) return;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int st = 1 << level, q = (2 * blockIdx.x) << level;
+    const int im = q - st, ip = q + st;
+    const bool has_m = q > 0, has_p = ip < a.ns;
+    const bool fromH = level == 0;
+    const long long SS = (long long)S * S;
+    const int part = blockIdx.y;
+    if (part < 2 * P) {
+        const bool isE = part >= P;
+        const int r = isE ? part - P : part, c = w;
+        if (c >= P || (!isE && c > r) || (isE && !has_m)) return;
+        const int R = 16 * r + (lane & 15), C = 16 * c + (lane & 15);
+        // every load first: both products' operands and the current block
+        double va[K4], vb[K4], ua[K4], ub[K4], base[4];
+        const bool use_p = !isE && has_p;
+        if (has_m) {
+            const double* Vm = a.V + im * SS;
+            const double* Bm = isE ? a.U + im * SS : Vm;
+#pragma unroll
+            for (int s = 0; s < K4; ++s) {
+                const int kk = 4 * s + (lane >> 4);
+                va[s] = Vm[kk * S + R];
+                vb[s] = Bm[kk * S + C];
+            }
+        }
+        if (use_p) {
+            const double* Up = a.U + ip * SS;
+#pragma unroll
+            for (int s = 0; s < K4; ++s) {
+                const int kk = 4 * s + (lane >> 4);
+                ua[s] = Up[kk * S + R];
+                ub[s] = Up[kk * S + C];
+            }
+        }
+        if (!isE) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int rr = 16 * r + (lane >> 4) + 4 * v, cc = 16 * c + (lane & 15);
+                base[v] = fromH ? h_ld<S>(a, q, q, max(rr, cc), min(rr, cc)) : a.D[q * SS + rr * S + cc];
+            }
+        }
+        d4 acc = 0.0;
+        if (has_m) {
+#pragma unroll
+            for (int s = 0; s < K4; ++s) acc = mfma(va[s], vb[s], acc);
+        }
+        if (use_p) {
+#pragma unroll
+            for (int s = 0; s < K4; ++s) acc = mfma(ua[s], ub[s], acc);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int rr = 16 * r + (lane >> 4) + 4 * v, cc = 16 * c + (lane & 15);
+            if (isE) a.E[q * SS + rr * S + cc] = -acc[v];
+            else a.D[q * SS + rr * S + cc] = base[v] - acc[v];
+        }
+        return;
+    }
+    // b: thread (t, h) sums the half h of k for entry t of both products;
+    // 2S <= 256 threads, halves combined through LDS
+    __shared__ double half[S];
+    const int t = tid % S, h = tid / S;
+    double s = 0.0;
+    if (h < 2) {
+        constexpr int KH = S / 2;
+        double vv[KH], yv[KH], uu[KH], yu[KH];
+        if (has_m) {
+            const double* Vm = a.V + im * SS;
+            const double* ym = a.y + (long long)im * S;
+#pragma unroll
+            for (int k = 0; k < KH; ++k) {
+                vv[k] = Vm[(KH * h + k) * S + t];
+                yv[k] = ym[KH * h + k];
+            }
+        }
+        if (has_p) {
+            const double* Up = a.U + ip * SS;
+            const double* yp = a.y + (long long)ip * S;
+#pragma unroll
+            for (int k = 0; k < KH; ++k) {
+                uu[k] = Up[(KH * h + k) * S + t];
+                yu[k] = yp[KH * h + k];
+            }
+        }
+        double b0 = 0.0;
+        if (h == 0) b0 = fromH ? g_ld<S>(a, q, t) : a.b[(long long)q * S + t];
+        s = b0;
+        if (has_m) {
+#pragma unroll
+            for (int k = 0; k < KH; ++k) s = fma(-vv[k], yv[k], s);
+        }
+        if (has_p) {
+#pragma unroll
+            for (int k = 0; k < KH; ++k) s = fma(-uu[k], yu[k], s);
+        }
+        if (h == 1) half[t] = s;
+    }
+    __syncthreads();
+    if (h == 0) a.b[(long long)q * S + t] = s + half[t];
+}
+
+// x_i = W_i^T (y_i - U_i x_p - V_i x_q) for the super-rows eliminated at this
+// level: four threads per row (a quarter of the columns each), every load
+// issued before the first product
+template <int P>
+__global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int level) {
+    constexpr int S = 16 * P, Q = S / 4;
+    __shared__ double z[S];
+    if (a.ctl->pcg_fail == 0) return;
+    const int tid = threadIdx.x;
+    const int st = 1 << level, i = (2 * blockIdx.x + 1) << level, p = i - st, q = i + st;
+    const long long SS = (long long)S * S;
+    const bool has_q = q < a.ns;
+    const int row = tid >> 2, pt = tid & 3;
+    const double* Ui = a.U + i * SS;
+    const double* Vi = a.V + i * SS;
+    const double* Wi = a.W + i * SS;
+    double uv[Q], vv[Q], wv[Q], xp[Q], xq[Q], yi = 0.0;
+    if (row < S) {
+#pragma unroll
+        for (int j = 0; j < Q; ++j) {
+            const int c = Q * pt + j;
+            uv[j] = Ui[row * S + c];
+            wv[j] = Wi[c * S + row];
+            xp[j] = a.x[S * p + c];   // rows of p lie below S i <= Pp
+        }
+        if (has_q) {
+#pragma unroll
+            for (int j = 0; j < Q; ++j) {
+                const int c = Q * pt + j, gq = S * q + c;
+                vv[j] = Vi[row * S + c];
+                const double xv = a.x[min(gq, a.Pp - 1)];
+                xq[j] = gq < a.Pp ? xv : 0.0;
+            }
+        }
+        yi = a.y[(long long)i * S + row];
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < Q; ++j) s = fma(uv[j], xp[j], s);
+        if (has_q) {
+#pragma unroll
+            for (int j = 0; j < Q; ++j) s = fma(vv[j], xq[j], s);
+        }
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        if (pt == 0) z[row] = yi - s;
+    }
+    __syncthreads();
+    if (row < S) {
+        // W is lower triangular: W(c, row) = 0 for c < row
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < Q; ++j) s = fma(wv[j], z[Q * pt + j], s);
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        const int gr = S * i + row;
+        if (pt == 0 && gr < a.Pp) a.x[gr] = s;
+    }
+}
+
+int cr_max_depth() { return 4; }
+
+size_t cr_work_doubles(int Pp, int D) {
+    const size_t S = 16 * (size_t)D, ns = (Pp + S - 1) / S;
+    return ns * (5 * S * S + 2 * S);
+}
+
+template <int P>
+static void launch_cr_p(const CrArgs& a, hipStream_t s) {
+    int L = 0;
+    while ((1 << L) < a.ns) ++L;
+    for (int l = 0; l < L; ++l) {
+        const int st = 1 << l;
+        const int n_elim = (a.ns - st + 2 * st - 1) / (2 * st);
+        const int n_surv = (a.ns + 2 * st - 1) / (2 * st);
+        hipLaunchKernelGGL(k_cr_factor<P>, dim3(n_elim), dim3(256), 0, s, a, l, 0);
+        hipLaunchKernelGGL(k_cr_uv<P>, dim3(n_elim, 2 * P + 1), dim3(256), 0, s, a, l);
+        hipLaunchKernelGGL(k_cr_update<P>, dim3(n_surv, 2 * P + 1), dim3(256), 0, s, a, l);
+    }
+    hipLaunchKernelGGL(k_cr_factor<P>, dim3(1), dim3(256), 0, s, a, L, 1);
+    for (int l = L - 1; l >= 0; --l) {
+        const int st = 1 << l;
+        const int n_elim = (a.ns - st + 2 * st - 1) / (2 * st);
+        hipLaunchKernelGGL(k_cr_back<P>, dim3(n_elim), dim3(256), 0, s, a, l);
+    }
+}
+
+void launch_cr(const double* H, const double* g, double* x, int Pp, int D, double* work, Control* ctl,
+               hipStream_t s) {
+    CrArgs a;
+    const size_t S = 16 * (size_t)D;
+    a.ns = (int)((Pp + S - 1) / S);
+    const size_t ns = a.ns;
+    a.H = H;
+    a.g = g;
+    a.x = x;
+    a.Pp = Pp;
+    a.ctl = ctl;
+    a.D = work;
+    a.E = a.D + ns * S * S;
+    a.W = a.E + ns * S * S;
+    a.U = a.W + ns * S * S;
+    a.V = a.U + ns * S * S;
+    a.b = a.V + ns * S * S;
+    a.y = a.b + ns * S;
+    switch (D) {
+    case 1: launch_cr_p<1>(a, s); break;
+    case 2: launch_cr_p<2>(a, s); break;
+    case 3: launch_cr_p<3>(a, s); break;
+    default: launch_cr_p<4>(a, s); break;
+    }
+}
+
+}  // namespace spgo

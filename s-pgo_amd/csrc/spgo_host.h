@@ -116,13 +116,14 @@ struct spgo_solver {
     bool band = false;          // envelope Cholesky in one workgroup (narrow profile)
     const int* first = nullptr; // first nonzero block column of every block row
     int band_depth = 0;         // max over block rows of (row - first)
-    int path = SPGO_FACTOR_DENSE;   // factorisation taken: SPGO_FACTOR_DENSE, _ENVELOPE or _ENVELOPE_SPLIT
+    int path = SPGO_FACTOR_DENSE;   // factorisation taken: SPGO_FACTOR_DENSE, _ENVELOPE, _ENVELOPE_SPLIT or _CR
     int s0 = 0, s1 = 0;         // split: separator block rows [s0, s1)
     const int* firstB = nullptr;   // split: profile of the reversed bottom part
     double *Ssep = nullptr, *Smrg = nullptr, *winvS = nullptr, *LpS = nullptr;   // split: separator scratch
     double* solve_y = nullptr;  // dense path: forward solution of the pipelined solves (Pp)
     int* solve_sync = nullptr;  // dense path: their tickets and ready flags (2 + 2 Pp / NB)
     double* Lpan = nullptr;     // envelope: factored panels for the backward solve
+    double* crw = nullptr;      // cyclic reduction: per super-row blocks and vectors
     // dense path: the factor's block profile (NB-wide blocks) and per step the
     // block rows it reaches (nact on the host for the launch sizes)
     const int *dense_first = nullptr, *dense_act_off = nullptr, *dense_act = nullptr;

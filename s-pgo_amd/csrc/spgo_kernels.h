@@ -55,6 +55,13 @@ size_t chol_panel_doubles(int Pp);
 int prepare_chol_band(int Dmax, size_t lds);
 int chol_band_block();
 size_t chol_band_lds(int Pp, int D);
+// block cyclic reduction for profiles at most cr_max_depth() blocks of 16 deep:
+// the factorisation and both solves of H x = g, 4 log2(super-rows) + 1 launches;
+// work holds cr_work_doubles(Pp, D) doubles
+void launch_cr(const double* H, const double* g, double* x, int Pp, int D, double* work, Control* ctl,
+               hipStream_t s);
+size_t cr_work_doubles(int Pp, int D);
+int cr_max_depth();
 void launch_chol_solve(const double* H, const double* Linv, const double* g, double* delta, int Pp,
                        const int* first, const int* act_off, const int* act, double* ywork, int* sync, Control* ctl,
                        hipStream_t s);

@@ -392,6 +392,7 @@ static int factor_path(const spgo_solver* h, const HostLayout& L, std::vector<in
     case SPGO_FACTOR_DENSE: return SPGO_FACTOR_DENSE;
     case SPGO_FACTOR_ENVELOPE: return fits ? SPGO_FACTOR_ENVELOPE : -1;
     case SPGO_FACTOR_ENVELOPE_SPLIT: return split_fits ? SPGO_FACTOR_ENVELOPE_SPLIT : -1;
+    case SPGO_FACTOR_CR: return depth <= cr_max_depth() ? SPGO_FACTOR_CR : -1;
     default:   // AUTO, and the Cholesky behind CG
         if (split_fits && nbk >= 2 * depth + 6) return SPGO_FACTOR_ENVELOPE_SPLIT;
         return fits ? SPGO_FACTOR_ENVELOPE : SPGO_FACTOR_DENSE;
@@ -581,6 +582,9 @@ int setup_layout(spgo_solver* h) {
             if (prepare_chol_split(h->Pp, bw) != 0)
                 return h->fail(SPGO_E_DEVICE, "cannot reserve LDS for the split envelope Cholesky");
         }
+        if (h->path == SPGO_FACTOR_CR &&
+            !(h->crw = h->mem.alloc<double>(cr_work_doubles(h->Pp, std::max(bw, 1)))))
+            return h->fail(SPGO_E_DEVICE, "device allocation failed");
     }
     h->H = h->mem.alloc<double>((size_t)h->Pp * h->Pp);
     h->gvec = h->mem.alloc<double>(h->Pp);
@@ -649,9 +653,11 @@ AsmArgs make_asm(const spgo_solver* h, int init) {
     return a;
 }
 
-// the envelope factorisation and both solves (one or two kernels)
+// the band factorisation and both solves: envelope (one or two kernels) or cyclic reduction
 static void launch_chol(spgo_solver* h, hipStream_t s) {
-    if (h->path == SPGO_FACTOR_ENVELOPE_SPLIT)
+    if (h->path == SPGO_FACTOR_CR)
+        launch_cr(h->H, h->gvec, h->delta, h->Pp, std::max(h->band_depth, 1), h->crw, h->ctl, s);
+    else if (h->path == SPGO_FACTOR_ENVELOPE_SPLIT)
         launch_chol_split(h->H, h->Pp, h->band_depth, h->s0, h->s1, h->first, h->firstB, h->gvec, h->delta, h->Linv,
                           h->Lpan, h->Ssep, h->Smrg, h->winvS, h->LpS, h->ctl, s);
     else
@@ -878,7 +884,7 @@ bool valid_opts(const spgo_options* o, std::string& why) {
     if (!(o->alpha_min > 0.0) || !(o->alpha_max >= o->alpha_min)) return why = "alpha bounds", false;
     if (o->chunk_landmarks < 0 || o->chunk_landmarks > CHUNK_LM) return why = "chunk_landmarks", false;
     if (o->group_chunks < 0) return why = "group_chunks", false;
-    if (o->factorization < 0 || o->factorization > 4) return why = "factorization", false;
+    if (o->factorization < 0 || o->factorization > 5) return why = "factorization", false;
     if (o->coordinates < 0 || o->coordinates > 2) return why = "coordinates", false;
     if (o->pcg_max_iter < 1) return why = "pcg_max_iter", false;
     if (!(o->pcg_tol > 0.0)) return why = "pcg_tol", false;

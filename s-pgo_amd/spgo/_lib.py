@@ -27,6 +27,7 @@ SPGO_FACTOR_DENSE = 1
 SPGO_FACTOR_ENVELOPE = 2
 SPGO_FACTOR_ENVELOPE_SPLIT = 3
 SPGO_FACTOR_PCG = 4
+SPGO_FACTOR_CR = 5
 SPGO_COORD_AUTO = 0
 SPGO_COORD_INCREMENTAL = 1
 SPGO_COORD_FRAMES = 2
