@@ -102,6 +102,16 @@ def dense_flops(n_poses: int) -> float:
     return P ** 3 / 3.0 + 2 * P ** 2
 
 
+# factorisation paths (layout_info()["factorization"]) that run the whole solve
+# in the "cholesky" phase, and the kernels each one launches
+BAND_PATHS = (2, 3, 5)
+CHOL_KERNELS = {1: "k_chol_panel + k_chol_update + k_solve_fwd + k_solve_bwd",
+                2: "k_chol_env", 3: "k_chol_split1 + k_chol_split2",
+                5: "k_cr_factor + k_cr_uv + k_cr_update + k_cr_tail + k_cr_back (cyclic reduction)"}
+CHOL_TRAFFIC = {2: "k_chol_env", 3: "k_chol_split", 5: "k_cr"}
+CHOL_NAMES = {1: "dense", 2: "envelope", 3: "envelope, split", 5: "cyclic reduction"}
+
+
 def profile_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed counter summary
     profiles/traffic_<kernel>.txt (a line 'traffic_bytes_per_launch: N')."""
@@ -216,8 +226,8 @@ def bench_c2(args) -> int:
     s.set_state(X0, p.alpha0.astype(np.float64))
     phases = s.profile_phases(args.steps)
     info = s.layout_info()
-    envelope = info["factorization"] in (2, 3)   # one workgroup, or split at a separator
-    split = info["factorization"] == 3
+    fac = info["factorization"]
+    envelope = fac in BAND_PATHS   # envelope (one workgroup, or split) or cyclic reduction
     if envelope:
         # one kernel factors and solves; the "solve" phase holds no kernel
         phases["cholesky"] += phases.pop("solve")
@@ -230,14 +240,13 @@ def bench_c2(args) -> int:
     if dom == "cholesky":
         flops = envelope_flops(p) if envelope else dense_flops(p.n_poses)
         ach = flops / (phases[dom] * 1e-3) / 1e12
-        name = ("k_chol_split1 + k_chol_split2" if split else "k_chol_env") if envelope else \
-            "k_chol_panel + k_chol_update + k_solve_fwd + k_solve_bwd"
-        roof = {"kernel": name, "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / FP64_PEAK_TFLOPS,
-                "traffic": profile_traffic("k_chol_split" if split else "k_chol_env") if envelope else None,
+        roof = {"kernel": CHOL_KERNELS[fac], "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                "traffic": profile_traffic(CHOL_TRAFFIC[fac]) if envelope else None,
                 "algorithmic_flops": flops,
-                "note": "the profile is walked column by column by one workgroup (two, split at a separator): "
-                        "latency-bound by the sequential column chain; FP64 peak is the whole chip's"}
+                "note": "envelope flops (profile Cholesky and both solves) over the phase's event time; the "
+                        "factorisation is a chain of small dense factorisations (latency-bound); FP64 peak is "
+                        "the whole chip's"}
     else:
         roof = dict(roof_asm)
         if dom != "assemble":
@@ -280,7 +289,7 @@ def bench_c2(args) -> int:
         "phases_ms": phases,
         "roofline": roof,
         "roofline_assemble": roof_asm,
-        "factorization": ("envelope, split" if split else "envelope") if envelope else "dense",
+        "factorization": CHOL_NAMES.get(fac, str(fac)),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
@@ -356,20 +365,19 @@ def reference_cpu_path_rate(n_timed: int = 5):
 
 def chol_roofline(p, phases, info):
     """Roofline object of the reduced-system factorisation phase."""
-    envelope = info["factorization"] in (2, 3)
-    split = info["factorization"] == 3
+    fac = info["factorization"]
+    envelope = fac in BAND_PATHS
     flops = envelope_flops(p) if envelope else dense_flops(p.n_poses)
     ms = phases["cholesky"] + (phases.get("solve", 0.0) if not envelope else 0.0)
     ach = flops / (ms * 1e-3) / 1e12
-    name = ("k_chol_split1 + k_chol_split2" if split else "k_chol_env") if envelope else \
-        "k_chol_panel + k_chol_update + k_solve_fwd + k_solve_bwd"
-    return {"kernel": name, "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": ach / FP64_PEAK_TFLOPS,
-            "traffic": profile_traffic("k_chol_split_c3" if split else "k_chol_env") if envelope else None,
+    traffic = {3: "k_chol_split_c3", 2: "k_chol_env", 5: "k_cr_c3"}.get(fac)
+    return {"kernel": CHOL_KERNELS[fac], "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+            "traffic": profile_traffic(traffic) if traffic else None,
             "algorithmic_flops": flops,
             "note": "envelope flops (profile Cholesky and both solves) over the phase's event time; the "
-                    "column chain of each diagonal block is sequential, so the kernel is latency-bound; "
-                    "FP64 peak is the whole chip's"}
+                    "factorisation is a chain of small dense factorisations (latency-bound); FP64 peak is "
+                    "the whole chip's"}
 
 
 def bench_c3(args) -> int:
@@ -420,7 +428,7 @@ def bench_c3(args) -> int:
     best = out[form]
     real = 4 if form == "mixed" else 8
     phases = dict(best["phases_ms"])
-    envelope = best["layout"]["factorization"] in (2, 3)
+    envelope = best["layout"]["factorization"] in BAND_PATHS
     if envelope:
         phases["cholesky"] += phases.pop("solve")
     nbytes = assembly_bytes(p, real)

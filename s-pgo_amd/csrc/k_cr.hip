@@ -255,44 +255,15 @@ __device__ __forceinline__ double g_ld(const CrArgs& a, int I, int t) {
 }
 }  // namespace
 
-// One workgroup per eliminated super-row i of this level (final != 0: super-row
-// 0 after the last level, which also solves for x_0 = W^T W b_0).
+// Factor the S x S block in A (lower blocks, the diagonal ones whole) as
+// A = L L^T with L in A's lower triangle and W = L^-1 in Wl (zeroed by the
+// caller), by the whole workgroup; the caller has synchronised after filling
+// A and Wl and synchronises before reading them.  Returns, on wavefront 0,
+// whether a pivot was not positive and finite.
 template <int P>
-__global__ __launch_bounds__(256) void k_cr_factor(CrArgs a, int level, int final_) {
-    constexpr int S = 16 * P, SP = S + 1, Q = S / 4;   // Q: a quarter row
-    __shared__ double A[S][SP];    // D, then its factor L (lower blocks)
-    __shared__ double Wl[S][SP];   // W = L^-1
-    __shared__ double vb[S], vy[S];
-    if (a.ctl->pcg_fail == 0) return;   // this iteration's CG solve converged
+__device__ __forceinline__ bool cr_factor(double (*A)[16 * P + 1], double (*Wl)[16 * P + 1]) {
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int i = final_ ? 0 : ((2 * blockIdx.x + 1) << level);
-    const bool fromH = level == 0;
-    const long long SS = (long long)S * S;
-    {
-        // all loads of the thread first, then the LDS stores
-        constexpr int NE = S * S / 256;
-        double ld[NE];
-        if (fromH) {
-#pragma unroll
-            for (int t = 0; t < NE; ++t) {
-                const int e = tid + 256 * t, r = e / S, c = e - (e / S) * S;
-                ld[t] = h_ld<S>(a, i, i, max(r, c), min(r, c));
-            }
-        } else {
-#pragma unroll
-            for (int t = 0; t < NE; ++t) ld[t] = a.D[i * SS + tid + 256 * t];
-        }
-        // the lower blocks, the diagonal ones whole: the diagonal factor's lanes
-        // also carry the entries above the diagonal, which the elimination
-        // reduces to rounding noise only if they start symmetric
-#pragma unroll
-        for (int t = 0; t < NE; ++t) {
-            const int e = tid + 256 * t, r = e / S, c = e - (e / S) * S;
-            A[r][c] = (c >> 4) <= (r >> 4) ? ld[t] : 0.0;
-            Wl[r][c] = 0.0;
-        }
-    }
-    __syncthreads();
+    (void)tid;
     bool bad = false;
     // Step k: wavefront 0 factors the diagonal block k while the others form W's
     // block row k - 1 and the trailing update of step k - 1 (all but the next
@@ -432,34 +403,233 @@ __global__ __launch_bounds__(256) void k_cr_factor(CrArgs a, int level, int fina
             for (int v = 0; v < 4; ++v) Wl[16 * bb + (lane >> 4) + 4 * v][16 * c + (lane & 15)] = -u[v];
         }
     }
+    return bad;
+}
+
+// Load super-row i's diagonal block into A (lower blocks, the diagonal ones
+// whole: the diagonal factor's lanes also carry the entries above the
+// diagonal, which the elimination reduces to rounding noise only if they start
+// symmetric) and zero Wl; all loads of a thread are issued before its stores.
+template <int P>
+__device__ __forceinline__ void cr_load_block(const CrArgs& a, int i, bool fromH, double (*A)[16 * P + 1],
+                                              double (*Wl)[16 * P + 1]) {
+    constexpr int S = 16 * P, NE = S * S / 256;
+    const int tid = threadIdx.x;
+    double ld[NE];
+    if (fromH) {
+#pragma unroll
+        for (int t = 0; t < NE; ++t) {
+            const int e = tid + 256 * t, r = e / S, c = e - (e / S) * S;
+            ld[t] = h_ld<S>(a, i, i, max(r, c), min(r, c));
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < NE; ++t) ld[t] = a.D[(long long)i * S * S + tid + 256 * t];
+    }
+#pragma unroll
+    for (int t = 0; t < NE; ++t) {
+        const int e = tid + 256 * t, r = e / S, c = e - (e / S) * S;
+        A[r][c] = (c >> 4) <= (r >> 4) ? ld[t] : 0.0;
+        Wl[r][c] = 0.0;
+    }
+}
+
+// One workgroup per super-row i eliminated at this level: D_i = L L^T and
+// W = L^-1, stored for the products and the back-substitution.
+template <int P>
+__global__ __launch_bounds__(256) void k_cr_factor(CrArgs a, int level) {
+    constexpr int S = 16 * P, SP = S + 1;
+    __shared__ double A[S][SP];    // D, then its factor L (lower blocks)
+    __shared__ double Wl[S][SP];   // W = L^-1
+    if (a.ctl->pcg_fail == 0) return;   // this iteration's CG solve converged
+    const int tid = threadIdx.x;
+    const int i = (2 * blockIdx.x + 1) << level;
+    const long long SS = (long long)S * S;
+    cr_load_block<P>(a, i, level == 0, A, Wl);
+    __syncthreads();
+    const bool bad = cr_factor<P>(A, Wl);
     if (bad && tid == 0) a.ctl->chol_ok = 0;
     __syncthreads();
-    if (!final_) {
-        double* Wo = a.W + i * SS;
-        for (int e = tid; e < S * S; e += 256) {
-            const int r = e / S, c = e - (e / S) * S;
-            Wo[e] = Wl[r][c];
-        }
-        return;
+    double* Wo = a.W + i * SS;
+    for (int e = tid; e < S * S; e += 256) {
+        const int r = e / S, c = e - (e / S) * S;
+        Wo[e] = Wl[r][c];
     }
-    // super-row 0 alone: x_0 = W^T (W b_0)
-    for (int t = tid; t < S; t += 256) vb[t] = fromH ? g_ld<S>(a, 0, t) : a.b[t];
-    __syncthreads();
+}
+
+// The super-rows left after the last level, q_k = k 2^L (k < t), form a block
+// tridiagonal system (D_k, coupling E_k of q_k to q_(k-1), b_k).  One
+// workgroup solves it by the block Cholesky factorisation, k in order:
+//     M_k = E_k W_(k-1)^T,  L_k L_k^T = D_k - M_k M_k^T,  W_k = L_k^-1,
+//     y_k = W_k (b_k - M_k y_(k-1)),
+// then x_(t-1) = W_(t-1)^T y_(t-1), x_k = W_k^T (y_k - M_(k+1)^T x_(k+1)).
+// W_k, M_k and y_k wait in the rows' workspace for the backward sweep.
+template <int P>
+__global__ __launch_bounds__(256) void k_cr_tail(CrArgs a, int L) {
+    constexpr int S = 16 * P, SP = S + 1, Q = S / 4;
+    __shared__ double A[S][SP];     // D_k, then L_k
+    __shared__ double W0[S][SP];    // W of the even steps
+    __shared__ double W1[S][SP];    // W of the odd steps
+    __shared__ double Mb[S][SP];    // E_k, then M_k
+    __shared__ double vb[S], vy[S], vx[S];
+    if (a.ctl->pcg_fail == 0) return;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int t = (a.ns + (1 << L) - 1) >> L;
+    const bool fromH = L == 0;
+    const long long SS = (long long)S * S;
     const int row = tid >> 2, part = tid & 3;
-    if (row < S) {
-        double s = 0.0;
-        for (int c = Q * part; c < Q * part + Q && c <= row; ++c) s = fma(Wl[row][c], vb[c], s);
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        if (part == 0) vy[row] = s;
+    bool bad = false;
+    for (int k = 0; k < t; ++k) {
+        const int q = k << L;
+        double(*Wc)[SP] = (k & 1) ? W1 : W0;
+        double(*Wp)[SP] = (k & 1) ? W0 : W1;
+        cr_load_block<P>(a, q, fromH, A, Wc);
+        for (int e = tid; e < S; e += 256) vb[e] = fromH ? g_ld<S>(a, q, e) : a.b[(long long)q * S + e];
+        if (k >= 1) {
+            constexpr int NE = S * S / 256;
+            double ld[NE];
+            if (fromH) {
+#pragma unroll
+                for (int u = 0; u < NE; ++u) {
+                    const int e = tid + 256 * u, r = e / S, c = e - (e / S) * S;
+                    ld[u] = h_ld<S>(a, q, q - 1, r, c);
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < NE; ++u) ld[u] = a.E[q * SS + tid + 256 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < NE; ++u) {
+                const int e = tid + 256 * u, r = e / S, c = e - (e / S) * S;
+                Mb[r][c] = ld[u];
+            }
+        }
+        __syncthreads();
+        if (k >= 1) {
+            // M = E W_prev^T, wavefront w forms block row w (all its E reads
+            // before its writes, so in place); W_prev is lower triangular
+            if (w < P) {
+                d4 acc[P];
+#pragma unroll
+                for (int c = 0; c < P; ++c) {
+                    acc[c] = 0.0;
+                    for (int m = 0; m <= c; ++m) {
+#pragma unroll
+                        for (int s4 = 0; s4 < 4; ++s4)
+                            acc[c] = mfma(Mb[16 * w + (lane & 15)][16 * m + 4 * s4 + (lane >> 4)],
+                                          Wp[16 * c + (lane & 15)][16 * m + 4 * s4 + (lane >> 4)], acc[c]);
+                    }
+                }
+                wave_lds_order();
+#pragma unroll
+                for (int c = 0; c < P; ++c)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) Mb[16 * w + (lane >> 4) + 4 * v][16 * c + (lane & 15)] = acc[c][v];
+            }
+            __syncthreads();
+            // D -= M M^T (lower tiles, the diagonal ones whole); b -= M y_prev
+            for (int tt = w; tt < P * (P + 1) / 2; tt += 4) {
+                int r = 0;
+                while ((r + 1) * (r + 2) / 2 <= tt) ++r;
+                const int c = tt - r * (r + 1) / 2;
+                d4 acc = 0.0;
+#pragma unroll
+                for (int s4 = 0; s4 < S / 4; ++s4)
+                    acc = mfma(Mb[16 * r + (lane & 15)][4 * s4 + (lane >> 4)], Mb[16 * c + (lane & 15)][4 * s4 + (lane >> 4)],
+                               acc);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) A[16 * r + (lane >> 4) + 4 * v][16 * c + (lane & 15)] -= acc[v];
+            }
+            if (row < S) {
+                double s = 0.0;
+                for (int c = Q * part; c < Q * part + Q; ++c) s = fma(Mb[row][c], vy[c], s);
+                s += __shfl_xor(s, 1);
+                s += __shfl_xor(s, 2);
+                if (part == 0) vb[row] -= s;
+            }
+            // M_k for the backward sweep
+            for (int e = tid; e < S * S; e += 256) {
+                const int r = e / S, c = e - (e / S) * S;
+                a.U[q * SS + e] = Mb[r][c];
+            }
+            __syncthreads();
+        }
+        bad = cr_factor<P>(A, Wc) || bad;
+        __syncthreads();
+        // y_k = W_k b
+        if (row < S) {
+            double s = 0.0;
+            for (int c = Q * part; c < Q * part + Q; ++c) s = fma(Wc[row][c], vb[c], s);
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            if (part == 0) {
+                vy[row] = s;
+                a.y[(long long)q * S + row] = s;
+            }
+        }
+        if (k + 1 < t)
+            for (int e = tid; e < S * S; e += 256) {
+                const int r = e / S, c = e - (e / S) * S;
+                a.W[q * SS + e] = Wc[r][c];
+            }
+        __syncthreads();
     }
-    __syncthreads();
-    if (row < S) {
-        double s = 0.0;
-        for (int r = max(Q * part, row); r < Q * part + Q; ++r) s = fma(Wl[r][row], vy[r], s);
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        if (part == 0 && row < a.Pp) a.x[row] = s;
+    if (bad && tid == 0) a.ctl->chol_ok = 0;
+    // backward: x_(t-1) = W^T y with W_(t-1) still in LDS
+    {
+        double(*Wc)[SP] = ((t - 1) & 1) ? W1 : W0;
+        if (row < S) {
+            double s = 0.0;
+            for (int r = Q * part; r < Q * part + Q; ++r) s = fma(Wc[r][row], vy[r], s);
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            if (part == 0) {
+                vx[row] = s;
+                const int gr = S * ((t - 1) << L) + row;
+                if (gr < a.Pp) a.x[gr] = s;
+            }
+        }
+    }
+    for (int k = t - 2; k >= 0; --k) {
+        const int q = k << L, q1 = (k + 1) << L;
+        __syncthreads();
+        // z = y_k - M_(k+1)^T x_(k+1) (M into Mb, W_k into A)
+        {
+            constexpr int NE = S * S / 256;
+            double lm[NE], lw[NE];
+#pragma unroll
+            for (int u = 0; u < NE; ++u) {
+                lm[u] = a.U[q1 * SS + tid + 256 * u];
+                lw[u] = a.W[q * SS + tid + 256 * u];
+            }
+            if (tid < S) vb[tid] = a.y[(long long)q * S + tid];
+#pragma unroll
+            for (int u = 0; u < NE; ++u) {
+                const int e = tid + 256 * u, r = e / S, c = e - (e / S) * S;
+                Mb[r][c] = lm[u];
+                A[r][c] = lw[u];
+            }
+        }
+        __syncthreads();
+        if (row < S) {
+            double s = 0.0;
+            for (int r = Q * part; r < Q * part + Q; ++r) s = fma(Mb[r][row], vx[r], s);
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            if (part == 0) vb[row] -= s;
+        }
+        __syncthreads();
+        if (row < S) {
+            double s = 0.0;
+            for (int r = Q * part; r < Q * part + Q; ++r) s = fma(A[r][row], vb[r], s);
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            if (part == 0) {
+                vx[row] = s;
+                const int gr = S * q + row;
+                if (gr < a.Pp) a.x[gr] = s;
+            }
+        }
     }
 }
 
@@ -712,19 +882,23 @@ size_t cr_work_doubles(int Pp, int D) {
     return ns * (5 * S * S + 2 * S);
 }
 
+// super-rows left to the sequential tail: each level of the reduction costs
+// about as much as two or three tail steps, so the levels stop at this many
+constexpr int CR_TAIL_ROWS = 3;
+
 template <int P>
 static void launch_cr_p(const CrArgs& a, hipStream_t s) {
     int L = 0;
-    while ((1 << L) < a.ns) ++L;
+    while (((a.ns + (1 << L) - 1) >> L) > CR_TAIL_ROWS) ++L;
     for (int l = 0; l < L; ++l) {
         const int st = 1 << l;
         const int n_elim = (a.ns - st + 2 * st - 1) / (2 * st);
         const int n_surv = (a.ns + 2 * st - 1) / (2 * st);
-        hipLaunchKernelGGL(k_cr_factor<P>, dim3(n_elim), dim3(256), 0, s, a, l, 0);
+        hipLaunchKernelGGL(k_cr_factor<P>, dim3(n_elim), dim3(256), 0, s, a, l);
         hipLaunchKernelGGL(k_cr_uv<P>, dim3(n_elim, 2 * P + 1), dim3(256), 0, s, a, l);
         hipLaunchKernelGGL(k_cr_update<P>, dim3(n_surv, 2 * P + 1), dim3(256), 0, s, a, l);
     }
-    hipLaunchKernelGGL(k_cr_factor<P>, dim3(1), dim3(256), 0, s, a, L, 1);
+    hipLaunchKernelGGL(k_cr_tail<P>, dim3(1), dim3(256), 0, s, a, L);
     for (int l = L - 1; l >= 0; --l) {
         const int st = 1 << l;
         const int n_elim = (a.ns - st + 2 * st - 1) / (2 * st);
