@@ -19,11 +19,12 @@
 //
 // The sequential part is one S x S factorisation per level (log2 of the
 // number of super-rows levels) instead of a column chain over all of H.
-// Per level three launches: k_cr_factor (one workgroup per eliminated
-// super-row: D_i = L L^T and W = L^-1), k_cr_uv (U, V and y as 16 x 16 FP64
-// MFMA tiles), k_cr_update (the survivors' new D, coupling and b as MFMA
-// tiles); then one k_cr_factor for super-row 0 with its solution, and one
-// k_cr_back per level.  Level 0 reads H and g in place (the lower triangle);
+// Per level two launches: k_cr_factor (one workgroup per eliminated
+// super-row: D_i = L L^T, W = L^-1, and U, V, y as 16 x 16 FP64 MFMA tiles)
+// and k_cr_update (the survivors' new D, coupling and b as MFMA tiles).  Once
+// at most three super-rows are left, k_cr_tail solves them in one workgroup as
+// a block-tridiagonal Cholesky, and one k_cr_back per level brings the
+// solution back down.  Level 0 reads H and g in place (the lower triangle);
 // later levels read the workspace.  Rows past the padded size are identity.
 //
 // k_cr_factor factors D right-looking over 16-wide block columns.  The
@@ -255,6 +256,30 @@ __device__ __forceinline__ double g_ld(const CrArgs& a, int I, int t) {
 }
 }  // namespace
 
+// sum_{m=c}^{b-1} L_bm W_mc (a 16 x 16 MFMA tile) for W's block (b, c), b > c:
+// every operand of the at most P - 1 block products is read from LDS first
+// (the addresses are valid for every m < P), then the products are taken
+template <int P>
+__device__ __forceinline__ d4 w_row_sum(double (*A)[16 * P + 1], double (*Wl)[16 * P + 1], int bb, int c,
+                                        int lane) {
+    double la[P][4], wb[P][4];
+#pragma unroll
+    for (int m = 0; m < P; ++m)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            la[m][s] = A[16 * bb + (lane & 15)][16 * m + 4 * s + (lane >> 4)];
+            wb[m][s] = Wl[16 * m + 4 * s + (lane >> 4)][16 * c + (lane & 15)];
+        }
+    d4 t = 0.0;
+#pragma unroll
+    for (int m = 0; m < P; ++m)
+        if (m >= c && m < bb) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) t = mfma(la[m][s], wb[m][s], t);
+        }
+    return t;
+}
+
 // Factor the S x S block in A (lower blocks, the diagonal ones whole) as
 // A = L L^T with L in A's lower triangle and W = L^-1 in Wl (zeroed by the
 // caller), by the whole workgroup; the caller has synchronised after filling
@@ -322,13 +347,7 @@ __device__ __forceinline__ bool cr_factor(double (*A)[16 * P + 1], double (*Wl)[
                 // W block row k - 1, column block c = w - 1 < k - 1:
                 // W_bc = -W_bb sum_{m=c}^{b-1} L_bm W_mc, b = k - 1
                 const int bb = k - 1, c = w - 1;
-                d4 t = 0.0;
-                for (int m = c; m < bb; ++m) {
-#pragma unroll
-                    for (int s = 0; s < 4; ++s)
-                        t = mfma(A[16 * bb + (lane & 15)][16 * m + 4 * s + (lane >> 4)],
-                                 Wl[16 * m + 4 * s + (lane >> 4)][16 * c + (lane & 15)], t);
-                }
+                const d4 t = w_row_sum<P>(A, Wl, bb, c, lane);
                 d4 u = 0.0;
 #pragma unroll
                 for (int s = 0; s < 4; ++s)
@@ -389,13 +408,7 @@ __device__ __forceinline__ bool cr_factor(double (*A)[16 * P + 1], double (*Wl)[
         // the last block row of W
         if (w < P - 1) {
             const int bb = P - 1, c = w;
-            d4 t = 0.0;
-            for (int m = c; m < bb; ++m) {
-#pragma unroll
-                for (int s = 0; s < 4; ++s)
-                    t = mfma(A[16 * bb + (lane & 15)][16 * m + 4 * s + (lane >> 4)],
-                             Wl[16 * m + 4 * s + (lane >> 4)][16 * c + (lane & 15)], t);
-            }
+            const d4 t = w_row_sum<P>(A, Wl, bb, c, lane);
             d4 u = 0.0;
 #pragma unroll
             for (int s = 0; s < 4; ++s) u = mfma(Wl[16 * bb + (lane & 15)][16 * bb + 4 * s + (lane >> 4)], t[s], u);
@@ -434,22 +447,96 @@ __device__ __forceinline__ void cr_load_block(const CrArgs& a, int i, bool fromH
     }
 }
 
-// One workgroup per super-row i eliminated at this level: D_i = L L^T and
-// W = L^-1, stored for the products and the back-substitution.
+// One workgroup per super-row i eliminated at this level: D_i = L L^T, W = L^-1,
+// then U_i = W E_i, V_i = W E_q^T (MFMA tiles, wavefront w taking tile column w
+// of both) and y_i = W b_i; W, U, V and y are stored for the survivors' update
+// and the back-substitution.  D_i, E_i, E_q and b_i are loaded at once.
 template <int P>
 __global__ __launch_bounds__(256) void k_cr_factor(CrArgs a, int level) {
-    constexpr int S = 16 * P, SP = S + 1;
+    constexpr int S = 16 * P, SP = S + 1, NE = S * S / 256, Q = S / 4;
     __shared__ double A[S][SP];    // D, then its factor L (lower blocks)
     __shared__ double Wl[S][SP];   // W = L^-1
+    __shared__ double Ei[S][SP];   // coupling of i to p = i - 2^l
+    __shared__ double Eq[S][SP];   // coupling of q = i + 2^l to i
+    __shared__ double vb[S];
     if (a.ctl->pcg_fail == 0) return;   // this iteration's CG solve converged
-    const int tid = threadIdx.x;
-    const int i = (2 * blockIdx.x + 1) << level;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int st = 1 << level, i = (2 * blockIdx.x + 1) << level, q = i + st;
+    const bool fromH = level == 0, has_q = q < a.ns;
     const long long SS = (long long)S * S;
-    cr_load_block<P>(a, i, level == 0, A, Wl);
+    {
+        double le[NE], lq[NE];
+        if (fromH) {
+#pragma unroll
+            for (int u = 0; u < NE; ++u) {
+                const int e = tid + 256 * u, r = e / S, c = e - (e / S) * S;
+                le[u] = h_ld<S>(a, i, i - 1, r, c);
+                lq[u] = has_q ? h_ld<S>(a, q, i, r, c) : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < NE; ++u) {
+                le[u] = a.E[i * SS + tid + 256 * u];
+                lq[u] = has_q ? a.E[q * SS + tid + 256 * u] : 0.0;
+            }
+        }
+        cr_load_block<P>(a, i, fromH, A, Wl);
+        if (tid < S) vb[tid] = fromH ? g_ld<S>(a, i, tid) : a.b[(long long)i * S + tid];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int e = tid + 256 * u, r = e / S, c = e - (e / S) * S;
+            Ei[r][c] = le[u];
+            Eq[r][c] = lq[u];
+        }
+    }
     __syncthreads();
     const bool bad = cr_factor<P>(A, Wl);
     if (bad && tid == 0) a.ctl->chol_ok = 0;
     __syncthreads();
+    if (w < P) {
+        // tile column c = w of U = W E_i and of V = W E_q^T; W is lower triangular
+        const int c = w, n = 16 * c + (lane & 15);
+        double* Uo = a.U + i * SS;
+        double* Vo = a.V + i * SS;
+        static_for<P>([&](auto Rr) {
+            constexpr int r = decltype(Rr)::value;
+            // the tile row's operands first, then its MFMAs
+            double wa[r + 1][4], eu[r + 1][4], ev[r + 1][4];
+#pragma unroll
+            for (int m = 0; m <= r; ++m)
+#pragma unroll
+                for (int s4 = 0; s4 < 4; ++s4) {
+                    const int kk = 16 * m + 4 * s4 + (lane >> 4);
+                    wa[m][s4] = Wl[16 * r + (lane & 15)][kk];
+                    eu[m][s4] = Ei[kk][n];
+                    ev[m][s4] = Eq[n][kk];
+                }
+            d4 u = 0.0, v = 0.0;
+#pragma unroll
+            for (int m = 0; m <= r; ++m)
+#pragma unroll
+                for (int s4 = 0; s4 < 4; ++s4) {
+                    u = mfma(wa[m][s4], eu[m][s4], u);
+                    v = mfma(wa[m][s4], ev[m][s4], v);
+                }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                Uo[(16 * r + (lane >> 4) + 4 * e) * S + n] = u[e];
+                if (has_q) Vo[(16 * r + (lane >> 4) + 4 * e) * S + n] = v[e];
+            }
+        });
+    }
+    // y = W b, four threads per row
+    {
+        const int row = tid >> 2, pt = tid & 3;
+        if (row < S) {
+            double sacc = 0.0;
+            for (int c = Q * pt; c < Q * pt + Q; ++c) sacc = fma(Wl[row][c], vb[c], sacc);
+            sacc += __shfl_xor(sacc, 1);
+            sacc += __shfl_xor(sacc, 2);
+            if (pt == 0) a.y[(long long)i * S + row] = sacc;
+        }
+    }
     double* Wo = a.W + i * SS;
     for (int e = tid; e < S * S; e += 256) {
         const int r = e / S, c = e - (e / S) * S;
@@ -510,16 +597,24 @@ __global__ __launch_bounds__(256) void k_cr_tail(CrArgs a, int L) {
             // before its writes, so in place); W_prev is lower triangular
             if (w < P) {
                 d4 acc[P];
+                double ea[P][4];
 #pragma unroll
-                for (int c = 0; c < P; ++c) {
+                for (int m = 0; m < P; ++m)
+#pragma unroll
+                    for (int s4 = 0; s4 < 4; ++s4) ea[m][s4] = Mb[16 * w + (lane & 15)][16 * m + 4 * s4 + (lane >> 4)];
+                static_for<P>([&](auto Cc) {
+                    constexpr int c = decltype(Cc)::value;
+                    double wb[c + 1][4];
+#pragma unroll
+                    for (int m = 0; m <= c; ++m)
+#pragma unroll
+                        for (int s4 = 0; s4 < 4; ++s4) wb[m][s4] = Wp[16 * c + (lane & 15)][16 * m + 4 * s4 + (lane >> 4)];
                     acc[c] = 0.0;
-                    for (int m = 0; m <= c; ++m) {
 #pragma unroll
-                        for (int s4 = 0; s4 < 4; ++s4)
-                            acc[c] = mfma(Mb[16 * w + (lane & 15)][16 * m + 4 * s4 + (lane >> 4)],
-                                          Wp[16 * c + (lane & 15)][16 * m + 4 * s4 + (lane >> 4)], acc[c]);
-                    }
-                }
+                    for (int m = 0; m <= c; ++m)
+#pragma unroll
+                        for (int s4 = 0; s4 < 4; ++s4) acc[c] = mfma(ea[m][s4], wb[m][s4], acc[c]);
+                });
                 wave_lds_order();
 #pragma unroll
                 for (int c = 0; c < P; ++c)
@@ -532,11 +627,15 @@ __global__ __launch_bounds__(256) void k_cr_tail(CrArgs a, int L) {
                 int r = 0;
                 while ((r + 1) * (r + 2) / 2 <= tt) ++r;
                 const int c = tt - r * (r + 1) / 2;
+                double ma[S / 4], mc[S / 4];
+#pragma unroll
+                for (int s4 = 0; s4 < S / 4; ++s4) {
+                    ma[s4] = Mb[16 * r + (lane & 15)][4 * s4 + (lane >> 4)];
+                    mc[s4] = Mb[16 * c + (lane & 15)][4 * s4 + (lane >> 4)];
+                }
                 d4 acc = 0.0;
 #pragma unroll
-                for (int s4 = 0; s4 < S / 4; ++s4)
-                    acc = mfma(Mb[16 * r + (lane & 15)][4 * s4 + (lane >> 4)], Mb[16 * c + (lane & 15)][4 * s4 + (lane >> 4)],
-                               acc);
+                for (int s4 = 0; s4 < S / 4; ++s4) acc = mfma(ma[s4], mc[s4], acc);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) A[16 * r + (lane >> 4) + 4 * v][16 * c + (lane & 15)] -= acc[v];
             }
@@ -630,81 +729,6 @@ __global__ __launch_bounds__(256) void k_cr_tail(CrArgs a, int L) {
                 if (gr < a.Pp) a.x[gr] = s;
             }
         }
-    }
-}
-
-// U_i = W E_i, V_i = W E_q^T (16 x 16 tiles: blockIdx.y < P a block row of U,
-// < 2P one of V, one tile column per wavefront) and y_i = W b_i (blockIdx.y = 2P).
-// W is lower triangular (zeros above), so every tile sums over all of W's row.
-template <int P>
-__global__ __launch_bounds__(256) void k_cr_uv(CrArgs a, int level) {
-    constexpr int S = 16 * P, Q = S / 4;
-    if (a.ctl->pcg_fail == 0) return;
-    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int st = 1 << level, i = (2 * blockIdx.x + 1) << level, q = i + st;
-    const bool fromH = level == 0;
-    const long long SS = (long long)S * S;
-    const double* Wi = a.W + i * SS;
-    const int part = blockIdx.y;
-    if (part < 2 * P) {
-        const bool isV = part >= P;
-        const int r = isV ? part - P : part, c = w;
-        if ((isV && q >= a.ns) || c >= P) return;
-        const int n = 16 * c + (lane & 15);
-        double wa[P][4], eb[P][4];
-#pragma unroll
-        for (int m = 0; m < P; ++m)
-#pragma unroll
-            for (int s = 0; s < 4; ++s) wa[m][s] = Wi[(16 * r + (lane & 15)) * S + 16 * m + 4 * s + (lane >> 4)];
-        if (fromH) {
-            // U: E_i(kk, n) = H(i, i - 1); V: E_q^T(kk, n) = H(q, i)(n, kk)
-            const int I = isV ? q : i, J = isV ? i : i - 1;
-#pragma unroll
-            for (int m = 0; m < P; ++m)
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const int kk = 16 * m + 4 * s + (lane >> 4);
-                    eb[m][s] = h_ld<S>(a, I, J, isV ? n : kk, isV ? kk : n);
-                }
-        } else {
-            const double* Eb = a.E + (isV ? q : i) * SS;
-#pragma unroll
-            for (int m = 0; m < P; ++m)
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const int kk = 16 * m + 4 * s + (lane >> 4);
-                    eb[m][s] = Eb[isV ? n * S + kk : kk * S + n];
-                }
-        }
-        d4 acc = 0.0;
-#pragma unroll
-        for (int m = 0; m < P; ++m)
-#pragma unroll
-            for (int s = 0; s < 4; ++s) acc = mfma(wa[m][s], eb[m][s], acc);
-        double* out = (isV ? a.V : a.U) + i * SS;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) out[(16 * r + (lane >> 4) + 4 * v) * S + n] = acc[v];
-        return;
-    }
-    // y: four threads per row, a quarter of the row each
-    const int row = tid >> 2, pt = tid & 3;
-    if (row < S) {
-        double wv[Q], bv[Q];
-#pragma unroll
-        for (int j = 0; j < Q; ++j) wv[j] = Wi[row * S + Q * pt + j];
-        if (fromH) {
-#pragma unroll
-            for (int j = 0; j < Q; ++j) bv[j] = g_ld<S>(a, i, Q * pt + j);
-        } else {
-#pragma unroll
-            for (int j = 0; j < Q; ++j) bv[j] = a.b[(long long)i * S + Q * pt + j];
-        }
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < Q; ++j) s = fma(wv[j], bv[j], s);
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        if (pt == 0) a.y[(long long)i * S + row] = s;
     }
 }
 
@@ -817,57 +841,62 @@ __global__ __launch_bounds__(256) void k_cr_update(CrArgs a, int level) {
 }
 
 // x_i = W_i^T (y_i - U_i x_p - V_i x_q) for the super-rows eliminated at this
-// level: four threads per row (a quarter of the columns each), every load
-// issued before the first product
+// level: U, V and W staged in LDS by coalesced loads (a thread's row-wise
+// global reads would touch a cache line per lane), then four threads per row
+// (a quarter of the columns each)
 template <int P>
 __global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int level) {
-    constexpr int S = 16 * P, Q = S / 4;
-    __shared__ double z[S];
+    constexpr int S = 16 * P, SP = S + 1, Q = S / 4, NE = S * S / 256;
+    __shared__ double Us[S][SP], Vs[S][SP], Ws[S][SP];
+    __shared__ double xp[S], xq[S], z[S], yv[S];
     if (a.ctl->pcg_fail == 0) return;
     const int tid = threadIdx.x;
     const int st = 1 << level, i = (2 * blockIdx.x + 1) << level, p = i - st, q = i + st;
     const long long SS = (long long)S * S;
     const bool has_q = q < a.ns;
+    {
+        double lu[NE], lv[NE], lw[NE];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            lu[u] = a.U[i * SS + tid + 256 * u];
+            lv[u] = has_q ? a.V[i * SS + tid + 256 * u] : 0.0;
+            lw[u] = a.W[i * SS + tid + 256 * u];
+        }
+        if (tid < S) {
+            xp[tid] = a.x[S * p + tid];   // rows of p lie below S i <= Pp
+            const int gq = S * q + tid;
+            const double v = a.x[min(gq, a.Pp - 1)];
+            xq[tid] = (has_q && gq < a.Pp) ? v : 0.0;
+            yv[tid] = a.y[(long long)i * S + tid];
+        }
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int e = tid + 256 * u, r = e / S, c = e - (e / S) * S;
+            Us[r][c] = lu[u];
+            Vs[r][c] = lv[u];
+            Ws[r][c] = lw[u];
+        }
+    }
+    __syncthreads();
     const int row = tid >> 2, pt = tid & 3;
-    const double* Ui = a.U + i * SS;
-    const double* Vi = a.V + i * SS;
-    const double* Wi = a.W + i * SS;
-    double uv[Q], vv[Q], wv[Q], xp[Q], xq[Q], yi = 0.0;
     if (row < S) {
+        double s = 0.0;
 #pragma unroll
         for (int j = 0; j < Q; ++j) {
             const int c = Q * pt + j;
-            uv[j] = Ui[row * S + c];
-            wv[j] = Wi[c * S + row];
-            xp[j] = a.x[S * p + c];   // rows of p lie below S i <= Pp
-        }
-        if (has_q) {
-#pragma unroll
-            for (int j = 0; j < Q; ++j) {
-                const int c = Q * pt + j, gq = S * q + c;
-                vv[j] = Vi[row * S + c];
-                const double xv = a.x[min(gq, a.Pp - 1)];
-                xq[j] = gq < a.Pp ? xv : 0.0;
-            }
-        }
-        yi = a.y[(long long)i * S + row];
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < Q; ++j) s = fma(uv[j], xp[j], s);
-        if (has_q) {
-#pragma unroll
-            for (int j = 0; j < Q; ++j) s = fma(vv[j], xq[j], s);
+            s = fma(Us[row][c], xp[c], s);
+            s = fma(Vs[row][c], xq[c], s);
         }
         s += __shfl_xor(s, 1);
         s += __shfl_xor(s, 2);
-        if (pt == 0) z[row] = yi - s;
+        if (pt == 0) z[row] = yv[row] - s;
     }
     __syncthreads();
     if (row < S) {
-        // W is lower triangular: W(c, row) = 0 for c < row
+        // W is lower triangular: W(r, row) = 0 for r < row
         double s = 0.0;
 #pragma unroll
-        for (int j = 0; j < Q; ++j) s = fma(wv[j], z[Q * pt + j], s);
+        for (int j = 0; j < Q; ++j) s = fma(Ws[Q * pt + j][row], z[Q * pt + j], s);
         s += __shfl_xor(s, 1);
         s += __shfl_xor(s, 2);
         const int gr = S * i + row;
@@ -895,7 +924,6 @@ static void launch_cr_p(const CrArgs& a, hipStream_t s) {
         const int n_elim = (a.ns - st + 2 * st - 1) / (2 * st);
         const int n_surv = (a.ns + 2 * st - 1) / (2 * st);
         hipLaunchKernelGGL(k_cr_factor<P>, dim3(n_elim), dim3(256), 0, s, a, l);
-        hipLaunchKernelGGL(k_cr_uv<P>, dim3(n_elim, 2 * P + 1), dim3(256), 0, s, a, l);
         hipLaunchKernelGGL(k_cr_update<P>, dim3(n_surv, 2 * P + 1), dim3(256), 0, s, a, l);
     }
     hipLaunchKernelGGL(k_cr_tail<P>, dim3(1), dim3(256), 0, s, a, L);
