@@ -107,7 +107,7 @@ def dense_flops(n_poses: int) -> float:
 BAND_PATHS = (2, 3, 5)
 CHOL_KERNELS = {1: "k_chol_panel + k_chol_update + k_solve_fwd + k_solve_bwd",
                 2: "k_chol_env", 3: "k_chol_split1 + k_chol_split2",
-                5: "k_cr_factor + k_cr_uv + k_cr_update + k_cr_tail + k_cr_back (cyclic reduction)"}
+                5: "k_cr_factor + k_cr_update + k_cr_tail + k_cr_back (cyclic reduction)"}
 CHOL_TRAFFIC = {2: "k_chol_env", 3: "k_chol_split", 5: "k_cr"}
 CHOL_NAMES = {1: "dense", 2: "envelope", 3: "envelope, split", 5: "cyclic reduction"}
 

@@ -79,14 +79,17 @@ __global__ __launch_bounds__(256) void k_form_system(Layout lay, StateBuf<T> B0,
         gdo[e] = f.d_off - 6 * f.anchor;
     }
     __syncthreads();
-    // items (r, col), col <= row0 + r: every lower entry of the rows is written
-    // (zeros outside the profile), so a failed factorisation cannot leave values
-    // behind for the next one
-    const int items = R * (row0 + 1) + R * (R - 1) / 2;
+    // items (r, col), c0 <= col <= row0 + r.  Left of c0, six times the first
+    // covering group's anchor (the smallest), the rows are zero in H and stay
+    // zero: H is filled with zeros when allocated, and no factorisation writes
+    // outside the profile (the dense one keeps its exact zeros there).
+    const int c0 = nc > 0 ? min(6 * canc[0] - cbase, row0) : row0;
+    const int w0 = row0 - c0 + 1;   // items of the first row
+    const int items = R * w0 + R * (R - 1) / 2;
     for (int t = tid; t < items; t += 256) {
         int r = 0;
-        while (r + 1 < R && t >= (r + 1) * (row0 + 1) + (r + 1) * r / 2) ++r;
-        const int col = t - (r * (row0 + 1) + r * (r - 1) / 2);
+        while (r + 1 < R && t >= (r + 1) * w0 + (r + 1) * r / 2) ++r;
+        const int col = c0 + t - (r * w0 + r * (r - 1) / 2);
         const int row = row0 + r, row_g = row_g0 + r;
         const int colg = col + cbase, bj = colg / 6;
         // covering groups with anchor <= bj form a prefix (anchors ascending):
