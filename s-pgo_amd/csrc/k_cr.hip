@@ -289,6 +289,13 @@ template <int P>
 __device__ __forceinline__ bool cr_factor(double (*A)[16 * P + 1], double (*Wl)[16 * P + 1]) {
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     (void)tid;
+    // the diagonal factor's per-lane factors (r > j) and (r == j), formed once
+    double gtv[16], eqv[16];
+    static_for<16>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        gtv[j] = (lane & 15) > j ? 1.0 : 0.0;
+        eqv[j] = (lane & 15) == j ? 1.0 : 0.0;
+    });
     bool bad = false;
     // Step k: wavefront 0 factors the diagonal block k while the others form W's
     // block row k - 1 and the trailing update of step k - 1 (all but the next
@@ -317,7 +324,6 @@ __device__ __forceinline__ bool cr_factor(double (*A)[16 * P + 1], double (*Wl)[
             const double c375 = 0.375;
             static_for<16>([&](auto J) {
                 constexpr int j = decltype(J)::value;
-                const double gtj = r > j ? 1.0 : 0.0, eqj = r == j ? 1.0 : 0.0;
                 double m, d, y0, t, e, inv, mi;
                 asm volatile((ColumnAsm<j>::text)
                              : "+v"(av[0]), "+v"(av[1]), "+v"(av[2]), "+v"(av[3]), "+v"(av[4]), "+v"(av[5]),
@@ -325,7 +331,7 @@ __device__ __forceinline__ bool cr_factor(double (*A)[16 * P + 1], double (*Wl)[
                                "+v"(av[12]), "+v"(av[13]), "+v"(av[14]), "+v"(av[15]), "+v"(yv[0]), "+v"(yv[1]),
                                "+v"(yv[2]), "+v"(yv[3]), "=&v"(m), "=&v"(d), "=&v"(y0), "=&v"(t), "=&v"(e),
                                "=&v"(inv), "=&v"(mi), "+v"(ivr)
-                             : "v"(mprev), "v"(c375), "v"(gtj), "v"(eqj));
+                             : "v"(mprev), "v"(c375), "v"(gtv[j]), "v"(eqv[j]));
                 mprev = m;
             });
             // every pivot positive and finite: 1/sqrt of it is too (ivr of lane r
