@@ -23,8 +23,8 @@
 // super-row: D_i = L L^T, W = L^-1, and U, V, y as 16 x 16 FP64 MFMA tiles)
 // and k_cr_update (the survivors' new D, coupling and b as MFMA tiles).  Once
 // at most three super-rows are left, k_cr_tail solves them in one workgroup as
-// a block-tridiagonal Cholesky, and one k_cr_back per level brings the
-// solution back down.  Level 0 reads H and g in place (the lower triangle);
+// a block-tridiagonal Cholesky, and one k_cr_back launch brings the solution
+// back down through the levels (workgroups handing solutions over by flags).  Level 0 reads H and g in place (the lower triangle);
 // later levels read the workspace.  Rows past the padded size are identity.
 //
 // k_cr_factor factors D right-looking over 16-wide block columns.  The
@@ -231,9 +231,37 @@ struct CrArgs {
     double* V;         // per eliminated super-row: W E_q^T
     double* b;         // per super-row: current right-hand side (S)
     double* y;         // per eliminated super-row: W b_i (S)
+    int* sync;         // back-substitution: [0] ticket, [1 + r] solution of super-row r ready
     Control* ctl;
     int Pp, ns;
 };
+
+// Solutions and ready flags pass between the back-substitution's workgroups by
+// relaxed device-scope atomics, coherent across the XCDs' L2 caches without an
+// acquire or release (which would write back or invalidate a whole L2): a row
+// waits until its solution stores have completed before it raises its flag,
+// and a reader loads the solution after it has seen the flag.
+__device__ __forceinline__ double xget(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xput(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+constexpr int CR_SPIN_MAX = 1 << 20;
+// wait for a ready flag; past the bound (or once the Cholesky flag is down) the
+// solution is discarded: the flag goes down and nothing waits any longer
+__device__ __forceinline__ void cr_wait(const int* flag, Control* ctl) {
+    volatile int* ok = &ctl->chol_ok;
+    int n = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        if (*ok == 0) break;
+        if (++n > CR_SPIN_MAX) {
+            *ok = 0;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
 
 // Element (r, c) of super-block (I, J) of H, identity past the padded size.
 // Branch-free: the address is clamped into H and the value selected after the
@@ -568,6 +596,8 @@ __global__ __launch_bounds__(256) void k_cr_tail(CrArgs a, int L) {
     if (a.ctl->pcg_fail == 0) return;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int t = (a.ns + (1 << L) - 1) >> L;
+    // the back-substitution's ticket and flags: the rows solved here are ready
+    for (int r = tid; r <= a.ns; r += 256) a.sync[r] = r == 0 ? 0 : (((r - 1) & ((1 << L) - 1)) == 0 ? 1 : 0);
     const bool fromH = L == 0;
     const long long SS = (long long)S * S;
     const int row = tid >> 2, part = tid & 3;
@@ -846,18 +876,34 @@ __global__ __launch_bounds__(256) void k_cr_update(CrArgs a, int level) {
     if (h == 0) a.b[(long long)q * S + t] = s + half[t];
 }
 
-// x_i = W_i^T (y_i - U_i x_p - V_i x_q) for the super-rows eliminated at this
-// level: U, V and W staged in LDS by coalesced loads (a thread's row-wise
-// global reads would touch a cache line per lane), then four threads per row
-// (a quarter of the columns each)
+// x_i = W_i^T (y_i - U_i x_p - V_i x_q) for every super-row eliminated by the
+// levels, one workgroup each, in one launch.  A workgroup takes its row from a
+// ticket counter, the highest level first, so it only waits on rows whose
+// workgroups started before it: the sweep cannot deadlock, whatever the number
+// of resident workgroups.  It stages its U, V, W (coalesced loads into LDS)
+// and y before it waits for the solutions of its two neighbours (one level
+// up, or rows of the tail), then publishes its own.
 template <int P>
-__global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int level) {
+__global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int L) {
     constexpr int S = 16 * P, SP = S + 1, Q = S / 4, NE = S * S / 256;
     __shared__ double Us[S][SP], Vs[S][SP], Ws[S][SP];
     __shared__ double xp[S], xq[S], z[S], yv[S];
+    __shared__ int srow[2];
     if (a.ctl->pcg_fail == 0) return;
     const int tid = threadIdx.x;
-    const int st = 1 << level, i = (2 * blockIdx.x + 1) << level, p = i - st, q = i + st;
+    if (tid == 0) {
+        int tk = __hip_atomic_fetch_add(a.sync, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int l = L - 1;
+        for (; l > 0; --l) {
+            const int st = 1 << l, n = (a.ns - st + 2 * st - 1) / (2 * st);
+            if (tk < n) break;
+            tk -= n;
+        }
+        srow[0] = (2 * tk + 1) << l;
+        srow[1] = l;
+    }
+    __syncthreads();
+    const int i = srow[0], st = 1 << srow[1], p = i - st, q = i + st;
     const long long SS = (long long)S * S;
     const bool has_q = q < a.ns;
     {
@@ -868,13 +914,7 @@ __global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int level) {
             lv[u] = has_q ? a.V[i * SS + tid + 256 * u] : 0.0;
             lw[u] = a.W[i * SS + tid + 256 * u];
         }
-        if (tid < S) {
-            xp[tid] = a.x[S * p + tid];   // rows of p lie below S i <= Pp
-            const int gq = S * q + tid;
-            const double v = a.x[min(gq, a.Pp - 1)];
-            xq[tid] = (has_q && gq < a.Pp) ? v : 0.0;
-            yv[tid] = a.y[(long long)i * S + tid];
-        }
+        if (tid < S) yv[tid] = a.y[(long long)i * S + tid];
 #pragma unroll
         for (int u = 0; u < NE; ++u) {
             const int e = tid + 256 * u, r = e / S, c = e - (e / S) * S;
@@ -882,6 +922,15 @@ __global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int level) {
             Vs[r][c] = lv[u];
             Ws[r][c] = lw[u];
         }
+    }
+    if (tid == 0) cr_wait(a.sync + 1 + p, a.ctl);
+    if (tid == 64 && has_q) cr_wait(a.sync + 1 + q, a.ctl);
+    __syncthreads();
+    if (tid < S) {
+        xp[tid] = xget(a.x + S * p + tid);   // rows of p lie below S i <= Pp
+        const int gq = S * q + tid;
+        const double v = xget(a.x + min(gq, a.Pp - 1));
+        xq[tid] = (has_q && gq < a.Pp) ? v : 0.0;
     }
     __syncthreads();
     const int row = tid >> 2, pt = tid & 3;
@@ -906,15 +955,21 @@ __global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int level) {
         s += __shfl_xor(s, 1);
         s += __shfl_xor(s, 2);
         const int gr = S * i + row;
-        if (pt == 0 && gr < a.Pp) a.x[gr] = s;
+        if (pt == 0 && gr < a.Pp) xput(a.x + gr, s);
     }
+    // every wavefront's solution stores complete, then the flag
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(a.sync + 1 + i, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 int cr_max_depth() { return 4; }
 
 size_t cr_work_doubles(int Pp, int D) {
     const size_t S = 16 * (size_t)D, ns = (Pp + S - 1) / S;
-    return ns * (5 * S * S + 2 * S);
+    return ns * (5 * S * S + 2 * S) + (ns + 2) / 2 + 1;   // the blocks, then the back-substitution's flags
 }
 
 // super-rows left to the sequential tail: each level of the reduction costs
@@ -933,11 +988,12 @@ static void launch_cr_p(const CrArgs& a, hipStream_t s) {
         hipLaunchKernelGGL(k_cr_update<P>, dim3(n_surv, 2 * P + 1), dim3(256), 0, s, a, l);
     }
     hipLaunchKernelGGL(k_cr_tail<P>, dim3(1), dim3(256), 0, s, a, L);
-    for (int l = L - 1; l >= 0; --l) {
+    int n_back = 0;
+    for (int l = 0; l < L; ++l) {
         const int st = 1 << l;
-        const int n_elim = (a.ns - st + 2 * st - 1) / (2 * st);
-        hipLaunchKernelGGL(k_cr_back<P>, dim3(n_elim), dim3(256), 0, s, a, l);
+        n_back += (a.ns - st + 2 * st - 1) / (2 * st);
     }
+    if (n_back > 0) hipLaunchKernelGGL(k_cr_back<P>, dim3(n_back), dim3(256), 0, s, a, L);
 }
 
 void launch_cr(const double* H, const double* g, double* x, int Pp, int D, double* work, Control* ctl,
@@ -958,6 +1014,7 @@ void launch_cr(const double* H, const double* g, double* x, int Pp, int D, doubl
     a.V = a.U + ns * S * S;
     a.b = a.V + ns * S * S;
     a.y = a.b + ns * S;
+    a.sync = reinterpret_cast<int*>(a.y + ns * S);
     switch (D) {
     case 1: launch_cr_p<1>(a, s); break;
     case 2: launch_cr_p<2>(a, s); break;
