@@ -782,6 +782,7 @@ __global__ __launch_bounds__(256) void k_cr_update(CrArgs a, int level) {
     const bool fromH = level == 0;
     const long long SS = (long long)S * S;
     const int part = blockIdx.y;
+    if (blockIdx.x == 0 && part == 0 && tid < 2) a.sync[a.ns + 1 + tid] = 0;   // k_cr_tail3's flag and ticket
     if (part < 2 * P) {
         const bool isE = part >= P;
         const int r = isE ? part - P : part, c = w;
@@ -965,11 +966,284 @@ __global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int L) {
     if (tid == 0) __hip_atomic_store(a.sync + 1 + i, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Helpers of the three-row tail (super-block tiles in LDS, 16 x 16 MFMA tiles).
+// C = X^T X (lower tiles, the diagonal ones whole), wavefront w taking tiles
+// w, w + 4, ...
+template <int P>
+__device__ __forceinline__ void cr_gram(double (*X)[16 * P + 1], double (*C)[16 * P + 1], int w, int lane) {
+    constexpr int S = 16 * P;
+    for (int tt = w; tt < P * (P + 1) / 2; tt += 4) {
+        int r = 0;
+        while ((r + 1) * (r + 2) / 2 <= tt) ++r;
+        const int c = tt - r * (r + 1) / 2;
+        double xa[S / 4], xc[S / 4];
+#pragma unroll
+        for (int s4 = 0; s4 < S / 4; ++s4) {
+            xa[s4] = X[4 * s4 + (lane >> 4)][16 * r + (lane & 15)];
+            xc[s4] = X[4 * s4 + (lane >> 4)][16 * c + (lane & 15)];
+        }
+        d4 acc = 0.0;
+#pragma unroll
+        for (int s4 = 0; s4 < S / 4; ++s4) acc = mfma(xa[s4], xc[s4], acc);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) C[16 * r + (lane >> 4) + 4 * v][16 * c + (lane & 15)] = acc[v];
+    }
+}
+// X = W E (tr = false) or W E^T (tr = true) in place of E in Eb, wavefront w
+// taking tile column w (its reads of Eb's column block, or row block, come
+// before its writes of column block w); W lower triangular
+template <int P>
+__device__ __forceinline__ void cr_times_w(double (*Wl)[16 * P + 1], double (*Eb)[16 * P + 1], bool tr, int w,
+                                           int lane) {
+    if (w >= P) return;
+    const int c = w, n = 16 * c + (lane & 15);
+    d4 acc[P];
+    static_for<P>([&](auto Rr) {
+        constexpr int r = decltype(Rr)::value;
+        double wa[r + 1][4], eb[r + 1][4];
+#pragma unroll
+        for (int m = 0; m <= r; ++m)
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int kk = 16 * m + 4 * s4 + (lane >> 4);
+                wa[m][s4] = Wl[16 * r + (lane & 15)][kk];
+                eb[m][s4] = tr ? Eb[n][kk] : Eb[kk][n];
+            }
+        acc[r] = 0.0;
+#pragma unroll
+        for (int m = 0; m <= r; ++m)
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) acc[r] = mfma(wa[m][s4], eb[m][s4], acc[r]);
+    });
+    if (tr) return;   // W E^T: the caller takes it from registers (rows of Eb are still read)
+    wave_lds_order();
+#pragma unroll
+    for (int r = 0; r < P; ++r)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Eb[16 * r + (lane >> 4) + 4 * v][n] = acc[r][v];
+}
+// out = M^T v (tr) or M v, four threads per entry
+template <int P>
+__device__ __forceinline__ double cr_matvec(double (*M)[16 * P + 1], const double* v, bool tr, int tid) {
+    constexpr int S = 16 * P, Q = S / 4;
+    const int row = tid >> 2, pt = tid & 3;
+    double s = 0.0;
+    if (row < S) {
+#pragma unroll
+        for (int j = 0; j < Q; ++j) {
+            const int k = Q * pt + j;
+            s = fma(tr ? M[k][row] : M[row][k], v[k], s);
+        }
+    }
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    return s;
+}
+
+// Three super-rows left (r0 = 0, r1 = 2^L, r2 = 2^(L+1)): the two ends are
+// eliminated concurrently, then the middle one, so only two factorisations are
+// sequential.  Two workgroups, by ticket:
+//   ticket 0: D_0 = L_0 L_0^T, W_0, V_0 = W_0 E_1^T, y_0 = W_0 b_0, and the
+//             middle row's share C_0 = V_0^T V_0, c_0 = V_0^T y_0; all of it
+//             published, then a ready flag;
+//   ticket 1: D_2 = L_2 L_2^T, W_2, U_2 = W_2 E_2, y_2, C_2 = U_2^T U_2,
+//             c_2 = U_2^T y_2; waits for ticket 0 (started before it, so the
+//             wait ends whatever the residency); factors D_1 - C_0 - C_2, solves
+//             x_1 = W_1^T W_1 (b_1 - c_0 - c_2), then x_2 = W_2^T (y_2 - U_2 x_1)
+//             and x_0 = W_0^T (y_0 - V_0 x_1).
+// What ticket 0 hands over moves by device-coherent atomics (xput / xget).
+// sync[ns + 1] (flag) and sync[ns + 2] (ticket) are cleared by k_cr_update.
+template <int P>
+__global__ __launch_bounds__(256) void k_cr_tail3(CrArgs a, int L) {
+    constexpr int S = 16 * P, SP = S + 1, NE = S * S / 256;
+    __shared__ double A[S][SP], Wl[S][SP], Mb[S][SP];
+    __shared__ double vb[S], vy[S], vc[S], vx[S], vz[S];
+    __shared__ int stk;
+    if (a.ctl->pcg_fail == 0) return;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const long long SS = (long long)S * S;
+    const int r0 = 0, r1 = 1 << L, r2 = 2 << L;
+    int* flag = a.sync + a.ns + 1;
+    if (tid == 0) stk = __hip_atomic_fetch_add(a.sync + a.ns + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int tk = stk;
+    const int e = tk == 0 ? r0 : r2;              // this workgroup's end row
+    const int en = tk == 0 ? r1 : r2;             // the coupling block: E of r1 (to r0) or of r2 (to r1)
+    // D_e, the coupling, b_e
+    {
+        double le[NE];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) le[u] = a.E[en * SS + tid + 256 * u];
+        cr_load_block<P>(a, e, false, A, Wl);
+        if (tid < S) vb[tid] = a.b[(long long)e * S + tid];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int q = tid + 256 * u, r = q / S, c = q - (q / S) * S;
+            Mb[r][c] = le[u];
+        }
+    }
+    __syncthreads();
+    bool bad = cr_factor<P>(A, Wl);
+    __syncthreads();
+    {
+        const double y = cr_matvec<P>(Wl, vb, false, tid);
+        if ((tid & 3) == 0 && (tid >> 2) < S) vy[tid >> 2] = y;
+    }
+    if (tk == 0) {
+        // V_0 = W_0 E_1^T from registers into A (L_0 is no longer needed)
+        if (w < P) {
+            const int c = w, n = 16 * c + (lane & 15);
+            static_for<P>([&](auto Rr) {
+                constexpr int r = decltype(Rr)::value;
+                double wa[r + 1][4], eb[r + 1][4];
+#pragma unroll
+                for (int m = 0; m <= r; ++m)
+#pragma unroll
+                    for (int s4 = 0; s4 < 4; ++s4) {
+                        const int kk = 16 * m + 4 * s4 + (lane >> 4);
+                        wa[m][s4] = Wl[16 * r + (lane & 15)][kk];
+                        eb[m][s4] = Mb[n][kk];
+                    }
+                d4 acc = 0.0;
+#pragma unroll
+                for (int m = 0; m <= r; ++m)
+#pragma unroll
+                    for (int s4 = 0; s4 < 4; ++s4) acc = mfma(wa[m][s4], eb[m][s4], acc);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) A[16 * r + (lane >> 4) + 4 * v][n] = acc[v];
+            });
+        }
+        __syncthreads();
+        cr_gram<P>(A, Mb, w, lane);   // C_0 into Mb
+        {
+            const double c0 = cr_matvec<P>(A, vy, true, tid);
+            if ((tid & 3) == 0 && (tid >> 2) < S) vc[tid >> 2] = c0;
+        }
+        __syncthreads();
+        for (int q = tid; q < S * S; q += 256) {
+            const int r = q / S, c = q - (q / S) * S;
+            xput(a.W + r0 * SS + q, Wl[r][c]);
+            xput(a.V + r0 * SS + q, A[r][c]);
+            xput(a.U + r0 * SS + q, Mb[r][c]);
+        }
+        if (tid < S) {
+            xput(a.y + (long long)r0 * S + tid, vy[tid]);
+            xput(a.b + (long long)r0 * S + tid, vc[tid]);
+        }
+        if (bad && tid == 0) a.ctl->chol_ok = 0;
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    // ticket 1: U_2 = W_2 E_2 in place in Mb, then C_2 = U_2^T U_2 into A, c_2
+    cr_times_w<P>(Wl, Mb, false, w, lane);
+    __syncthreads();
+    cr_gram<P>(Mb, A, w, lane);
+    {
+        const double c2 = cr_matvec<P>(Mb, vy, true, tid);
+        if ((tid & 3) == 0 && (tid >> 2) < S) vc[tid >> 2] = c2;
+    }
+    // W_2 waits in its row's workspace; the back-substitution's ticket and flags
+    // (the rows solved here are ready)
+    for (int q = tid; q < S * S; q += 256) {
+        const int r = q / S, c = q - (q / S) * S;
+        a.W[r2 * SS + q] = Wl[r][c];
+    }
+    for (int r = tid; r <= a.ns; r += 256) a.sync[r] = r == 0 ? 0 : (((r - 1) & ((1 << L) - 1)) == 0 ? 1 : 0);
+    if (tid < S) vz[tid] = vy[tid];   // y_2
+    if (tid == 0) cr_wait(flag, a.ctl);
+    __syncthreads();
+    // D_1 - C_0 - C_2 into A, b_1 - c_0 - c_2 into vb
+    {
+        double ld[NE], lc[NE];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            ld[u] = a.D[r1 * SS + tid + 256 * u];
+            lc[u] = xget(a.U + r0 * SS + tid + 256 * u);
+        }
+        if (tid < S) vb[tid] = a.b[(long long)r1 * S + tid] - xget(a.b + (long long)r0 * S + tid) - vc[tid];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int q = tid + 256 * u, r = q / S, c = q - (q / S) * S;
+            A[r][c] = (c >> 4) <= (r >> 4) ? ld[u] - lc[u] - A[r][c] : 0.0;
+            Wl[r][c] = 0.0;
+        }
+    }
+    __syncthreads();
+    bad = cr_factor<P>(A, Wl) || bad;
+    __syncthreads();
+    {
+        const double y1 = cr_matvec<P>(Wl, vb, false, tid);
+        if ((tid & 3) == 0 && (tid >> 2) < S) vy[tid >> 2] = y1;
+    }
+    __syncthreads();
+    {
+        const double x1 = cr_matvec<P>(Wl, vy, true, tid);
+        if ((tid & 3) == 0 && (tid >> 2) < S) {
+            vx[tid >> 2] = x1;
+            const int gr = S * r1 + (tid >> 2);
+            if (gr < a.Pp) a.x[gr] = x1;
+        }
+    }
+    __syncthreads();
+    // x_2 = W_2^T (y_2 - U_2 x_1): W_2 back into Wl (U_2 is still in Mb)
+    {
+        double lw[NE];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) lw[u] = a.W[r2 * SS + tid + 256 * u];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int q = tid + 256 * u, r = q / S, c = q - (q / S) * S;
+            Wl[r][c] = lw[u];
+        }
+        const double ux = cr_matvec<P>(Mb, vx, false, tid);
+        if ((tid & 3) == 0 && (tid >> 2) < S) vz[tid >> 2] -= ux;
+    }
+    __syncthreads();
+    {
+        const double x2 = cr_matvec<P>(Wl, vz, true, tid);
+        const int gr = S * r2 + (tid >> 2);
+        if ((tid & 3) == 0 && (tid >> 2) < S && gr < a.Pp) a.x[gr] = x2;
+    }
+    __syncthreads();
+    // x_0 = W_0^T (y_0 - V_0 x_1)
+    {
+        double lw[NE], lv[NE];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            lw[u] = xget(a.W + r0 * SS + tid + 256 * u);
+            lv[u] = xget(a.V + r0 * SS + tid + 256 * u);
+        }
+        if (tid < S) vz[tid] = xget(a.y + (long long)r0 * S + tid);
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int q = tid + 256 * u, r = q / S, c = q - (q / S) * S;
+            Wl[r][c] = lw[u];
+            Mb[r][c] = lv[u];
+        }
+    }
+    __syncthreads();
+    {
+        const double vx1 = cr_matvec<P>(Mb, vx, false, tid);
+        if ((tid & 3) == 0 && (tid >> 2) < S) vz[tid >> 2] -= vx1;
+    }
+    __syncthreads();
+    {
+        const double x0 = cr_matvec<P>(Wl, vz, true, tid);
+        const int gr = S * r0 + (tid >> 2);
+        if ((tid & 3) == 0 && (tid >> 2) < S && gr < a.Pp) a.x[gr] = x0;
+    }
+    if (bad && tid == 0) a.ctl->chol_ok = 0;
+}
+
 int cr_max_depth() { return 4; }
 
 size_t cr_work_doubles(int Pp, int D) {
     const size_t S = 16 * (size_t)D, ns = (Pp + S - 1) / S;
-    return ns * (5 * S * S + 2 * S) + (ns + 2) / 2 + 1;   // the blocks, then the back-substitution's flags
+    return ns * (5 * S * S + 2 * S) + (ns + 3) / 2 + 1;   // the blocks, then the flags and tickets
 }
 
 // super-rows left to the sequential tail: each level of the reduction costs
@@ -987,7 +1261,10 @@ static void launch_cr_p(const CrArgs& a, hipStream_t s) {
         hipLaunchKernelGGL(k_cr_factor<P>, dim3(n_elim), dim3(256), 0, s, a, l);
         hipLaunchKernelGGL(k_cr_update<P>, dim3(n_surv, 2 * P + 1), dim3(256), 0, s, a, l);
     }
-    hipLaunchKernelGGL(k_cr_tail<P>, dim3(1), dim3(256), 0, s, a, L);
+    if (L >= 1 && ((a.ns + (1 << L) - 1) >> L) == 3)
+        hipLaunchKernelGGL(k_cr_tail3<P>, dim3(2), dim3(256), 0, s, a, L);
+    else
+        hipLaunchKernelGGL(k_cr_tail<P>, dim3(1), dim3(256), 0, s, a, L);
     int n_back = 0;
     for (int l = 0; l < L; ++l) {
         const int st = 1 << l;
