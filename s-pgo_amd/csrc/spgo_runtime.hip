@@ -587,9 +587,11 @@ int setup_layout(spgo_solver* h) {
             if (prepare_chol_split(h->Pp, bw) != 0)
                 return h->fail(SPGO_E_DEVICE, "cannot reserve LDS for the split envelope Cholesky");
         }
-        if (h->path == SPGO_FACTOR_CR &&
-            !(h->crw = h->mem.alloc<double>(cr_work_doubles(h->Pp, std::max(bw, 1)))))
-            return h->fail(SPGO_E_DEVICE, "device allocation failed");
+        if (h->path == SPGO_FACTOR_CR) {
+            const size_t n = cr_work_doubles(h->Pp, std::max(bw, 1));
+            if (!(h->crw = h->mem.alloc<double>(n))) return h->fail(SPGO_E_DEVICE, "device allocation failed");
+            launch_fill(h->crw, 0.0, (long long)n, h->stream);   // the reduction's counters start at zero
+        }
     }
     h->H = h->mem.alloc<double>((size_t)h->Pp * h->Pp);
     h->gvec = h->mem.alloc<double>(h->Pp);
