@@ -4,8 +4,9 @@
 // global chain G_0 = I, G_{k+1} = G_k X_k^-1, G^R = G T_r_to_l
 // (Solver_cuda.cu:440-493), computed as a parallel prefix product instead of
 // the reference's single-thread loop: every thread multiplies its chunk of
-// poses, a Hillis-Steele scan over the chunk products runs in LDS, and each
-// thread re-walks its chunk from its exclusive prefix.
+// poses, the chunk products are scanned across each wavefront by shuffles and
+// across the wavefronts in LDS, and each thread re-walks its chunk from its
+// exclusive prefix (with one pose per thread the prefix is the result).
 // k_accept: robust cost of the candidate (sum of the group partials in a fixed
 // order), accept if it decreased, lambda update, per-iteration record.
 #include "spgo_device.h"
@@ -18,7 +19,7 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
                                                     const Control* ctls, const double* delta, double step,
                                                     int init) {
     // one workgroup per problem: its N poses start at pose_base, its frames at frame_base
-    __shared__ double scan[2][WG][12];
+    __shared__ double wtot[WG / 64][12];   // per wavefront: the product of its chunks
     const int tid = threadIdx.x;
     const ProbDesc pd = lay.probs[blockIdx.x];
     const Control* ctl = ctls + blockIdx.x;
@@ -54,7 +55,6 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
     const int nact = (N + chunk - 1) / chunk;   // threads holding poses
     const int k0 = min(N, tid * chunk), k1 = min(N, k0 + chunk);
     double P[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
-    double Xkeep[12];   // with one pose per thread it stays in registers for the second pass
     for (int k = k0; k < k1; ++k) {
         double X[12], Xi[12], tmp[12];
         if (one) {
@@ -88,35 +88,52 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
                 bout.X[12 * (pd.pose_base + k) + q] = tmp[q];
             }
         }
-#pragma unroll
-        for (int q = 0; q < 12; ++q) Xkeep[q] = X[q];
         pose_inv(X, Xi);
         pose_mul(P, Xi, tmp);
 #pragma unroll
         for (int q = 0; q < 12; ++q) P[q] = tmp[q];
     }
+    // inclusive scan of the chunk products over the workgroup: a Kogge-Stone
+    // scan across each wavefront's lanes by shuffles (no LDS, no barriers), then
+    // the wavefront totals, combined in LDS.  Threads past the last chunk hold
+    // the identity.
+    const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
-    for (int q = 0; q < 12; ++q) scan[0][tid][q] = P[q];
-    __syncthreads();
-    int src = 0;
-    for (int d = 1; d < nact; d <<= 1) {
-        double out[12];
-        if (tid >= d) {
-            pose_mul(scan[src][tid - d], scan[src][tid], out);
-        } else {
+    for (int d = 1; d < 64; d <<= 1) {
+        double Q[12], tmp[12];
 #pragma unroll
-            for (int q = 0; q < 12; ++q) out[q] = scan[src][tid][q];
+        for (int q = 0; q < 12; ++q) Q[q] = __shfl_up(P[q], d, 64);
+        if (lane >= d) {
+            pose_mul(Q, P, tmp);
+#pragma unroll
+            for (int q = 0; q < 12; ++q) P[q] = tmp[q];
         }
+    }
+    if (lane == 63) {
 #pragma unroll
-        for (int q = 0; q < 12; ++q) scan[src ^ 1][tid][q] = out[q];
-        src ^= 1;
-        __syncthreads();
+        for (int q = 0; q < 12; ++q) wtot[wv][q] = P[q];
+    }
+    __syncthreads();
+    // product of the totals of the wavefronts before this one
+    double W[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    for (int w = 0; w < wv; ++w) {
+        double tmp[12];
+        pose_mul(W, wtot[w], tmp);
+#pragma unroll
+        for (int q = 0; q < 12; ++q) W[q] = tmp[q];
+    }
+    if (wv > 0) {
+        double tmp[12];
+        pose_mul(W, P, tmp);
+#pragma unroll
+        for (int q = 0; q < 12; ++q) P[q] = tmp[q];
     }
     // exclusive prefix = inclusive scan of the previous thread
-    double G[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
-    if (tid > 0) {
+    double G[12];
 #pragma unroll
-        for (int q = 0; q < 12; ++q) G[q] = scan[src][tid - 1][q];
+    for (int q = 0; q < 12; ++q) {
+        const double up = __shfl_up(P[q], 1, 64);
+        G[q] = lane > 0 ? up : W[q];
     }
     double Trl[12];
 #pragma unroll
@@ -135,14 +152,14 @@ __global__ __launch_bounds__(WG) void k_pose_update(Layout lay, Camera cam, Stat
         }
     };
     if (tid == 0) emit(0, G);
+    if (chunk == 1) {
+        // one pose per thread: G_{k+1} is this thread's inclusive prefix
+        if (k0 < k1) emit(k0 + 1, P);
+        return;
+    }
     for (int k = k0; k < k1; ++k) {
         double X[12], Xi[12], tmp[12];
-        if (chunk == 1) {
-#pragma unroll
-            for (int q = 0; q < 12; ++q) X[q] = Xkeep[q];
-        } else {
-            pose_load(bout.X + 12 * (pd.pose_base + k), X);   // this thread's own stores: program order
-        }
+        pose_load(bout.X + 12 * (pd.pose_base + k), X);   // this thread's own stores: program order
         pose_inv(X, Xi);
         pose_mul(G, Xi, tmp);
 #pragma unroll
