@@ -64,7 +64,7 @@ __global__ __launch_bounds__(WG, (KT <= 12 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 
     const int K = gdp->K;
     const int prob = gdp->prob, a_loc = gdp->a_loc;
     const int chunk_begin = gdp->chunk_begin, chunk_end = gdp->chunk_end;
-    const int glm0 = gdp->lm_begin, d_off = gdp->d_off;
+    const int glm0 = gdp->lm_begin, d_off = gdp->d_off, gid = gdp->gid;
     const long long voff = gdp->v_off, D_off = gdp->D_off, delta_off = gdp->delta_off;
     const int4 chunk0 = gdp->chunk0;
     const Control* ctl = A.ctl + prob;          // the group's problem (a batch holds several)
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(WG, (KT <= 12 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 
     if (tid == 0) {
         double c = 0.0;
         for (int w = 0; w < NWAVE; ++w) c += red[w];
-        bout.cost[g] = c;
+        bout.cost[gid] = c;
     }
 }
 

@@ -63,11 +63,11 @@ __global__ __launch_bounds__(WG, KT <= 16 ? 2 : 1) void k_assemble_frames(AsmArg
     const int a = gdp->anchor;
     const int m = gdp->K;                      // target slots 1..m
     const int prob = gdp->prob;
-    const int glm0 = gdp->lm_begin, d_off = gdp->d_off;
+    const int glm0 = gdp->lm_begin, d_off = gdp->d_off, gid = gdp->gid;
     const long long voff = gdp->v_off, D_off = gdp->D_off, delta_off = gdp->delta_off, r_off = gdp->r_off;
     const int4 ch = gdp->chunk0;
     const int lm0 = ch.x, nl = ch.y, ob0 = ch.z;
-    const int* frames = lay.grp_frames + (long long)g * (KTF + 1);
+    const int* frames = lay.grp_frames + (long long)gid * (KTF + 1);
     const int* soff = lay.chunk_soff + (long long)gdp->chunk_begin * lay.soff_stride;
     const int NF = lay.N;                      // z of frame N is zero
     const Control* ctl = A.ctl + prob;
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(WG, KT <= 16 ? 2 : 1) void k_assemble_frames(AsmArg
     if (tid == 0) {
         double c = 0.0;
         for (int w = 0; w < NWAVE; ++w) c += red[w];
-        bout.cost[g] = c;
+        bout.cost[gid] = c;
     }
     (void)glm0;
 }

@@ -331,7 +331,7 @@ int setup(spgo_batch* b) {
         (rc = bupload(b, C.ob_uv, &lay.ob_uv)) || (rc = bupload(b, C.lm_id, &lay.lm_id)) ||
         (rc = bupload(b, C.lm_ray, &lay.lm_ray)) || (rc = bupload(b, C.lm_pos, &lay.lm_pos)) ||
         (rc = bupload(b, C.chunk, &lay.chunk)) || (rc = bupload(b, C.chunk_soff, &lay.chunk_soff)) ||
-        (rc = bupload(b, C.grp_desc, &lay.grp_desc)) || (rc = bupload(b, C.fs_begin, &lay.fs_begin)) ||
+        (rc = bupload(b, launch_order(C.grp_desc), &lay.grp_desc)) || (rc = bupload(b, C.fs_begin, &lay.fs_begin)) ||
         (rc = bupload(b, C.fs_list, &lay.fs_list)) || (rc = bupload(b, pose_prob, &lay.pose_prob)) ||
         (rc = bupload(b, first_all, &b->first)) || (rc = bupload(b, b->pd, &probs)))
         return rc;

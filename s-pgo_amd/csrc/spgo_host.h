@@ -178,6 +178,9 @@ void global_chain(int N, const double* X, double* G);
 void frames_to_incremental(int N, const double* Z, int ld, const double* gz, const double* G, double* H, double* g);
 // rows of the padded reduced pose system of n_poses poses
 int reduced_size(int n_poses);
+// the assembly groups in launch order, each with gid = its place in `in`
+// (see spgo_runtime.hip)
+std::vector<GroupDesc> launch_order(const std::vector<GroupDesc>& in);
 // block profile of the reduced system: fb[i] first nonzero block column of
 // block row i; returns the depth max(i - fb[i])
 int profile_blocks(int n_poses, const HostLayout& L, std::vector<int>& fb);

@@ -26,6 +26,7 @@ struct alignas(16) GroupDesc {
     long long r_off;                         // frame coordinates: the group's anchor blocks in Rz
     int4 chunk0;                  // (first active landmark, n landmarks, first obs, n obs)
     int soff0[KMAX + 2];          // slot offsets of the first chunk
+    int gid;                      // index of the group's cost partial (its place in the layout)
 };
 
 // One landmark group whose pose span covers a given pose row block (the

@@ -331,6 +331,35 @@ int build_layout(spgo_solver* h, int chunk_landmarks, int group_chunks_opt, Host
 // either factorisation.
 int reduced_size(int n_poses) { return ((6 * n_poses + NB - 1) / NB) * NB; }
 
+// Launch order of the assembly groups.  The device hands workgroup i to XCD
+// i mod nx (round-robin dispatch), so the groups are cut into nx contiguous
+// ranges of nearly equal size, which are interleaved: each range (neighbouring
+// anchors; whole windows of a batch) runs on one XCD, whose L2 then serves the
+// poses and updates its groups share instead of every XCD fetching them.  The
+// kernels write their cost partial at gid, the group's place in the layout,
+// so the results do not depend on the order.
+std::vector<GroupDesc> launch_order(const std::vector<GroupDesc>& in) {
+    int dev = 0, cus = 0;
+    int nx = 1;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+        nx = cus % 32 == 0 ? cus / 32 : (cus % 38 == 0 ? cus / 38 : 1);   // 32 CUs per XCD on gfx950
+    const int G = (int)in.size();
+    std::vector<GroupDesc> out(in);
+    for (int g = 0; g < G; ++g) out[g].gid = g;
+    if (nx <= 1 || G < 2 * nx) return out;
+    // range x holds base + (x < rem) groups from start(x); workgroup i = nx s + x
+    // takes element s of range x (the ranges that are one longer come first)
+    const int base = G / nx, rem = G % nx;
+    for (int i = 0; i < G; ++i) {
+        const int x = i % nx, s = i / nx;
+        const int src = x * base + std::min(x, rem) + s;
+        out[i] = in[src];
+        out[i].gid = src;
+    }
+    return out;
+}
+
 // Block profile of the reduced system for the envelope factorisation: pose j
 // couples back to the smallest anchor among the landmark groups covering it,
 // fb[i] is the first nonzero block column of block row i and depth the
@@ -490,7 +519,7 @@ int setup_layout(spgo_solver* h) {
     if ((rc = upload(h, L.ob_t, &lay.ob_t)) || (rc = upload(h, L.ob_meta, &lay.ob_meta)) ||
         (rc = upload(h, L.ob_uv, &lay.ob_uv)) || (rc = upload(h, L.lm_id, &lay.lm_id)) ||
         (rc = upload(h, L.lm_ray, &lay.lm_ray)) || (rc = upload(h, L.lm_pos, &lay.lm_pos)) ||
-        (rc = upload(h, L.chunk, &lay.chunk)) || (rc = upload(h, L.chunk_soff, &lay.chunk_soff)) || (rc = upload(h, L.grp_desc, &lay.grp_desc)) || (rc = upload(h, L.fs_begin, &lay.fs_begin)) || (rc = upload(h, L.fs_list, &lay.fs_list)))
+        (rc = upload(h, L.chunk, &lay.chunk)) || (rc = upload(h, L.chunk_soff, &lay.chunk_soff)) || (rc = upload(h, launch_order(L.grp_desc), &lay.grp_desc)) || (rc = upload(h, L.fs_begin, &lay.fs_begin)) || (rc = upload(h, L.fs_list, &lay.fs_list)))
         return rc;
     h->prec = h->opt.precision;
     if (h->prec == SPGO_PREC_MIXED)
