@@ -234,6 +234,7 @@ struct CrArgs {
     int* sync;         // back-substitution: [0] ticket, [1 + r] solution of super-row r ready
     Control* ctl;
     int Pp, ns;
+    int gate;          // 1: a CG solve may have solved this iteration (return when ctl->pcg_fail is 0)
 };
 
 // Solutions and ready flags pass between the back-substitution's workgroups by
@@ -484,7 +485,7 @@ __device__ __forceinline__ void cr_load_block(const CrArgs& a, int i, bool fromH
 // One workgroup per super-row i eliminated at this level: D_i = L L^T, W = L^-1,
 // then U_i = W E_i, V_i = W E_q^T (MFMA tiles, wavefront w taking tile column w
 // of both) and y_i = W b_i; W, U, V and y are stored for the survivors' update
-// and the back-substitution.  D_i, E_i, E_q and b_i are loaded at once.
+// and the back-substitution.
 template <int P>
 __global__ __launch_bounds__(256) void k_cr_factor(CrArgs a, int level) {
     constexpr int S = 16 * P, SP = S + 1, NE = S * S / 256, Q = S / 4;
@@ -493,13 +494,29 @@ __global__ __launch_bounds__(256) void k_cr_factor(CrArgs a, int level) {
     __shared__ double Ei[S][SP];   // coupling of i to p = i - 2^l
     __shared__ double Eq[S][SP];   // coupling of q = i + 2^l to i
     __shared__ double vb[S];
-    if (a.ctl->pcg_fail == 0) return;   // this iteration's CG solve converged
+    if (a.gate && a.ctl->pcg_fail == 0) return;   // this iteration's CG solve converged
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int st = 1 << level, i = (2 * blockIdx.x + 1) << level, q = i + st;
     const bool fromH = level == 0, has_q = q < a.ns;
     const long long SS = (long long)S * S;
+    // D_i and b_i first; the coupling blocks E_i, E_q are requested behind them
+    // and stay in registers while D_i is factored, so their loads overlap the
+    // factorisation instead of preceding it
+    double le[NE], lq[NE];
     {
-        double le[NE], lq[NE];
+        constexpr int NB = NE;
+        double ld[NB];
+        if (fromH) {
+#pragma unroll
+            for (int t = 0; t < NB; ++t) {
+                const int e = tid + 256 * t, r = e / S, c = e - (e / S) * S;
+                ld[t] = h_ld<S>(a, i, i, max(r, c), min(r, c));
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < NB; ++t) ld[t] = a.D[(long long)i * SS + tid + 256 * t];
+        }
+        const double bv = tid < S ? (fromH ? g_ld<S>(a, i, tid) : a.b[(long long)i * S + tid]) : 0.0;
         if (fromH) {
 #pragma unroll
             for (int u = 0; u < NE; ++u) {
@@ -514,18 +531,23 @@ __global__ __launch_bounds__(256) void k_cr_factor(CrArgs a, int level) {
                 lq[u] = has_q ? a.E[q * SS + tid + 256 * u] : 0.0;
             }
         }
-        cr_load_block<P>(a, i, fromH, A, Wl);
-        if (tid < S) vb[tid] = fromH ? g_ld<S>(a, i, tid) : a.b[(long long)i * S + tid];
 #pragma unroll
-        for (int u = 0; u < NE; ++u) {
-            const int e = tid + 256 * u, r = e / S, c = e - (e / S) * S;
-            Ei[r][c] = le[u];
-            Eq[r][c] = lq[u];
+        for (int t = 0; t < NB; ++t) {
+            const int e = tid + 256 * t, r = e / S, c = e - (e / S) * S;
+            A[r][c] = (c >> 4) <= (r >> 4) ? ld[t] : 0.0;
+            Wl[r][c] = 0.0;
         }
+        if (tid < S) vb[tid] = bv;
     }
     __syncthreads();
     const bool bad = cr_factor<P>(A, Wl);
     if (bad && tid == 0) a.ctl->chol_ok = 0;
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+        const int e = tid + 256 * u, r = e / S, c = e - (e / S) * S;
+        Ei[r][c] = le[u];
+        Eq[r][c] = lq[u];
+    }
     __syncthreads();
     if (w < P) {
         // tile column c = w of U = W E_i and of V = W E_q^T; W is lower triangular
@@ -593,7 +615,7 @@ __global__ __launch_bounds__(256) void k_cr_tail(CrArgs a, int L) {
     __shared__ double W1[S][SP];    // W of the odd steps
     __shared__ double Mb[S][SP];    // E_k, then M_k
     __shared__ double vb[S], vy[S], vx[S];
-    if (a.ctl->pcg_fail == 0) return;
+    if (a.gate && a.ctl->pcg_fail == 0) return;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int t = (a.ns + (1 << L) - 1) >> L;
     // the back-substitution's ticket and flags: the rows solved here are ready
@@ -774,7 +796,7 @@ __global__ __launch_bounds__(256) void k_cr_tail(CrArgs a, int L) {
 template <int P>
 __global__ __launch_bounds__(256) void k_cr_update(CrArgs a, int level) {
     constexpr int S = 16 * P, K4 = S / 4;
-    if (a.ctl->pcg_fail == 0) return;
+    if (a.gate && a.ctl->pcg_fail == 0) return;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int st = 1 << level, q = (2 * blockIdx.x) << level;
     const int im = q - st, ip = q + st;
@@ -890,7 +912,7 @@ __global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int L) {
     __shared__ double Us[S][SP], Vs[S][SP], Ws[S][SP];
     __shared__ double xp[S], xq[S], z[S], yv[S];
     __shared__ int srow[2];
-    if (a.ctl->pcg_fail == 0) return;
+    if (a.gate && a.ctl->pcg_fail == 0) return;
     const int tid = threadIdx.x;
     if (tid == 0) {
         int tk = __hip_atomic_fetch_add(a.sync, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1059,7 +1081,7 @@ __global__ __launch_bounds__(256) void k_cr_tail3(CrArgs a, int L) {
     __shared__ double A[S][SP], Wl[S][SP], Mb[S][SP];
     __shared__ double vb[S], vy[S], vc[S], vx[S], vz[S];
     __shared__ int stk;
-    if (a.ctl->pcg_fail == 0) return;
+    if (a.gate && a.ctl->pcg_fail == 0) return;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const long long SS = (long long)S * S;
     const int r0 = 0, r1 = 1 << L, r2 = 2 << L;
@@ -1273,7 +1295,7 @@ static void launch_cr_p(const CrArgs& a, hipStream_t s) {
     if (n_back > 0) hipLaunchKernelGGL(k_cr_back<P>, dim3(n_back), dim3(256), 0, s, a, L);
 }
 
-void launch_cr(const double* H, const double* g, double* x, int Pp, int D, double* work, Control* ctl,
+void launch_cr(const double* H, const double* g, double* x, int Pp, int D, double* work, Control* ctl, int gated,
                hipStream_t s) {
     CrArgs a;
     const size_t S = 16 * (size_t)D;
@@ -1284,6 +1306,7 @@ void launch_cr(const double* H, const double* g, double* x, int Pp, int D, doubl
     a.x = x;
     a.Pp = Pp;
     a.ctl = ctl;
+    a.gate = gated;
     a.D = work;
     a.E = a.D + ns * S * S;
     a.W = a.E + ns * S * S;

@@ -56,9 +56,10 @@ int prepare_chol_band(int Dmax, size_t lds);
 int chol_band_block();
 size_t chol_band_lds(int Pp, int D);
 // block cyclic reduction for profiles at most cr_max_depth() blocks of 16 deep:
-// the factorisation and both solves of H x = g, 4 log2(super-rows) + 1 launches;
-// work holds cr_work_doubles(Pp, D) doubles
-void launch_cr(const double* H, const double* g, double* x, int Pp, int D, double* work, Control* ctl,
+// the factorisation and both solves of H x = g, 2 log2(super-rows) + 2 launches;
+// work holds cr_work_doubles(Pp, D) doubles; gated: skipped when ctl->pcg_fail
+// is 0 (the iteration's CG solve converged)
+void launch_cr(const double* H, const double* g, double* x, int Pp, int D, double* work, Control* ctl, int gated,
                hipStream_t s);
 size_t cr_work_doubles(int Pp, int D);
 int cr_max_depth();

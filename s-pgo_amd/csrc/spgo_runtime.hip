@@ -692,7 +692,7 @@ AsmArgs make_asm(const spgo_solver* h, int init) {
 // the band factorisation and both solves: envelope (one or two kernels) or cyclic reduction
 static void launch_chol(spgo_solver* h, hipStream_t s) {
     if (h->path == SPGO_FACTOR_CR)
-        launch_cr(h->H, h->gvec, h->delta, h->Pp, std::max(h->band_depth, 1), h->crw, h->ctl, s);
+        launch_cr(h->H, h->gvec, h->delta, h->Pp, std::max(h->band_depth, 1), h->crw, h->ctl, h->pcg ? 1 : 0, s);
     else if (h->path == SPGO_FACTOR_ENVELOPE_SPLIT)
         launch_chol_split(h->H, h->Pp, h->band_depth, h->s0, h->s1, h->first, h->firstB, h->gvec, h->delta, h->Linv,
                           h->Lpan, h->Ssep, h->Smrg, h->winvS, h->LpS, h->ctl, s);
