@@ -71,6 +71,44 @@ def assembly_bytes(p, real_bytes: int = 8) -> int:
     return int(b)
 
 
+def minimal_bytes_fields(nbytes: int, ms: float) -> dict:
+    """The roofline fields of the minimal byte count beside the builder's own."""
+    ach = nbytes / (ms * 1e-3) / 1e9
+    return {"algorithmic_bytes_minimal": nbytes, "achieved_minimal": ach, "frac_minimal": ach / HBM_PEAK_GBS}
+
+
+def assembly_bytes_minimal(p, real_bytes: int = 4) -> int:
+    """The minimal algorithmic bytes of one linearisation, counted per item
+    of the problem with nothing re-read: per measurement 3 int32 ids and 4
+    pixels (28 B), per landmark the anchor pixel and alpha (12 B), per pose
+    its 3x4 matrix (48 B) read; per landmark C and g_alpha (8 B), its B
+    column (6 k_l values) and per coupled pose pair one 6x6 block written.
+    Floating-point terms scale with real_bytes / 4.  Unlike assembly_bytes it
+    leaves out the B-column read of the back-substitution and the per-anchor
+    blocks' full squares."""
+    f = real_bytes / 4
+    a, t, l = p.meas_a.astype(np.int64), p.meas_t.astype(np.int64), p.meas_l.astype(np.int64)
+    O = a.size
+    span = np.zeros(p.n_slots, np.int64)
+    np.maximum.at(span, l, np.maximum(t - a, 0))
+    used = np.unique(l)
+    k_l = span[used]
+    anchors = np.zeros(p.n_slots, np.int64)
+    anchors[l] = a
+    # coupled pose pairs (i >= j): poses a .. a + k - 1 of each landmark couple pairwise
+    first = np.arange(p.n_poses)
+    for a0, k in zip(anchors[used], k_l):
+        if k > 0:
+            first[a0:a0 + k] = np.minimum(first[a0:a0 + k], a0)
+    covered = np.zeros(p.n_poses, bool)
+    for a0, k in zip(anchors[used], k_l):
+        covered[a0:a0 + k] = True
+    n_pairs = int(np.sum((np.arange(p.n_poses) - first + 1)[covered]))
+    rd = O * (12 + 16 * f) + used.size * 12 * f + p.n_poses * 48 * f
+    wr = used.size * 8 * f + int(np.sum(6 * k_l)) * 4 * f + n_pairs * 36 * 4 * f
+    return int(rd + wr)
+
+
 def scalar_profile(p) -> np.ndarray:
     """First coupled column of every row of the reduced pose system: pose j
     couples back to the first frame of every landmark whose observations span
@@ -236,6 +274,7 @@ def bench_c2(args) -> int:
     roof_asm = {"kernel": "k_assemble<double>", "bound": "hbm", "achieved": ach_a, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach_a / HBM_PEAK_GBS, "traffic": profile_traffic("k_assemble"),
                 "algorithmic_bytes": nbytes}
+    roof_asm.update(minimal_bytes_fields(assembly_bytes_minimal(p, 8), phases["assemble"]))
     dom = max(phases, key=phases.get)
     if dom == "cholesky":
         flops = envelope_flops(p) if envelope else dense_flops(p.n_poses)
@@ -436,6 +475,7 @@ def bench_c3(args) -> int:
     roof_asm = {"kernel": f"k_assemble<{'float' if real == 4 else 'double'}>", "bound": "hbm", "achieved": ach_a,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach_a / HBM_PEAK_GBS,
                 "traffic": profile_traffic(f"k_assemble_c3_{form}"), "algorithmic_bytes": nbytes}
+    roof_asm.update(minimal_bytes_fields(assembly_bytes_minimal(p, real), phases["assemble"]))
     dom = max(phases, key=phases.get)
     if dom == "cholesky":
         roof = chol_roofline(p, phases, best["layout"])
@@ -565,6 +605,7 @@ def bench_c4(args) -> int:
     ach = nbytes / (phases["assemble"] * 1e-3) / 1e9
     roof = {"kernel": "k_assemble<float>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "traffic": profile_traffic("k_assemble_c4"), "algorithmic_bytes": nbytes}
+    roof.update(minimal_bytes_fields(int(sum(assembly_bytes_minimal(w, 4) for w in ws)), phases["assemble"]))
     dom = max(phases, key=phases.get)
     if dom != "assemble":
         roof["note"] = f"dominant phase is {dom}; roofline given for the assembly kernel"
