@@ -251,7 +251,13 @@ __global__ __launch_bounds__(WG, (KT <= 12 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 
             const T Pa[3] = {lmP[j][0], lmP[j][1], lmP[j][2]};
             const T dv[3] = {lmP[j][3], lmP[j][4], lmP[j][5]};
             const T kappa = (A.rb.near_boost && abs(t - a) < 2) ? (T)16 : (T)1;
-            T u[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
+            // The measurement's four weighted rows are J = [P_a x v, v] with v = w n
+            // (n: a row of M R_{a->t}).  Its sums are formed from v alone --
+            // Mv = sum v v^T, r = sum rk v, q = sum ja v -- and expanded once with
+            // A = [P_a]x: J^T J = [[A Mv A^T, A Mv], [., Mv]], J^T rk = [A r, r],
+            // J^T ja = [A q, q].
+            T M00 = (T)0, M01 = (T)0, M02 = (T)0, M11 = (T)0, M12 = (T)0, M22 = (T)0;
+            T r0 = (T)0, r1 = (T)0, r2 = (T)0, q0 = (T)0, q1 = (T)0, q2 = (T)0;
             T cc = (T)0, gg = (T)0;
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
@@ -277,43 +283,59 @@ __global__ __launch_bounds__(WG, (KT <= 12 && sizeof(T) == 4) ? 3 : (KT <= 12 ? 
                 const T ey = (c ? (T)uv.w : (T)uv.y) - (fy * Py * iz + cy);
                 T w;
                 robust_eval(A.rb, inv_param, ex * ex + ey * ey, kappa, w, cost);
-                // rows of M R_{a->t}: M = K_c Pi'(P_t) (LMutils.cu:615-628)
-                const T m0a = fx * iz, m0c = -fx * Px * iz * iz;
-                const T m1b = fy * iz, m1c = -fy * Py * iz * iz;
+                // rows of M R_{a->t}: M = K_c Pi'(P_t) (LMutils.cu:615-628), weighted
+                const T wfx = w * fx, wfy = w * fy;
+                const T m0a = wfx * iz, m0c = -wfx * Px * iz * iz;
+                const T m1b = wfy * iz, m1c = -wfy * Py * iz * iz;
 #pragma unroll
                 for (int k = 0; k < 2; ++k) {
-                    T n[3];
+                    T v0, v1, v2;
                     if (k == 0) {
-                        n[0] = m0a * R[0] + m0c * R[8];
-                        n[1] = m0a * R[1] + m0c * R[9];
-                        n[2] = m0a * R[2] + m0c * R[10];
+                        v0 = m0a * R[0] + m0c * R[8];
+                        v1 = m0a * R[1] + m0c * R[9];
+                        v2 = m0a * R[2] + m0c * R[10];
                     } else {
-                        n[0] = m1b * R[4] + m1c * R[8];
-                        n[1] = m1b * R[5] + m1c * R[9];
-                        n[2] = m1b * R[6] + m1c * R[10];
+                        v0 = m1b * R[4] + m1c * R[8];
+                        v1 = m1b * R[5] + m1c * R[9];
+                        v2 = m1b * R[6] + m1c * R[10];
                     }
-                    // weighted row [P_a x n, n] (the -[P]x of compute_del_tn_del_xi)
-                    T J[6];
-                    J[0] = w * (Pa[1] * n[2] - Pa[2] * n[1]);
-                    J[1] = w * (Pa[2] * n[0] - Pa[0] * n[2]);
-                    J[2] = w * (Pa[0] * n[1] - Pa[1] * n[0]);
-                    J[3] = w * n[0];
-                    J[4] = w * n[1];
-                    J[5] = w * n[2];
-                    const T ja = w * (n[0] * dv[0] + n[1] * dv[1] + n[2] * dv[2]);
+                    const T ja = v0 * dv[0] + v1 * dv[1] + v2 * dv[2];
                     const T rk = w * (k == 0 ? ex : ey);
-                    int idx = 0;
-#pragma unroll
-                    for (int p = 0; p < 6; ++p) {
-#pragma unroll
-                        for (int q = p; q < 6; ++q) pr[idx++] += J[p] * J[q];
-                        pr[21 + p] += J[p] * rk;
-                        u[p] += J[p] * ja;
-                    }
+                    M00 += v0 * v0;
+                    M01 += v0 * v1;
+                    M02 += v0 * v2;
+                    M11 += v1 * v1;
+                    M12 += v1 * v2;
+                    M22 += v2 * v2;
+                    r0 += rk * v0;
+                    r1 += rk * v1;
+                    r2 += rk * v2;
+                    q0 += ja * v0;
+                    q1 += ja * v1;
+                    q2 += ja * v2;
                     cc += ja * ja;
                     gg += ja * rk;
                 }
             }
+            // the expansion with A = [P_a]x (A x = P_a x x, the -[P]x of
+            // compute_del_tn_del_xi): B = A Mv, then A Mv A^T = B A^T
+            const T a0 = Pa[0], a1 = Pa[1], a2 = Pa[2];
+            const T B00 = a1 * M02 - a2 * M01, B01 = a1 * M12 - a2 * M11, B02 = a1 * M22 - a2 * M12;
+            const T B10 = a2 * M00 - a0 * M02, B11 = a2 * M01 - a0 * M12, B12 = a2 * M02 - a0 * M22;
+            const T B20 = a0 * M01 - a1 * M00, B21 = a0 * M11 - a1 * M01, B22 = a0 * M12 - a1 * M02;
+            const T JJ[21] = {a1 * B02 - a2 * B01, a2 * B00 - a0 * B02, a0 * B01 - a1 * B00, B00, B01, B02,
+                              a2 * B10 - a0 * B12, a0 * B11 - a1 * B10, B10, B11, B12,
+                              a0 * B21 - a1 * B20, B20, B21, B22,
+                              M00, M01, M02, M11, M12, M22};
+#pragma unroll
+            for (int i = 0; i < 21; ++i) pr[i] += JJ[i];
+            pr[21] += a1 * r2 - a2 * r1;
+            pr[22] += a2 * r0 - a0 * r2;
+            pr[23] += a0 * r1 - a1 * r0;
+            pr[24] += r0;
+            pr[25] += r1;
+            pr[26] += r2;
+            const T u[6] = {a1 * q2 - a2 * q1, a2 * q0 - a0 * q2, a0 * q1 - a1 * q0, q0, q1, q2};
             if (seg > 0) {
                 // a cell written twice is two measurements of one landmark in one
                 // slot (the reference counts both): add, do not overwrite.  The
