@@ -145,7 +145,8 @@ def dense_flops(n_poses: int) -> float:
 BAND_PATHS = (2, 3, 5)
 CHOL_KERNELS = {1: "k_chol_panel + k_chol_update + k_solve_fwd + k_solve_bwd",
                 2: "k_chol_env", 3: "k_chol_split1 + k_chol_split2",
-                5: "k_cr_factor + k_cr_update + k_cr_tail + k_cr_back (cyclic reduction)"}
+                5: "k_cr_factor + k_cr_update + k_cr_tail3 + k_cr_back (cyclic reduction; the candidate's pose "
+                   "update runs in k_cr_back's last workgroup, so the pose_update phase holds no kernel)"}
 CHOL_TRAFFIC = {2: "k_chol_env", 3: "k_chol_split", 5: "k_cr"}
 CHOL_NAMES = {1: "dense", 2: "envelope", 3: "envelope, split", 5: "cyclic reduction"}
 

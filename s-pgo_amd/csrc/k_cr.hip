@@ -41,6 +41,7 @@
 
 #include "spgo_device.h"
 #include "spgo_kernels.h"
+#include "spgo_pose.h"
 
 namespace spgo {
 
@@ -257,7 +258,7 @@ __device__ __forceinline__ void cr_wait(const int* flag, Control* ctl) {
     while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
         if (*ok == 0) break;
         if (++n > CR_SPIN_MAX) {
-            *ok = 0;
+            __hip_atomic_store(&ctl->chol_ok, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -907,7 +908,7 @@ __global__ __launch_bounds__(256) void k_cr_update(CrArgs a, int level) {
 // and y before it waits for the solutions of its two neighbours (one level
 // up, or rows of the tail), then publishes its own.
 template <int P>
-__global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int L) {
+__global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int L, PoseStep ps, int fuse) {
     constexpr int S = 16 * P, SP = S + 1, Q = S / 4, NE = S * S / 256;
     __shared__ double Us[S][SP], Vs[S][SP], Ws[S][SP];
     __shared__ double xp[S], xq[S], z[S], yv[S];
@@ -986,6 +987,19 @@ __global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int L) {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __syncthreads();
     if (tid == 0) __hip_atomic_store(a.sync + 1 + i, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (fuse) {
+        // the last workgroup to finish finds every row's solution stored and runs
+        // the pose update (it reads the solution with device-coherent loads);
+        // it leaves the counter at zero for the next solve
+        if (tid == 0)
+            srow[0] = __hip_atomic_fetch_add(a.sync + a.ns + 3, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                      (int)gridDim.x - 1;
+        __syncthreads();
+        if (srow[0]) {
+            if (tid == 0) __hip_atomic_store(a.sync + a.ns + 3, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            pose_update_block<true>(ps.pd, ps.frame_mode, ps.cam, ps.b, a.ctl, a.x, ps.step, 0);
+        }
+    }
 }
 
 // Helpers of the three-row tail (super-block tiles in LDS, 16 x 16 MFMA tiles).
@@ -1265,7 +1279,7 @@ int cr_max_depth() { return 4; }
 
 size_t cr_work_doubles(int Pp, int D) {
     const size_t S = 16 * (size_t)D, ns = (Pp + S - 1) / S;
-    return ns * (5 * S * S + 2 * S) + (ns + 3) / 2 + 1;   // the blocks, then the flags and tickets
+    return ns * (5 * S * S + 2 * S) + (ns + 5) / 2 + 1;   // the blocks, then the flags and tickets
 }
 
 // super-rows left to the sequential tail: each level of the reduction costs
@@ -1273,7 +1287,7 @@ size_t cr_work_doubles(int Pp, int D) {
 constexpr int CR_TAIL_ROWS = 3;
 
 template <int P>
-static void launch_cr_p(const CrArgs& a, hipStream_t s) {
+static bool launch_cr_p(const CrArgs& a, const PoseStep* ps, hipStream_t s) {
     int L = 0;
     while (((a.ns + (1 << L) - 1) >> L) > CR_TAIL_ROWS) ++L;
     for (int l = 0; l < L; ++l) {
@@ -1292,11 +1306,13 @@ static void launch_cr_p(const CrArgs& a, hipStream_t s) {
         const int st = 1 << l;
         n_back += (a.ns - st + 2 * st - 1) / (2 * st);
     }
-    if (n_back > 0) hipLaunchKernelGGL(k_cr_back<P>, dim3(n_back), dim3(256), 0, s, a, L);
+    if (n_back == 0) return false;
+    hipLaunchKernelGGL(k_cr_back<P>, dim3(n_back), dim3(256), 0, s, a, L, ps ? *ps : PoseStep{}, ps ? 1 : 0);
+    return ps != nullptr;
 }
 
-void launch_cr(const double* H, const double* g, double* x, int Pp, int D, double* work, Control* ctl, int gated,
-               hipStream_t s) {
+bool launch_cr(const double* H, const double* g, double* x, int Pp, int D, double* work, Control* ctl, int gated,
+               const PoseStep* ps, hipStream_t s) {
     CrArgs a;
     const size_t S = 16 * (size_t)D;
     a.ns = (int)((Pp + S - 1) / S);
@@ -1316,10 +1332,10 @@ void launch_cr(const double* H, const double* g, double* x, int Pp, int D, doubl
     a.y = a.b + ns * S;
     a.sync = reinterpret_cast<int*>(a.y + ns * S);
     switch (D) {
-    case 1: launch_cr_p<1>(a, s); break;
-    case 2: launch_cr_p<2>(a, s); break;
-    case 3: launch_cr_p<3>(a, s); break;
-    default: launch_cr_p<4>(a, s); break;
+    case 1: return launch_cr_p<1>(a, ps, s);
+    case 2: return launch_cr_p<2>(a, ps, s);
+    case 3: return launch_cr_p<3>(a, ps, s);
+    default: return launch_cr_p<4>(a, ps, s);
     }
 }
 

@@ -108,6 +108,7 @@ struct spgo_solver {
     DevMem mem;          // problem + buffers (rebuilt with the layout)
     HostLayout hl;
     Layout lay{};
+    spgo::ProbDesc pd1{};   // the problem table's one entry (host copy)
     int prec = SPGO_PREC_FP64;
     StateBuf<double> bd[2]{};
     StateBuf<float> bf[2]{};

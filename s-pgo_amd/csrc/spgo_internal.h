@@ -147,4 +147,25 @@ struct StateBuf {
     double* Dw;       // frame coordinates: per group B in world axes (D holds it in the anchor's)
 };
 
+// The pose arrays of both state buffers (double in either precision).
+struct PoseBufs {
+    double* X[2];
+    double* G[2];
+    double* GinvL[2];
+    double* GinvR[2];
+};
+template <typename T>
+inline __host__ __device__ PoseBufs pose_bufs(const StateBuf<T>& b0, const StateBuf<T>& b1) {
+    return PoseBufs{{b0.X, b1.X}, {b0.G, b1.G}, {b0.GinvL, b1.GinvL}, {b0.GinvR, b1.GinvR}};
+}
+
+// A pose update run by the last workgroup of another kernel (one problem).
+struct PoseStep {
+    ProbDesc pd;
+    Camera cam;
+    PoseBufs b;
+    double step;
+    int frame_mode;
+};
+
 }  // namespace spgo

@@ -58,9 +58,11 @@ size_t chol_band_lds(int Pp, int D);
 // block cyclic reduction for profiles at most cr_max_depth() blocks of 16 deep:
 // the factorisation and both solves of H x = g, 2 log2(super-rows) + 2 launches;
 // work holds cr_work_doubles(Pp, D) doubles; gated: skipped when ctl->pcg_fail
-// is 0 (the iteration's CG solve converged)
-void launch_cr(const double* H, const double* g, double* x, int Pp, int D, double* work, Control* ctl, int gated,
-               hipStream_t s);
+// is 0 (the iteration's CG solve converged).  With ps, the last workgroup of
+// the back-substitution also runs that pose update (x is its step); returns
+// whether it did (not when the system is solved by the tail kernel alone).
+bool launch_cr(const double* H, const double* g, double* x, int Pp, int D, double* work, Control* ctl, int gated,
+               const PoseStep* ps, hipStream_t s);
 size_t cr_work_doubles(int Pp, int D);
 int cr_max_depth();
 void launch_chol_solve(const double* H, const double* Linv, const double* g, double* delta, int Pp,

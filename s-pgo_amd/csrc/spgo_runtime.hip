@@ -638,6 +638,7 @@ int setup_layout(spgo_solver* h) {
         pd.Pp = h->Pp;
         pd.D = h->band_depth;
         std::vector<ProbDesc> probs(1, pd);
+        h->pd1 = pd;
         std::vector<int> pose_prob(h->N, 0);
         if ((rc = upload(h, probs, &lay.probs)) || (rc = upload(h, pose_prob, &lay.pose_prob))) return rc;
         lay.n_probs = 1;
@@ -689,16 +690,32 @@ AsmArgs make_asm(const spgo_solver* h, int init) {
     return a;
 }
 
-// the band factorisation and both solves: envelope (one or two kernels) or cyclic reduction
-static void launch_chol(spgo_solver* h, hipStream_t s) {
+// the band factorisation and both solves: envelope (one or two kernels) or cyclic reduction.
+// Cyclic reduction without CG also runs the pose update ps in its last
+// back-substitution workgroup; returns whether it did.
+static bool launch_chol(spgo_solver* h, const PoseStep* ps, hipStream_t s) {
     if (h->path == SPGO_FACTOR_CR)
-        launch_cr(h->H, h->gvec, h->delta, h->Pp, std::max(h->band_depth, 1), h->crw, h->ctl, h->pcg ? 1 : 0, s);
-    else if (h->path == SPGO_FACTOR_ENVELOPE_SPLIT)
+        return launch_cr(h->H, h->gvec, h->delta, h->Pp, std::max(h->band_depth, 1), h->crw, h->ctl,
+                         h->pcg ? 1 : 0, h->pcg ? nullptr : ps, s);
+    if (h->path == SPGO_FACTOR_ENVELOPE_SPLIT)
         launch_chol_split(h->H, h->Pp, h->band_depth, h->s0, h->s1, h->first, h->firstB, h->gvec, h->delta, h->Linv,
                           h->Lpan, h->Ssep, h->Smrg, h->winvS, h->LpS, h->ctl, s);
     else
         launch_chol_band(h->lay.probs, 1, h->band_depth, chol_band_lds(h->Pp, h->band_depth), h->H, h->first, h->gvec,
                          h->delta, h->Linv, h->Lpan, h->ctl, s);
+    return false;
+}
+
+// the candidate's pose update (step size s) as the cyclic reduction runs it
+template <typename T>
+static PoseStep pose_step(spgo_solver* h, StateBuf<T>* b) {
+    PoseStep ps{};
+    ps.pd = h->pd1;
+    ps.cam = make_camera(h);
+    ps.b = pose_bufs(b[0], b[1]);
+    ps.step = (double)h->step_size;
+    ps.frame_mode = h->lay.frame_mode;
+    return ps;
 }
 
 template <typename T>
@@ -719,16 +736,18 @@ int launch_iteration(spgo_solver* h, StateBuf<T>* b) {
             return rc;
         factor = *h->pcg_flag_host != 0;
     }
+    bool posed = false;
     if (factor) {
         if (h->band) {
-            launch_chol(h, s);
+            const PoseStep ps = pose_step(h, b);
+            posed = launch_chol(h, &ps, s);
         } else {
             launch_cholesky(h->H, h->Linv, h->Pp, h->dense_act_off, h->dense_act, h->dense_nact.data(), h->ctl, s);
             launch_chol_solve(h->H, h->Linv, h->gvec, h->delta, h->Pp, h->dense_first, h->dense_act_off,
                               h->dense_act, h->solve_y, h->solve_sync, h->ctl, s);
         }
     }
-    launch_pose_update<T>(h->lay, make_camera(h), b, h->ctl, h->delta, (double)h->step_size, 0, s);
+    if (!posed) launch_pose_update<T>(h->lay, make_camera(h), b, h->ctl, h->delta, (double)h->step_size, 0, s);
     launch_assemble<T>(make_asm(h, 0), b, s);
     launch_accept(b[0].cost, b[1].cost, h->lay, h->ctl, h->rec, h->opt.mode == SPGO_MODE_REFERENCE,
                   h->opt.lam_up, h->opt.lam_down, h->opt.lam_min, h->opt.lam_max, 0, s);
@@ -1389,10 +1408,18 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
             if (rc) break;
             factor = *h->pcg_flag_host != 0;
         }
+        bool posed = false;
         if (!factor) {
             (void)hipEventRecord(ev[2], s);
         } else if (h->band) {
-            launch_chol(h, s);
+            // with cyclic reduction the pose update runs inside it, in this phase
+            if (mixed) {
+                const PoseStep ps = pose_step(h, h->bf);
+                posed = launch_chol(h, &ps, s);
+            } else {
+                const PoseStep ps = pose_step(h, h->bd);
+                posed = launch_chol(h, &ps, s);
+            }
             (void)hipEventRecord(ev[2], s);
         } else {
             launch_cholesky(h->H, h->Linv, h->Pp, h->dense_act_off, h->dense_act, h->dense_nact.data(), h->ctl, s);
@@ -1401,8 +1428,10 @@ int spgo_profile_phases(spgo_solver* h, int32_t iterations, double ms[SPGO_N_PHA
                               h->dense_act, h->solve_y, h->solve_sync, h->ctl, s);
         }
         (void)hipEventRecord(ev[3], s);
-        if (mixed) launch_pose_update<float>(h->lay, make_camera(h), h->bf, h->ctl, h->delta, h->step_size, 0, s);
-        else launch_pose_update<double>(h->lay, make_camera(h), h->bd, h->ctl, h->delta, h->step_size, 0, s);
+        if (!posed && mixed)
+            launch_pose_update<float>(h->lay, make_camera(h), h->bf, h->ctl, h->delta, h->step_size, 0, s);
+        else if (!posed)
+            launch_pose_update<double>(h->lay, make_camera(h), h->bd, h->ctl, h->delta, h->step_size, 0, s);
         (void)hipEventRecord(ev[4], s);
         if (mixed) launch_assemble<float>(make_asm(h, 0), h->bf, s);
         else launch_assemble<double>(make_asm(h, 0), h->bd, s);
