@@ -805,7 +805,7 @@ __global__ __launch_bounds__(256) void k_cr_update(CrArgs a, int level) {
     const bool fromH = level == 0;
     const long long SS = (long long)S * S;
     const int part = blockIdx.y;
-    if (blockIdx.x == 0 && part == 0 && tid < 2) a.sync[a.ns + 1 + tid] = 0;   // k_cr_tail3's flag and ticket
+    if (blockIdx.x == 0 && part == 0 && tid < 4) a.sync[a.ns + 1 + tid] = 0;   // k_cr_tail3's flags and ticket, k_cr_back's count
     if (part < 2 * P) {
         const bool isE = part >= P;
         const int r = isE ? part - P : part, c = w;
@@ -1080,26 +1080,32 @@ __device__ __forceinline__ double cr_matvec(double (*M)[16 * P + 1], const doubl
 // eliminated concurrently, then the middle one, so only two factorisations are
 // sequential.  Two workgroups, by ticket:
 //   ticket 0: D_0 = L_0 L_0^T, W_0, V_0 = W_0 E_1^T, y_0 = W_0 b_0, and the
-//             middle row's share C_0 = V_0^T V_0, c_0 = V_0^T y_0; all of it
-//             published, then a ready flag;
+//             middle row's share C_0 = V_0^T V_0, c_0 = V_0^T y_0, published
+//             with a ready flag; then, with W_0, V_0 and y_0 still in LDS, it
+//             waits for x_1 and solves x_0 = W_0^T (y_0 - V_0 x_1);
 //   ticket 1: D_2 = L_2 L_2^T, W_2, U_2 = W_2 E_2, y_2, C_2 = U_2^T U_2,
 //             c_2 = U_2^T y_2; waits for ticket 0 (started before it, so the
 //             wait ends whatever the residency); factors D_1 - C_0 - C_2, solves
-//             x_1 = W_1^T W_1 (b_1 - c_0 - c_2), then x_2 = W_2^T (y_2 - U_2 x_1)
-//             and x_0 = W_0^T (y_0 - V_0 x_1).
-// What ticket 0 hands over moves by device-coherent atomics (xput / xget).
-// sync[ns + 1] (flag) and sync[ns + 2] (ticket) are cleared by k_cr_update.
+//             x_1 = W_1^T W_1 (b_1 - c_0 - c_2), publishes it with a second flag,
+//             then x_2 = W_2^T (y_2 - U_2 x_1) while ticket 0 forms x_0.
+// Ticket 0's wait for x_1 relies on the grid's two workgroups being resident
+// together (two workgroups on an otherwise idle device); like every wait here
+// it is bounded, and a wait that runs out marks the factorisation failed.
+// What the workgroups hand over moves by device-coherent atomics (xput / xget).
+// sync[ns + 1] and sync[ns + 4] (flags) and sync[ns + 2] (ticket) are cleared
+// by k_cr_update.
 template <int P>
 __global__ __launch_bounds__(256) void k_cr_tail3(CrArgs a, int L) {
     constexpr int S = 16 * P, SP = S + 1, NE = S * S / 256;
-    __shared__ double A[S][SP], Wl[S][SP], Mb[S][SP];
+    __shared__ double A[S][SP], Wl[S][SP], Mb[S][SP], W2s[S][SP];
     __shared__ double vb[S], vy[S], vc[S], vx[S], vz[S];
     __shared__ int stk;
     if (a.gate && a.ctl->pcg_fail == 0) return;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const long long SS = (long long)S * S;
     const int r0 = 0, r1 = 1 << L, r2 = 2 << L;
-    int* flag = a.sync + a.ns + 1;
+    int* flag = a.sync + a.ns + 1;    // ticket 0's share of the middle row published
+    int* flagx = a.sync + a.ns + 4;   // x_1 published
     if (tid == 0) stk = __hip_atomic_fetch_add(a.sync + a.ns + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const int tk = stk;
@@ -1158,20 +1164,30 @@ __global__ __launch_bounds__(256) void k_cr_tail3(CrArgs a, int L) {
         __syncthreads();
         for (int q = tid; q < S * S; q += 256) {
             const int r = q / S, c = q - (q / S) * S;
-            xput(a.W + r0 * SS + q, Wl[r][c]);
-            xput(a.V + r0 * SS + q, A[r][c]);
             xput(a.U + r0 * SS + q, Mb[r][c]);
         }
-        if (tid < S) {
-            xput(a.y + (long long)r0 * S + tid, vy[tid]);
-            xput(a.b + (long long)r0 * S + tid, vc[tid]);
-        }
+        if (tid < S) xput(a.b + (long long)r0 * S + tid, vc[tid]);
         if (bad && tid == 0) a.ctl->chol_ok = 0;
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         __syncthreads();
         if (tid == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // x_0 = W_0^T (y_0 - V_0 x_1): W_0 in Wl, V_0 in A, y_0 in vy
+        if (tid == 0) cr_wait(flagx, a.ctl);
+        __syncthreads();
+        if (tid < S) vx[tid] = xget(a.x + S * r1 + tid);   // rows of r1 lie below S r2 < Pp
+        __syncthreads();
+        {
+            const double vx1 = cr_matvec<P>(A, vx, false, tid);
+            if ((tid & 3) == 0 && (tid >> 2) < S) vz[tid >> 2] = vy[tid >> 2] - vx1;
+        }
+        __syncthreads();
+        {
+            const double x0 = cr_matvec<P>(Wl, vz, true, tid);
+            const int gr = S * r0 + (tid >> 2);
+            if ((tid & 3) == 0 && (tid >> 2) < S && gr < a.Pp) a.x[gr] = x0;
+        }
         return;
     }
     // ticket 1: U_2 = W_2 E_2 in place in Mb, then C_2 = U_2^T U_2 into A, c_2
@@ -1182,11 +1198,11 @@ __global__ __launch_bounds__(256) void k_cr_tail3(CrArgs a, int L) {
         const double c2 = cr_matvec<P>(Mb, vy, true, tid);
         if ((tid & 3) == 0 && (tid >> 2) < S) vc[tid >> 2] = c2;
     }
-    // W_2 waits in its row's workspace; the back-substitution's ticket and flags
-    // (the rows solved here are ready)
+    // W_2 waits in LDS; the back-substitution's ticket and flags (the rows
+    // solved here are ready)
     for (int q = tid; q < S * S; q += 256) {
         const int r = q / S, c = q - (q / S) * S;
-        a.W[r2 * SS + q] = Wl[r][c];
+        W2s[r][c] = Wl[r][c];
     }
     for (int r = tid; r <= a.ns; r += 256) a.sync[r] = r == 0 ? 0 : (((r - 1) & ((1 << L) - 1)) == 0 ? 1 : 0);
     if (tid < S) vz[tid] = vy[tid];   // y_2
@@ -1217,60 +1233,28 @@ __global__ __launch_bounds__(256) void k_cr_tail3(CrArgs a, int L) {
     }
     __syncthreads();
     {
+        // x_1, published to ticket 0 with the second flag
         const double x1 = cr_matvec<P>(Wl, vy, true, tid);
         if ((tid & 3) == 0 && (tid >> 2) < S) {
             vx[tid >> 2] = x1;
-            const int gr = S * r1 + (tid >> 2);
-            if (gr < a.Pp) a.x[gr] = x1;
+            xput(a.x + S * r1 + (tid >> 2), x1);   // rows of r1 lie below S r2 < Pp
         }
     }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __syncthreads();
-    // x_2 = W_2^T (y_2 - U_2 x_1): W_2 back into Wl (U_2 is still in Mb)
+    if (tid == 0) __hip_atomic_store(flagx, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // x_2 = W_2^T (y_2 - U_2 x_1) (U_2 is still in Mb)
     {
-        double lw[NE];
-#pragma unroll
-        for (int u = 0; u < NE; ++u) lw[u] = a.W[r2 * SS + tid + 256 * u];
-#pragma unroll
-        for (int u = 0; u < NE; ++u) {
-            const int q = tid + 256 * u, r = q / S, c = q - (q / S) * S;
-            Wl[r][c] = lw[u];
-        }
         const double ux = cr_matvec<P>(Mb, vx, false, tid);
         if ((tid & 3) == 0 && (tid >> 2) < S) vz[tid >> 2] -= ux;
     }
     __syncthreads();
     {
-        const double x2 = cr_matvec<P>(Wl, vz, true, tid);
+        const double x2 = cr_matvec<P>(W2s, vz, true, tid);
         const int gr = S * r2 + (tid >> 2);
         if ((tid & 3) == 0 && (tid >> 2) < S && gr < a.Pp) a.x[gr] = x2;
-    }
-    __syncthreads();
-    // x_0 = W_0^T (y_0 - V_0 x_1)
-    {
-        double lw[NE], lv[NE];
-#pragma unroll
-        for (int u = 0; u < NE; ++u) {
-            lw[u] = xget(a.W + r0 * SS + tid + 256 * u);
-            lv[u] = xget(a.V + r0 * SS + tid + 256 * u);
-        }
-        if (tid < S) vz[tid] = xget(a.y + (long long)r0 * S + tid);
-#pragma unroll
-        for (int u = 0; u < NE; ++u) {
-            const int q = tid + 256 * u, r = q / S, c = q - (q / S) * S;
-            Wl[r][c] = lw[u];
-            Mb[r][c] = lv[u];
-        }
-    }
-    __syncthreads();
-    {
-        const double vx1 = cr_matvec<P>(Mb, vx, false, tid);
-        if ((tid & 3) == 0 && (tid >> 2) < S) vz[tid >> 2] -= vx1;
-    }
-    __syncthreads();
-    {
-        const double x0 = cr_matvec<P>(Wl, vz, true, tid);
-        const int gr = S * r0 + (tid >> 2);
-        if ((tid & 3) == 0 && (tid >> 2) < S && gr < a.Pp) a.x[gr] = x0;
     }
     if (bad && tid == 0) a.ctl->chol_ok = 0;
 }
@@ -1279,7 +1263,7 @@ int cr_max_depth() { return 4; }
 
 size_t cr_work_doubles(int Pp, int D) {
     const size_t S = 16 * (size_t)D, ns = (Pp + S - 1) / S;
-    return ns * (5 * S * S + 2 * S) + (ns + 5) / 2 + 1;   // the blocks, then the flags and tickets
+    return ns * (5 * S * S + 2 * S) + (ns + 7) / 2 + 1;   // the blocks, then the flags and tickets
 }
 
 // super-rows left to the sequential tail: each level of the reduction costs
