@@ -1078,46 +1078,48 @@ __device__ __forceinline__ double cr_matvec(double (*M)[16 * P + 1], const doubl
 
 // Three super-rows left (r0 = 0, r1 = 2^L, r2 = 2^(L+1)): the two ends are
 // eliminated concurrently, then the middle one, so only two factorisations are
-// sequential.  Two workgroups, by ticket:
-//   ticket 0: D_0 = L_0 L_0^T, W_0, V_0 = W_0 E_1^T, y_0 = W_0 b_0, and the
+// sequential.  Two workgroups:
+//   workgroup 0: D_0 = L_0 L_0^T, W_0, V_0 = W_0 E_1^T, y_0 = W_0 b_0, and the
 //             middle row's share C_0 = V_0^T V_0, c_0 = V_0^T y_0, published
 //             with a ready flag; then, with W_0, V_0 and y_0 still in LDS, it
 //             waits for x_1 and solves x_0 = W_0^T (y_0 - V_0 x_1);
-//   ticket 1: D_2 = L_2 L_2^T, W_2, U_2 = W_2 E_2, y_2, C_2 = U_2^T U_2,
-//             c_2 = U_2^T y_2; waits for ticket 0 (started before it, so the
-//             wait ends whatever the residency); factors D_1 - C_0 - C_2, solves
-//             x_1 = W_1^T W_1 (b_1 - c_0 - c_2), publishes it with a second flag,
-//             then x_2 = W_2^T (y_2 - U_2 x_1) while ticket 0 forms x_0.
-// Ticket 0's wait for x_1 relies on the grid's two workgroups being resident
-// together (two workgroups on an otherwise idle device); like every wait here
-// it is bounded, and a wait that runs out marks the factorisation failed.
+//   workgroup 1: D_2 = L_2 L_2^T, W_2, U_2 = W_2 E_2, y_2, C_2 = U_2^T U_2,
+//             c_2 = U_2^T y_2; waits for workgroup 0's share; factors
+//             D_1 - C_0 - C_2, solves x_1 = W_1^T W_1 (b_1 - c_0 - c_2),
+//             publishes it with a second flag, then x_2 = W_2^T (y_2 - U_2 x_1)
+//             while workgroup 0 forms x_0.
+// The two waits rely on the grid's two workgroups being resident together
+// (two workgroups on an otherwise idle device); like every wait here they are
+// bounded, and a wait that runs out marks the factorisation failed.
 // What the workgroups hand over moves by device-coherent atomics (xput / xget).
-// sync[ns + 1] and sync[ns + 4] (flags) and sync[ns + 2] (ticket) are cleared
-// by k_cr_update.
+// sync[ns + 1] and sync[ns + 4] (the flags) are cleared by k_cr_update.
 template <int P>
 __global__ __launch_bounds__(256) void k_cr_tail3(CrArgs a, int L) {
     constexpr int S = 16 * P, SP = S + 1, NE = S * S / 256;
     __shared__ double A[S][SP], Wl[S][SP], Mb[S][SP], W2s[S][SP];
     __shared__ double vb[S], vy[S], vc[S], vx[S], vz[S];
-    __shared__ int stk;
     if (a.gate && a.ctl->pcg_fail == 0) return;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const long long SS = (long long)S * S;
     const int r0 = 0, r1 = 1 << L, r2 = 2 << L;
-    int* flag = a.sync + a.ns + 1;    // ticket 0's share of the middle row published
+    int* flag = a.sync + a.ns + 1;    // workgroup 0's share of the middle row published
     int* flagx = a.sync + a.ns + 4;   // x_1 published
-    if (tid == 0) stk = __hip_atomic_fetch_add(a.sync + a.ns + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int tk = stk;
+    const int tk = blockIdx.x;                    // 0: the left end; 1: the right end, then the middle row
     const int e = tk == 0 ? r0 : r2;              // this workgroup's end row
     const int en = tk == 0 ? r1 : r2;             // the coupling block: E of r1 (to r0) or of r2 (to r1)
-    // D_e, the coupling, b_e
+    // D_e, the coupling, b_e; workgroup 1 also requests D_1 and b_1, which it
+    // holds in registers until the left end's share has arrived
+    double ld1[NE];
+    double b1v = 0.0;
     {
         double le[NE];
 #pragma unroll
         for (int u = 0; u < NE; ++u) le[u] = a.E[en * SS + tid + 256 * u];
         cr_load_block<P>(a, e, false, A, Wl);
         if (tid < S) vb[tid] = a.b[(long long)e * S + tid];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) ld1[u] = tk == 1 ? a.D[r1 * SS + tid + 256 * u] : 0.0;
+        if (tk == 1 && tid < S) b1v = a.b[(long long)r1 * S + tid];
 #pragma unroll
         for (int u = 0; u < NE; ++u) {
             const int q = tid + 256 * u, r = q / S, c = q - (q / S) * S;
@@ -1190,7 +1192,7 @@ __global__ __launch_bounds__(256) void k_cr_tail3(CrArgs a, int L) {
         }
         return;
     }
-    // ticket 1: U_2 = W_2 E_2 in place in Mb, then C_2 = U_2^T U_2 into A, c_2
+    // workgroup 1: U_2 = W_2 E_2 in place in Mb, then C_2 = U_2^T U_2 into A, c_2
     cr_times_w<P>(Wl, Mb, false, w, lane);
     __syncthreads();
     cr_gram<P>(Mb, A, w, lane);
@@ -1210,17 +1212,14 @@ __global__ __launch_bounds__(256) void k_cr_tail3(CrArgs a, int L) {
     __syncthreads();
     // D_1 - C_0 - C_2 into A, b_1 - c_0 - c_2 into vb
     {
-        double ld[NE], lc[NE];
+        double lc[NE];
 #pragma unroll
-        for (int u = 0; u < NE; ++u) {
-            ld[u] = a.D[r1 * SS + tid + 256 * u];
-            lc[u] = xget(a.U + r0 * SS + tid + 256 * u);
-        }
-        if (tid < S) vb[tid] = a.b[(long long)r1 * S + tid] - xget(a.b + (long long)r0 * S + tid) - vc[tid];
+        for (int u = 0; u < NE; ++u) lc[u] = xget(a.U + r0 * SS + tid + 256 * u);
+        if (tid < S) vb[tid] = b1v - xget(a.b + (long long)r0 * S + tid) - vc[tid];
 #pragma unroll
         for (int u = 0; u < NE; ++u) {
             const int q = tid + 256 * u, r = q / S, c = q - (q / S) * S;
-            A[r][c] = (c >> 4) <= (r >> 4) ? ld[u] - lc[u] - A[r][c] : 0.0;
+            A[r][c] = (c >> 4) <= (r >> 4) ? ld1[u] - lc[u] - A[r][c] : 0.0;
             Wl[r][c] = 0.0;
         }
     }
@@ -1233,7 +1232,7 @@ __global__ __launch_bounds__(256) void k_cr_tail3(CrArgs a, int L) {
     }
     __syncthreads();
     {
-        // x_1, published to ticket 0 with the second flag
+        // x_1, published to workgroup 0 with the second flag
         const double x1 = cr_matvec<P>(Wl, vy, true, tid);
         if ((tid & 3) == 0 && (tid >> 2) < S) {
             vx[tid >> 2] = x1;
