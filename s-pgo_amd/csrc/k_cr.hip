@@ -232,17 +232,17 @@ struct CrArgs {
     double* V;         // per eliminated super-row: W E_q^T
     double* b;         // per super-row: current right-hand side (S)
     double* y;         // per eliminated super-row: W b_i (S)
-    int* sync;         // back-substitution: [0] ticket, [1 + r] solution of super-row r ready
+    int* sync;         // [ns + 1], [ns + 4]: k_cr_tail3's flags; [ns + 3]: k_cr_back's finished workgroups
     Control* ctl;
     int Pp, ns;
     int gate;          // 1: a CG solve may have solved this iteration (return when ctl->pcg_fail is 0)
 };
 
-// Solutions and ready flags pass between the back-substitution's workgroups by
+// Blocks, solutions and ready flags pass between workgroups of one launch by
 // relaxed device-scope atomics, coherent across the XCDs' L2 caches without an
-// acquire or release (which would write back or invalidate a whole L2): a row
-// waits until its solution stores have completed before it raises its flag,
-// and a reader loads the solution after it has seen the flag.
+// acquire or release (which would write back or invalidate a whole L2): a
+// writer waits until its stores have completed before it raises a flag, and a
+// reader loads the data after it has seen the flag.
 __device__ __forceinline__ double xget(const double* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -263,6 +263,24 @@ __device__ __forceinline__ void cr_wait(const int* flag, Control* ctl) {
         }
         __builtin_amdgcn_s_sleep(1);
     }
+}
+
+// The back-substitution hands solutions over through the solution vector
+// itself: the tail kernel sets every row it does not solve to NaN, and a row's
+// entry is ready once it is no longer NaN (a NaN from a failed factorisation
+// runs into the bounded wait, which marks the factorisation failed).
+__device__ __forceinline__ double cr_poll(const double* p, Control* ctl) {
+    int n = 0;
+    double v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (__builtin_isnan(v)) {
+        if (++n > CR_SPIN_MAX) {
+            __hip_atomic_store(&ctl->chol_ok, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return v;
 }
 
 // Element (r, c) of super-block (I, J) of H, identity past the padded size.
@@ -483,6 +501,15 @@ __device__ __forceinline__ void cr_load_block(const CrArgs& a, int i, bool fromH
     }
 }
 
+// Rows the back-substitution solves (those not a multiple of 2^L) to NaN:
+// not ready (see cr_poll).
+template <int S>
+__device__ __forceinline__ void cr_mark_unsolved(const CrArgs& a, int L, int tid) {
+    const double nan = __builtin_nan("");
+    for (int e = tid; e < a.Pp; e += 256)
+        if (((e / S) & ((1 << L) - 1)) != 0) a.x[e] = nan;
+}
+
 // One workgroup per super-row i eliminated at this level: D_i = L L^T, W = L^-1,
 // then U_i = W E_i, V_i = W E_q^T (MFMA tiles, wavefront w taking tile column w
 // of both) and y_i = W b_i; W, U, V and y are stored for the survivors' update
@@ -619,8 +646,6 @@ __global__ __launch_bounds__(256) void k_cr_tail(CrArgs a, int L) {
     if (a.gate && a.ctl->pcg_fail == 0) return;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int t = (a.ns + (1 << L) - 1) >> L;
-    // the back-substitution's ticket and flags: the rows solved here are ready
-    for (int r = tid; r <= a.ns; r += 256) a.sync[r] = r == 0 ? 0 : (((r - 1) & ((1 << L) - 1)) == 0 ? 1 : 0);
     const bool fromH = L == 0;
     const long long SS = (long long)S * S;
     const int row = tid >> 2, part = tid & 3;
@@ -789,6 +814,8 @@ __global__ __launch_bounds__(256) void k_cr_tail(CrArgs a, int L) {
             }
         }
     }
+    // the rows the back-substitution solves start unsolved
+    if (L > 0) cr_mark_unsolved<S>(a, L, tid);
 }
 
 // Survivor q of this level: D_q -= V_im^T V_im + U_ip^T U_ip (blockIdx.y < P: a
@@ -805,7 +832,7 @@ __global__ __launch_bounds__(256) void k_cr_update(CrArgs a, int level) {
     const bool fromH = level == 0;
     const long long SS = (long long)S * S;
     const int part = blockIdx.y;
-    if (blockIdx.x == 0 && part == 0 && tid < 4) a.sync[a.ns + 1 + tid] = 0;   // k_cr_tail3's flags and ticket, k_cr_back's count
+    if (blockIdx.x == 0 && part == 0 && tid < 4) a.sync[a.ns + 1 + tid] = 0;   // k_cr_tail3's flags, k_cr_back's count
     if (part < 2 * P) {
         const bool isE = part >= P;
         const int r = isE ? part - P : part, c = w;
@@ -901,33 +928,30 @@ __global__ __launch_bounds__(256) void k_cr_update(CrArgs a, int level) {
 }
 
 // x_i = W_i^T (y_i - U_i x_p - V_i x_q) for every super-row eliminated by the
-// levels, one workgroup each, in one launch.  A workgroup takes its row from a
-// ticket counter, the highest level first, so it only waits on rows whose
-// workgroups started before it: the sweep cannot deadlock, whatever the number
-// of resident workgroups.  It stages its U, V, W (coalesced loads into LDS)
-// and y before it waits for the solutions of its two neighbours (one level
-// up, or rows of the tail), then publishes its own.
+// levels, one workgroup each, in one launch.  The rows of the highest level
+// have the lowest workgroup indices, so a row waits only for rows of
+// workgroups dispatched before it (the launch's one-per-CU workgroups are
+// resident together on any device with as many CUs; every wait is bounded
+// regardless).  A workgroup stages its U, V, W (coalesced loads into LDS) and y,
+// then reads its two neighbours' solutions (one level up, or rows of the tail)
+// entry by entry as they become ready (cr_poll), and stores its own.
 template <int P>
 __global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int L, PoseStep ps, int fuse) {
     constexpr int S = 16 * P, SP = S + 1, Q = S / 4, NE = S * S / 256;
     __shared__ double Us[S][SP], Vs[S][SP], Ws[S][SP];
     __shared__ double xp[S], xq[S], z[S], yv[S];
-    __shared__ int srow[2];
+    __shared__ int last;
     if (a.gate && a.ctl->pcg_fail == 0) return;
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        int tk = __hip_atomic_fetch_add(a.sync, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int l = L - 1;
-        for (; l > 0; --l) {
-            const int st = 1 << l, n = (a.ns - st + 2 * st - 1) / (2 * st);
-            if (tk < n) break;
-            tk -= n;
-        }
-        srow[0] = (2 * tk + 1) << l;
-        srow[1] = l;
+    // the rows of the highest level first: a row waits only for rows of higher
+    // levels (and the tail's), which have lower workgroup indices
+    int tk = blockIdx.x, l = L - 1;
+    for (; l > 0; --l) {
+        const int stl = 1 << l, n = (a.ns - stl + 2 * stl - 1) / (2 * stl);
+        if (tk < n) break;
+        tk -= n;
     }
-    __syncthreads();
-    const int i = srow[0], st = 1 << srow[1], p = i - st, q = i + st;
+    const int i = (2 * tk + 1) << l, st = 1 << l, p = i - st, q = i + st;
     const long long SS = (long long)S * S;
     const bool has_q = q < a.ns;
     {
@@ -947,14 +971,12 @@ __global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int L, PoseStep ps, i
             Ws[r][c] = lw[u];
         }
     }
-    if (tid == 0) cr_wait(a.sync + 1 + p, a.ctl);
-    if (tid == 64 && has_q) cr_wait(a.sync + 1 + q, a.ctl);
-    __syncthreads();
-    if (tid < S) {
-        xp[tid] = xget(a.x + S * p + tid);   // rows of p lie below S i <= Pp
-        const int gq = S * q + tid;
-        const double v = xget(a.x + min(gq, a.Pp - 1));
-        xq[tid] = (has_q && gq < a.Pp) ? v : 0.0;
+    // the neighbours' solutions, entry by entry as they become ready (wavefront
+    // 0 row p, wavefront 1 row q)
+    if (tid < S) xp[tid] = cr_poll(a.x + S * p + tid, a.ctl);   // rows of p lie below S i <= Pp
+    if (tid >= 64 && tid < 64 + S) {
+        const int gq = S * q + tid - 64;
+        xq[tid - 64] = (has_q && gq < a.Pp) ? cr_poll(a.x + gq, a.ctl) : 0.0;
     }
     __syncthreads();
     const int row = tid >> 2, pt = tid & 3;
@@ -981,21 +1003,20 @@ __global__ __launch_bounds__(256) void k_cr_back(CrArgs a, int L, PoseStep ps, i
         const int gr = S * i + row;
         if (pt == 0 && gr < a.Pp) xput(a.x + gr, s);
     }
-    // every wavefront's solution stores complete, then the flag
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __syncthreads();
-    if (tid == 0) __hip_atomic_store(a.sync + 1 + i, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (fuse) {
-        // the last workgroup to finish finds every row's solution stored and runs
-        // the pose update (it reads the solution with device-coherent loads);
-        // it leaves the counter at zero for the next solve
-        if (tid == 0)
-            srow[0] = __hip_atomic_fetch_add(a.sync + a.ns + 3, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-                      (int)gridDim.x - 1;
+        // the last workgroup to finish (its count taken after its own solution
+        // stores completed) finds every row solved and runs the pose update (it
+        // reads the solution with device-coherent loads); it leaves the counter
+        // at zero for the next solve
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         __syncthreads();
-        if (srow[0]) {
+        if (tid == 0)
+            last = __hip_atomic_fetch_add(a.sync + a.ns + 3, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                   (int)gridDim.x - 1;
+        __syncthreads();
+        if (last) {
             if (tid == 0) __hip_atomic_store(a.sync + a.ns + 3, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             pose_update_block<true>(ps.pd, ps.frame_mode, ps.cam, ps.b, a.ctl, a.x, ps.step, 0);
         }
@@ -1200,13 +1221,12 @@ __global__ __launch_bounds__(256) void k_cr_tail3(CrArgs a, int L) {
         const double c2 = cr_matvec<P>(Mb, vy, true, tid);
         if ((tid & 3) == 0 && (tid >> 2) < S) vc[tid >> 2] = c2;
     }
-    // W_2 waits in LDS; the back-substitution's ticket and flags (the rows
-    // solved here are ready)
+    // W_2 waits in LDS; the rows the back-substitution solves start unsolved
     for (int q = tid; q < S * S; q += 256) {
         const int r = q / S, c = q - (q / S) * S;
         W2s[r][c] = Wl[r][c];
     }
-    for (int r = tid; r <= a.ns; r += 256) a.sync[r] = r == 0 ? 0 : (((r - 1) & ((1 << L) - 1)) == 0 ? 1 : 0);
+    cr_mark_unsolved<S>(a, L, tid);
     if (tid < S) vz[tid] = vy[tid];   // y_2
     if (tid == 0) cr_wait(flag, a.ctl);
     __syncthreads();
