@@ -424,9 +424,10 @@ static int factor_path(const spgo_solver* h, const HostLayout& L, std::vector<in
     case SPGO_FACTOR_CR: return depth <= cr_max_depth() ? SPGO_FACTOR_CR : -1;
     default:   // AUTO, and the Cholesky behind CG
         // cyclic reduction once the profile is long: at 8 super-rows (384 rows,
-        // depth 3) it takes as long as the split envelope, at 24 (1,536 rows,
-        // depth 4) 0.73 of it
-        if (depth >= 1 && depth <= cr_max_depth() && nbk >= 12 * depth * 16 / chol_band_block())
+        // depth 3) an LM iteration takes 0.98 of the split envelope's (its
+        // back-substitution also runs the pose update), at 24 (1,536 rows,
+        // depth 4) the solve takes 0.56 of it
+        if (depth >= 1 && depth <= cr_max_depth() && nbk >= 8 * depth * 16 / chol_band_block())
             return SPGO_FACTOR_CR;
         if (split_fits && nbk >= 2 * depth + 6) return SPGO_FACTOR_ENVELOPE_SPLIT;
         return fits ? SPGO_FACTOR_ENVELOPE : SPGO_FACTOR_DENSE;

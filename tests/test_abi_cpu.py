@@ -130,8 +130,9 @@ def test_envelope_profile_depth_of_the_64_keyframe_problem():
     first_row = 6 * first_pose[rows // 6]
     depth = max(b - first_row[16 * b:16 * b + 16].min() // 16 for b in range(6 * p.n_poses // 16))
     assert info["profile_depth"] == depth == 3
-    # 24 block rows: long enough for the split envelope factorisation (>= 2 x 3 + 6)
-    assert info["factorization"] == _lib.SPGO_FACTOR_ENVELOPE_SPLIT and info["reduced_size"] == 384
+    # 24 block rows of a profile 3 deep: 8 super-rows of 48, long enough for
+    # cyclic reduction
+    assert info["factorization"] == _lib.SPGO_FACTOR_CR and info["reduced_size"] == 384
 
 
 def test_layout_coordinates_for_mixed_anchors_and_wide_spans():
