@@ -4,7 +4,7 @@
 // G_0 = I, G_{k+1} = G_k X_k^-1, G^R = G T_r_to_l (Solver_cuda.cu:440-493),
 // computed as a parallel prefix product instead of the reference's
 // single-thread loop: every thread multiplies its chunk of poses, the chunk
-// products are scanned across each wavefront by shuffles and across the
+// products are scanned across each wavefront by DPP lane moves and across the
 // wavefronts in LDS, and each thread re-walks its chunk from its exclusive
 // prefix (with one pose per thread the prefix is the result).
 // COHERENT: the step delta is read with device-coherent loads (it was written
@@ -14,8 +14,30 @@
 
 namespace spgo {
 
+// A pose (12 doubles) from another lane by DPP: CTRL selects the source lane,
+// ROW_MASK the 16-lane rows written; the other lanes keep their own value.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void pose_dpp(const double* P, double* Q) {
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+        const long long b = __double_as_longlong(P[q]);
+        const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, ROW_MASK, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, ROW_MASK, 0xF, false);
+        Q[q] = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+}
+// P <- Q P where take holds
+__device__ __forceinline__ void pose_lmul_if(bool take, const double* Q, double* P) {
+    if (take) {
+        double t[12];
+        pose_mul(Q, P, t);
+#pragma unroll
+        for (int q = 0; q < 12; ++q) P[q] = t[q];
+    }
+}
+
 template <bool COHERENT>
-__device__ __forceinline__ void pose_update_block(const ProbDesc& pd, int frame_mode, const Camera& cam,
+__device__ __forceinline__ void pose_update_block(const ProbDesc pd, int frame_mode, const Camera& cam,
                                                   const PoseBufs& B, const Control* ctl, const double* delta,
                                                   double step, int init) {
     __shared__ double wtot[WG / 64][12];   // per wavefront: the product of its chunks
@@ -39,11 +61,20 @@ __device__ __forceinline__ void pose_update_block(const ProbDesc& pd, int frame_
     }
     const int cur = ctl->cur;
     const int bi = cur, bo = init ? cur : cur ^ 1;   // linearisation point, state written
+    // the selected buffers' arrays, chosen by value (an array indexed at run
+    // time would leave the pointers without their global address space and
+    // turn every access into a flat one)
+    const double* const Xin = bi ? B.X[1] : B.X[0];
+    const double* const GiLin = bi ? B.GinvL[1] : B.GinvL[0];
+    double* const Xout = bo ? B.X[1] : B.X[0];
+    double* const Gout = bo ? B.G[1] : B.G[0];
+    double* const GiLout = bo ? B.GinvL[1] : B.GinvL[0];
+    double* const GiRout = bo ? B.GinvR[1] : B.GinvR[0];
     const int okv = COHERENT ? __hip_atomic_load(&ctl->chol_ok, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ctl->chol_ok;
     const bool ok = init ? false : (okv != 0);
     if (one && frame && !init) {
         double Gi[12], x[6];
-        pose_load(B.GinvL[bi] + 12 * (pd.frame_base + tid), Gi);
+        pose_load(GiLin + 12 * (pd.frame_base + tid), Gi);
 #pragma unroll
         for (int q = 0; q < 6; ++q) x[q] = dpre[q];
         adjoint_apply(Gi, x, dpre);
@@ -59,7 +90,7 @@ __device__ __forceinline__ void pose_update_block(const ProbDesc& pd, int frame_
 #pragma unroll
             for (int q = 0; q < 12; ++q) X[q] = cur ? Xb[q] : Xa[q];
         } else {
-            pose_load(B.X[bi] + 12 * (pd.pose_base + k), X);
+            pose_load(Xin + 12 * (pd.pose_base + k), X);
         }
         if (!init) {
             double xi[6], E[12], dk[6];
@@ -68,7 +99,7 @@ __device__ __forceinline__ void pose_update_block(const ProbDesc& pd, int frame_
                 for (int q = 0; q < 6; ++q) dk[q] = dpre[q];
             } else if (frame) {
                 double Gi[12], x[6];
-                pose_load(B.GinvL[bi] + 12 * (pd.frame_base + k), Gi);
+                pose_load(GiLin + 12 * (pd.frame_base + k), Gi);
 #pragma unroll
                 for (int q = 0; q < 6; ++q) x[q] = dl(6 * k + q) - (k + 1 < N ? dl(6 * (k + 1) + q) : 0.0);
                 adjoint_apply(Gi, x, dk);
@@ -83,7 +114,7 @@ __device__ __forceinline__ void pose_update_block(const ProbDesc& pd, int frame_
 #pragma unroll
             for (int q = 0; q < 12; ++q) {
                 X[q] = tmp[q];
-                B.X[bo][12 * (pd.pose_base + k) + q] = tmp[q];
+                Xout[12 * (pd.pose_base + k) + q] = tmp[q];
             }
         }
         pose_inv(X, Xi);
@@ -91,47 +122,69 @@ __device__ __forceinline__ void pose_update_block(const ProbDesc& pd, int frame_
 #pragma unroll
         for (int q = 0; q < 12; ++q) P[q] = tmp[q];
     }
-    // inclusive scan of the chunk products over the workgroup: a Kogge-Stone
-    // scan across each wavefront's lanes by shuffles (no LDS, no barriers), then
-    // the wavefront totals, combined in LDS.  Threads past the last chunk hold
-    // the identity.
+    // inclusive scan of the chunk products over the workgroup: a scan across
+    // each wavefront's lanes by DPP lane moves (no LDS, no barriers), then the
+    // wavefront totals, combined in LDS.  Threads past the last chunk hold the
+    // identity.  Only the wavefronts holding chunks scan, and only the steps
+    // that reach their last chunk.
     const int lane = tid & 63, wv = tid >> 6;
+    const int nw = (nact + 63) >> 6;   // wavefronts holding chunks
+    if (wv < nw) {
+        // within each 16-lane row (row_shr), then lane 15 of a row into the
+        // next row (row_bcast:15, rows 1 and 3) and lane 31 into rows 2 and 3
+        // (row_bcast:31)
+        double Q[12];
+        const int r16 = lane & 15;
+        if (nact > 1) {
+            pose_dpp<0x111, 0xF>(P, Q);
+            pose_lmul_if(r16 >= 1, Q, P);
+        }
+        if (nact > 2) {
+            pose_dpp<0x112, 0xF>(P, Q);
+            pose_lmul_if(r16 >= 2, Q, P);
+        }
+        if (nact > 4) {
+            pose_dpp<0x114, 0xF>(P, Q);
+            pose_lmul_if(r16 >= 4, Q, P);
+        }
+        if (nact > 8) {
+            pose_dpp<0x118, 0xF>(P, Q);
+            pose_lmul_if(r16 >= 8, Q, P);
+        }
+        if (nact > 16) {
+            pose_dpp<0x142, 0xA>(P, Q);
+            pose_lmul_if((lane >> 4) & 1, Q, P);
+        }
+        if (nact > 32) {
+            pose_dpp<0x143, 0xC>(P, Q);
+            pose_lmul_if(lane >= 32, Q, P);
+        }
+        if (lane == 63 && nw > 1) {
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        double Q[12], tmp[12];
-#pragma unroll
-        for (int q = 0; q < 12; ++q) Q[q] = __shfl_up(P[q], d, 64);
-        if (lane >= d) {
-            pose_mul(Q, P, tmp);
-#pragma unroll
-            for (int q = 0; q < 12; ++q) P[q] = tmp[q];
+            for (int q = 0; q < 12; ++q) wtot[wv][q] = P[q];
         }
     }
-    if (lane == 63) {
-#pragma unroll
-        for (int q = 0; q < 12; ++q) wtot[wv][q] = P[q];
-    }
-    __syncthreads();
+    if (nw > 1) __syncthreads();
     // product of the totals of the wavefronts before this one
     double W[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
-    for (int w = 0; w < wv; ++w) {
+    for (int w = 0; w < wv && wv < nw; ++w) {
         double tmp[12];
         pose_mul(W, wtot[w], tmp);
 #pragma unroll
         for (int q = 0; q < 12; ++q) W[q] = tmp[q];
     }
-    if (wv > 0) {
+    if (wv > 0 && wv < nw) {
         double tmp[12];
         pose_mul(W, P, tmp);
 #pragma unroll
         for (int q = 0; q < 12; ++q) P[q] = tmp[q];
     }
-    // exclusive prefix = inclusive scan of the previous thread
+    // exclusive prefix = inclusive scan of the previous thread (wave_shr:1)
     double G[12];
+    pose_dpp<0x138, 0xF>(P, G);
+    if (lane == 0) {
 #pragma unroll
-    for (int q = 0; q < 12; ++q) {
-        const double up = __shfl_up(P[q], 1, 64);
-        G[q] = lane > 0 ? up : W[q];
+        for (int q = 0; q < 12; ++q) G[q] = W[q];
     }
     double Trl[12];
 #pragma unroll
@@ -144,9 +197,9 @@ __device__ __forceinline__ void pose_update_block(const ProbDesc& pd, int frame_
         const long long o = 12LL * (pd.frame_base + f);
 #pragma unroll
         for (int q = 0; q < 12; ++q) {
-            B.G[bo][o + q] = Gf[q];
-            B.GinvL[bo][o + q] = Gi[q];
-            B.GinvR[bo][o + q] = GRi[q];
+            Gout[o + q] = Gf[q];
+            GiLout[o + q] = Gi[q];
+            GiRout[o + q] = GRi[q];
         }
     };
     if (tid == 0) emit(0, G);
@@ -157,7 +210,7 @@ __device__ __forceinline__ void pose_update_block(const ProbDesc& pd, int frame_
     }
     for (int k = k0; k < k1; ++k) {
         double X[12], Xi[12], tmp[12];
-        pose_load(B.X[bo] + 12 * (pd.pose_base + k), X);   // this thread's own stores: program order
+        pose_load(Xout + 12 * (pd.pose_base + k), X);   // this thread's own stores: program order
         pose_inv(X, Xi);
         pose_mul(G, Xi, tmp);
 #pragma unroll
