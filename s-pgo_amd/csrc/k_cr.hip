@@ -253,10 +253,9 @@ constexpr int CR_SPIN_MAX = 1 << 20;
 // wait for a ready flag; past the bound (or once the Cholesky flag is down) the
 // solution is discarded: the flag goes down and nothing waits any longer
 __device__ __forceinline__ void cr_wait(const int* flag, Control* ctl) {
-    volatile int* ok = &ctl->chol_ok;
     int n = 0;
     while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-        if (*ok == 0) break;
+        if (__hip_atomic_load(&ctl->chol_ok, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
         if (++n > CR_SPIN_MAX) {
             __hip_atomic_store(&ctl->chol_ok, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;

@@ -1339,12 +1339,14 @@ __device__ __forceinline__ void xput(double* p, double v) {
 // Once the Cholesky flag is down (a failed pivot, or a wait past its bound)
 // the solution is discarded, and no row waits any longer.
 __device__ __forceinline__ void solve_wait(const int* flag, Control* ctl) {
-    volatile int* ok = &ctl->chol_ok;
+    // the Cholesky flag by device-coherent atomics (global accesses, and another
+    // workgroup's write is seen)
+    int* ok = &ctl->chol_ok;
     int n = 0;
     while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-        if (*ok == 0) break;
+        if (__hip_atomic_load(ok, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
         if (++n > SOLVE_SPIN_MAX) {
-            *ok = 0;
+            __hip_atomic_store(ok, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
         }
         __builtin_amdgcn_s_sleep(1);
