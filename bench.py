@@ -193,6 +193,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-cg", action="store_true", help="c3: skip the CG forms (and the network's training)")
     ap.add_argument("--chunk-landmarks", type=int, default=0)
     ap.add_argument("--group-chunks", type=int, default=0)
     ap.add_argument("--config", default="c3", choices=["c2", "c3", "c4", "c5"],
@@ -420,6 +421,46 @@ def chol_roofline(p, phases, info):
                     "the whole chip's"}
 
 
+PCG_PATH = 4   # SPGO_FACTOR_PCG
+
+
+def c3_network():
+    """The preconditioner network for the 256-keyframe workload: trained on
+    systems the solver generates for problems of the same generator (128-256
+    keyframes, 10 observations per landmark, 5 % outliers, Huber), none of them
+    the benchmark problem; its CG iterations on held-out systems of a further
+    problem beside block Jacobi's."""
+    from spgo.lpcg import LearnedPCG, solver_systems
+    from spgo.problems import make_problem
+    from spgo.solver import make_options
+
+    K = 10
+    W = K - 1
+    opts = make_options(mode="lm", robust="huber", robust_param=2.0)
+    t0 = time.perf_counter()
+    train = [make_problem(n, 390 * n, K, seed=3300 + n, outlier_frac=0.05) for n in (128, 192, 256)]
+    held = [make_problem(256, 390 * 256, K, seed=3700, outlier_frac=0.05)]
+    systems = solver_systems(train, W, states_per_problem=3, iters_between=1, restarts=20, options=opts, seed=31)
+    held_sys = solver_systems(held, W, states_per_problem=3, iters_between=1, restarts=2, options=opts, seed=37)
+    t_data = time.perf_counter() - t0
+    net = LearnedPCG(W, seed=13)
+    net.load(systems)
+    t0 = time.perf_counter()
+    steps = 300
+    losses = net.train(steps, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    net.load(held_sys)
+    it_l = [int(v) for v in net.solve(learned=True, tol=1e-8, max_iter=2000)[0]]
+    it_b = [int(v) for v in net.solve(learned=False, tol=1e-8, max_iter=2000)[0]]
+    log(f"network: {len(systems)} systems in {t_data:.1f} s, {steps} steps in {t_train:.1f} s, loss "
+        f"{losses[0]:.4f} -> {losses[-1]:.4f}; held-out CG iterations learned {it_l}, block Jacobi {it_b}")
+    info = {"systems": len(systems), "problem_keyframes": [q.n_poses for q in train], "steps": steps,
+            "loss_first": float(losses[0]), "loss_last": float(losses[-1]), "data_s": t_data, "train_s": t_train,
+            "held_out_systems": len(held_sys), "held_out_cg_iterations_learned": it_l,
+            "held_out_cg_iterations_block_jacobi": it_b}
+    return net, info
+
+
 def bench_c3(args) -> int:
     """256 keyframes, 100,000 landmarks, 10 observations each, 5% outliers, Huber
     2 px: LM iterations/s of the plain FP64 path (the baseline) and of the
@@ -439,8 +480,17 @@ def bench_c3(args) -> int:
     tile = {"chunk_landmarks": args.chunk_landmarks or 64, "group_chunks": args.group_chunks or 0}
     X0 = p.X0.astype(np.float64)
     a0 = p.alpha0.astype(np.float64)
-    for prec in ("fp64", "mixed"):
-        s = solver_from_problem(p, make_options(mode="lm", robust="huber", robust_param=2.0, precision=prec, **tile))
+    net, cg_training = (None, None) if args.no_cg else c3_network()
+    forms = [("fp64", {"precision": "fp64"}, False), ("mixed", {"precision": "mixed"}, False)]
+    if net is not None:
+        # CG on the reduced system (tolerance 1e-8, Cholesky fallback), block
+        # Jacobi and the trained network
+        forms += [("mixed_cg_block_jacobi", {"precision": "mixed", "factorization": PCG_PATH}, False),
+                  ("mixed_cg_learned", {"precision": "mixed", "factorization": PCG_PATH}, True)]
+    for prec, kw, use_net in forms:
+        s = solver_from_problem(p, make_options(mode="lm", robust="huber", robust_param=2.0, **kw, **tile))
+        if use_net:
+            s.set_preconditioner(net)
         if args.warmup > 0:
             s.step(args.warmup)
         s.set_state(X0, a0)
@@ -455,11 +505,18 @@ def bench_c3(args) -> int:
                      "cost_final": recs[-1]["cost"] if recs else c0,
                      "accepted": int(sum(r["accepted"] for r in recs)),
                      "accuracy": accuracy.summary(X, p.X_true, al, p.alpha_true, np.unique(p.meas_l))}
+        if kw.get("factorization") == PCG_PATH:
+            out[prec]["cg_iterations"] = [r["pcg_iters"] for r in recs]
+            out[prec]["fallbacks"] = int(sum(r["pcg_fallback"] for r in recs))
         s.set_state(X0, a0)
         out[prec]["phases_ms"] = s.profile_phases(min(args.steps, 10))
         out[prec]["layout"] = s.layout_info()
-        log(f"{prec}: {out[prec]['value']:.1f} LM it/s, final cost {out[prec]['cost_final']:.10g}")
+        log(f"{prec}: {out[prec]['value']:.1f} LM it/s, final cost {out[prec]['cost_final']:.10g}"
+            + (f", CG iterations {out[prec]['cg_iterations']}, fallbacks {out[prec]['fallbacks']}"
+               if "cg_iterations" in out[prec] else ""))
         s.close()
+    if net is not None:
+        net.close()
     base = out["fp64"]
     for r in out.values():
         r["final_cost_rel_diff_vs_fp64"] = abs(r["cost_final"] - base["cost_final"]) / base["cost_final"]
@@ -478,7 +535,7 @@ def bench_c3(args) -> int:
                 "traffic": profile_traffic(f"k_assemble_c3_{form}"), "algorithmic_bytes": nbytes}
     roof_asm.update(minimal_bytes_fields(assembly_bytes_minimal(p, real), phases["assemble"]))
     dom = max(phases, key=phases.get)
-    if dom == "cholesky":
+    if dom == "cholesky" and best["layout"]["factorization"] != PCG_PATH:
         roof = chol_roofline(p, phases, best["layout"])
     else:
         roof = dict(roof_asm)
@@ -516,6 +573,7 @@ def bench_c3(args) -> int:
                    "keyframes": p.n_poses, "landmarks": p.n_landmarks, "measurements": p.n_obs,
                    "reduced_system": 6 * p.n_poses},
         "final_cost_rel_diff_vs_fp64": best["final_cost_rel_diff_vs_fp64"],
+        "cg_training": cg_training,
         "forms": out,
         "phases_ms": phases,
         "roofline": roof,
