@@ -164,25 +164,43 @@ constexpr int WG_NODES = 256;          // nodes per workgroup of k_lp_wgrad
 // B[c][k] = W1[k][c], no second input).  C <= 64, I1 + I2 <= 128.  With hrelu
 // also max(Z, 0); with accumulate Z is added to.  Wavefront w computes nodes
 // 16w .. 16w + 15 of the workgroup's 64 against all 64 columns (four 16x16 tiles).
+template <bool TRANS>
 __global__ __launch_bounds__(256) void k_lp_mm(int T, int C, int I1, const float* in1, const float* W1, int I2,
-                                               const float* in2, const float* W2, int trans, const float* bias,
-                                               float* z, float* hrelu, int accumulate) {
+                                               const float* in2, const float* W2, const float* bias, float* z,
+                                               float* hrelu, int accumulate) {
     __shared__ short As[MM_NODES * MM_KP];
     __shared__ short Bs[64 * MM_KP];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int n0 = blockIdx.x * MM_NODES;
     const int K = I1 + I2, KP = (K + 15) & ~15;
-    for (int e = tid; e < 64 * KP; e += 256) {
-        const int c = e / KP, k = e - c * KP;
-        float v = 0.f;
-        if (c < C && k < K) v = trans ? W1[(long long)k * C + c] : (k < I1 ? W1[c * I1 + k] : W2[c * I2 + (k - I1)]);
-        Bs[c * MM_KP + k] = to_bf16(v);
-    }
-    for (int e = tid; e < MM_NODES * KP; e += 256) {
-        const int r = e / KP, k = e - r * KP, n = n0 + r;
-        float v = 0.f;
-        if (n < T && k < K) v = k < I1 ? in1[(long long)n * I1 + k] : in2[(long long)n * I2 + (k - I1)];
-        As[r * MM_KP + k] = to_bf16(v);
+    // both operands staged in batches of eight loads in flight: thread t
+    // stages entries 32 (t & 3) .. + 31 of row t >> 2 of each (a loop with a
+    // run-time trip count waited for each load in turn)
+    const int sr = tid >> 2, kb = 32 * (tid & 3);
+    if (kb < KP) {
+        const int n = n0 + sr;
+        const float* w1 = TRANS ? W1 + sr + (long long)kb * C : W1 + sr * I1 + kb;
+        const float* w2 = TRANS ? W1 : W2 + sr * I2 + kb - I1;
+        const float* a1 = in1 + (long long)n * I1 + kb;
+        const float* a2 = in2 + (long long)n * I2 + kb - I1;
+#pragma unroll 1
+        for (int jb = 0; jb < 32; jb += 8) {
+            float vb[8], va[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = kb + jb + j;
+                vb[j] = 0.f;
+                va[j] = 0.f;
+                if (sr < C && k < K) vb[j] = TRANS ? w1[(long long)(jb + j) * C] : (k < I1 ? w1[jb + j] : w2[jb + j]);
+                if (n < T && k < K) va[j] = k < I1 ? a1[jb + j] : a2[jb + j];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (kb + jb + j < KP) {
+                    Bs[sr * MM_KP + kb + jb + j] = to_bf16(vb[j]);
+                    As[sr * MM_KP + kb + jb + j] = to_bf16(va[j]);
+                }
+        }
     }
     __syncthreads();
     f32x4 acc[4];
@@ -332,38 +350,49 @@ __device__ __forceinline__ void tril_index(int k, int& r, int& c) {
     c = k - r * (r + 1) / 2;
 }
 
-// G_ii = (I + tril(out[0:21])) C_i, G_{i,i-1} = out[21:57] C_{i-1}; out == nullptr: block Jacobi
+// G_ii = (I + tril(out[0:21])) C_i, G_{i,i-1} = out[21:57] C_{i-1}; out == nullptr: block Jacobi.
+// Six threads per node, one row r of both blocks each; row r of the unit lower
+// factor held in registers (compile-time indices), each entry summed over k
+// from its column up, as G_ii(r, c) = sum_{k=c..r} A(r, k) C_i(k, c).
 __global__ void k_lp_buildG(int T, const float* out, const double* C, const int* node_sys, const int* sys_begin,
                             double* Gd, double* Go) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / 6, r = t - 6 * (t / 6);
     if (i >= T) return;
     const int lo = sys_lo(sys_begin, node_sys, i);
-    double A[36];
-    for (int k = 0; k < 36; ++k) A[k] = (k % 7 == 0) ? 1.0 : 0.0;
-    if (out) {
-        for (int k = 0; k < 21; ++k) {
-            int r, c;
-            tril_index(k, r, c);
-            A[r * 6 + c] += (double)out[(long long)i * LP_NOUT + k];
-        }
+    const float* oi = out ? out + (long long)i * LP_NOUT : nullptr;
+    double A[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        A[k] = k == r ? 1.0 : 0.0;
+        if (oi && k <= r) A[k] += (double)oi[r * (r + 1) / 2 + k];
     }
     const double* Ci = C + (long long)i * 36;
-    for (int r = 0; r < 6; ++r)
+    double* gd = Gd + (long long)i * 36 + 6 * r;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        double a = 0.0;
+#pragma unroll
+        for (int k = c; k < 6; ++k)
+            if (k <= r) a += A[k] * Ci[k * 6 + c];
+        gd[c] = a;
+    }
+    double* go = Go + (long long)i * 36 + 6 * r;
+    if (oi && i > lo) {
+        const double* Cj = C + (long long)(i - 1) * 36;
+        double B[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) B[k] = (double)oi[21 + r * 6 + k];
+#pragma unroll
         for (int c = 0; c < 6; ++c) {
             double a = 0.0;
-            for (int k = c; k <= r; ++k) a += A[r * 6 + k] * Ci[k * 6 + c];
-            Gd[(long long)i * 36 + r * 6 + c] = a;
+#pragma unroll
+            for (int k = c; k < 6; ++k) a += B[k] * Cj[k * 6 + c];
+            go[c] = a;
         }
-    if (out && i > lo) {
-        const double* Cj = C + (long long)(i - 1) * 36;
-        for (int r = 0; r < 6; ++r)
-            for (int c = 0; c < 6; ++c) {
-                double a = 0.0;
-                for (int k = c; k < 6; ++k) a += (double)out[(long long)i * LP_NOUT + 21 + r * 6 + k] * Cj[k * 6 + c];
-                Go[(long long)i * 36 + r * 6 + c] = a;
-            }
     } else {
-        for (int k = 0; k < 36; ++k) Go[(long long)i * 36 + k] = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) go[c] = 0.0;
     }
 }
 
@@ -860,15 +889,14 @@ void lp_launch_features(const LpSet& s, double* C, double* sfeat, float* X, doub
 
 void lp_launch_linear(int T, int O, int I1, const float* in1, const float* W1, int I2, const float* in2,
                       const float* W2, const float* b, float* z, float* h, hipStream_t st) {
-    hipLaunchKernelGGL(k_lp_mm, dim3(nblk(T, MM_NODES)), dim3(256), 0, st, T, O, I1, in1, W1, I2, in2, W2, 0, b, z,
-                       h, 0);
+    hipLaunchKernelGGL(k_lp_mm<false>, dim3(nblk(T, MM_NODES)), dim3(256), 0, st, T, O, I1, in1, W1, I2, in2, W2, b,
+                       z, h, 0);
 }
 
 void lp_launch_linearT(int T, int O, int I, const float* dy, const float* Wt, float* dx, int acc, hipStream_t st) {
     // dx (T x I) = dy (T x O) W (O x I): B[c][k] = W[k][c]
-    hipLaunchKernelGGL(k_lp_mm, dim3(nblk(T, MM_NODES)), dim3(256), 0, st, T, I, O, dy, Wt, 0,
-                       (const float*)nullptr, (const float*)nullptr, 1, (const float*)nullptr, dx, (float*)nullptr,
-                       acc);
+    hipLaunchKernelGGL(k_lp_mm<true>, dim3(nblk(T, MM_NODES)), dim3(256), 0, st, T, I, O, dy, Wt, 0,
+                       (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, dx, (float*)nullptr, acc);
 }
 
 int lp_wgrad_blocks(int T) { return (T + WG_NODES - 1) / WG_NODES; }
@@ -896,7 +924,7 @@ void lp_launch_reluback(int n, const float* dh, const float* z, float* dz, hipSt
 }
 
 void lp_launch_buildG(const LpSet& s, const float* out, const double* C, double* Gd, double* Go, hipStream_t st) {
-    hipLaunchKernelGGL(k_lp_buildG, dim3(nblk(s.T)), dim3(256), 0, st, s.T, out, C, s.node_sys, s.sys_begin, Gd, Go);
+    hipLaunchKernelGGL(k_lp_buildG, dim3(nblk(6 * s.T)), dim3(256), 0, st, s.T, out, C, s.node_sys, s.sys_begin, Gd, Go);
 }
 
 void lp_launch_kap(const LpSet& s, const float* out, const double* N1, double* tnode, double* ldnode, double* Tsys,
